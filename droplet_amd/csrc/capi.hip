@@ -1,0 +1,322 @@
+// C ABI of libdroplet_hip.so: device context, errors, buffers, renderer and
+// scene handles, the standalone stable sort and the numerics self-check.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "dl_common.h"
+#include "dl_internal.h"
+#include "render.h"
+
+struct dl_context {
+  dl::Context c;
+};
+struct dl_renderer {
+  dl::Renderer r;
+};
+struct dl_scene {
+  dl::Scene s;
+};
+
+namespace dl {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? DL_ERR_OOM : DL_ERR_DEVICE;
+  fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DeviceBuffer::~DeviceBuffer() { release(); }
+
+void DeviceBuffer::release() {
+  if (ptr_) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  cap_ = 0;
+}
+
+void DeviceBuffer::reserve(size_t bytes) {
+  if (bytes <= cap_) return;
+  release();
+  // grow by 1/8 headroom so frames of slightly different size reuse memory
+  size_t want = bytes + bytes / 8;
+  want = (want + 255) & ~(size_t)255;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    want = (bytes + 255) & ~(size_t)255;
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(DL_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  ptr_ = p;
+  cap_ = want;
+}
+
+void Context::make_current() const { DL_HIP(hipSetDevice(device)); }
+
+static void upload_array(DeviceBuffer &b, const float *src, size_t count, const char *name, hipStream_t s) {
+  if (count == 0) {
+    b.reserve(256);
+    return;
+  }
+  if (!src) fail(DL_ERR_INVALID, std::string(name) + " is null");
+  b.reserve(count * sizeof(float));
+  DL_HIP(hipMemcpyAsync(b.get(), src, count * sizeof(float), hipMemcpyHostToDevice, s));
+}
+
+void Scene::upload(const dl_splats &sp) {
+  if (sp.sh_bands > 3) fail(DL_ERR_INVALID, "shBands must be 0..3");
+  const size_t n = sp.count;
+  const uint32_t M = sp.sh_bands == 0 ? 0 : sp.sh_bands == 1 ? 3 : sp.sh_bands == 2 ? 8 : 15;
+  ctx->make_current();
+  hipStream_t s = ctx->stream;
+  upload_array(pos, sp.positions, 3 * n, "positions", s);
+  upload_array(rot, sp.rotations, 4 * n, "rotations", s);
+  upload_array(ls, sp.log_scales, 3 * n, "logScales", s);
+  upload_array(op, sp.opacity_logits, n, "opacityLogits", s);
+  upload_array(dc, sp.colors_dc, 3 * n, "colorsDC", s);
+  upload_array(sh, sp.colors_sh, 3 * (size_t)M * n, "colorsSH", s);
+  DL_HIP(hipStreamSynchronize(s));
+  count = sp.count;
+  bands = sp.sh_bands;
+}
+
+SceneView Scene::view() const {
+  return SceneView{pos.as<const float>(), rot.as<const float>(), ls.as<const float>(), op.as<const float>(),
+                   dc.as<const float>(), sh.as<const float>(), count, bands};
+}
+
+// ---------------------------------------------------------------------------
+// numerics self-check kernel
+// ---------------------------------------------------------------------------
+__global__ void k_debug_math(int op, const float *a, const float *b, float *out, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float r = 0.0f;
+  switch (op) {
+    case 0: r = dl_expf(a[i]); break;
+    case 1: r = __builtin_sqrtf(a[i]); break;
+    case 2: r = a[i] / b[i]; break;
+    case 3: r = bits_f32(sortable_key((float)((double)a[i] * (double)b[i]))); break;
+    case 4: r = dl_sigmoidf(a[i]); break;
+    default: break;
+  }
+  out[i] = r;
+}
+
+}  // namespace dl
+
+using namespace dl;
+
+extern "C" {
+
+const char *dl_last_error(void) { return g_last_error.c_str(); }
+
+int dl_context_create(const dl_device_opts *opts, dl_context **out) {
+  return guard([&] {
+    if (!out) fail(DL_ERR_INVALID, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+      (void)hipGetLastError();
+      fail(DL_ERR_DEVICE, "No HIP device available");
+    }
+    const int dev = opts ? opts->device_index : 0;
+    if (dev < 0 || dev >= count) fail(DL_ERR_INVALID, "device_index out of range");
+    std::unique_ptr<dl_context> c(new dl_context());
+    c->c.device = dev;
+    DL_HIP(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    DL_HIP(hipGetDeviceProperties(&prop, dev));
+    c->c.num_cus = prop.multiProcessorCount;
+    c->c.arch = prop.gcnArchName;
+    DL_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    *out = c.release();
+  });
+}
+
+void dl_context_destroy(dl_context *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.sort_keys.release();
+  ctx->c.sort_vals.release();
+  ctx->c.sort_tmp_keys.release();
+  ctx->c.sort_tmp_vals.release();
+  ctx->c.sort_ws.release();
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+int dl_context_device_name(dl_context *ctx, char *buf, size_t cap) {
+  return guard([&] {
+    if (!ctx || !buf || cap == 0) fail(DL_ERR_INVALID, "null argument");
+    std::strncpy(buf, ctx->c.arch.c_str(), cap - 1);
+    buf[cap - 1] = 0;
+  });
+}
+
+int dl_renderer_create(dl_context *ctx, const dl_renderer_opts *opts, dl_renderer **out) {
+  return guard([&] {
+    if (!ctx || !out) fail(DL_ERR_INVALID, "null argument");
+    std::unique_ptr<dl_renderer> r(new dl_renderer());
+    r->r.ctx = &ctx->c;
+    if (opts) r->r.opts = *opts;
+    r->r.max_cov = (opts && opts->max_coverage_per_splat) ? opts->max_coverage_per_splat : kMaxCoverageDefault;
+    r->r.own_scene.ctx = &ctx->c;
+    *out = r.release();
+  });
+}
+
+void dl_renderer_destroy(dl_renderer *r) {
+  if (!r) return;
+  (void)hipSetDevice(r->r.ctx->device);
+  (void)hipStreamSynchronize(r->r.ctx->stream);
+  delete r;
+}
+
+static dl_background bg_or_default(const dl_background *bg) {
+  return bg ? *bg : dl_background{0.0f, 0.0f, 0.0f, 1.0f};
+}
+
+int dl_render(dl_renderer *r, const dl_splats *splats, const dl_camera *cam, const dl_background *bg,
+              uint8_t *out_rgba) {
+  return guard([&] {
+    if (!r || !splats || !cam || !out_rgba) fail(DL_ERR_INVALID, "null argument");
+    r->r.own_scene.upload(*splats);
+    r->r.render(r->r.own_scene.view(), *cam, bg_or_default(bg));
+    r->r.read_image(out_rgba);
+  });
+}
+
+int dl_scene_create(dl_context *ctx, const dl_splats *splats, dl_scene **out) {
+  return guard([&] {
+    if (!ctx || !splats || !out) fail(DL_ERR_INVALID, "null argument");
+    std::unique_ptr<dl_scene> sc(new dl_scene());
+    sc->s.ctx = &ctx->c;
+    sc->s.upload(*splats);
+    *out = sc.release();
+  });
+}
+
+void dl_scene_destroy(dl_scene *scene) {
+  if (!scene) return;
+  (void)hipSetDevice(scene->s.ctx->device);
+  (void)hipStreamSynchronize(scene->s.ctx->stream);
+  delete scene;
+}
+
+int dl_render_scene(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, const dl_background *bg,
+                    uint8_t *out_rgba) {
+  return guard([&] {
+    if (!r || !scene || !cam) fail(DL_ERR_INVALID, "null argument");
+    if (scene->s.ctx != r->r.ctx) fail(DL_ERR_INVALID, "scene and renderer belong to different contexts");
+    r->r.render(scene->s.view(), *cam, bg_or_default(bg));
+    if (out_rgba) r->r.read_image(out_rgba);
+  });
+}
+
+int dl_renderer_read_image(dl_renderer *r, uint8_t *out_rgba) {
+  return guard([&] {
+    if (!r || !out_rgba) fail(DL_ERR_INVALID, "null argument");
+    r->r.read_image(out_rgba);
+  });
+}
+
+int dl_renderer_set_timing(dl_renderer *r, int enabled) {
+  return guard([&] {
+    if (!r) fail(DL_ERR_INVALID, "null renderer");
+    r->r.timing = enabled != 0;
+  });
+}
+
+int dl_renderer_stats(dl_renderer *r, dl_render_stats *out) {
+  return guard([&] {
+    if (!r || !out) fail(DL_ERR_INVALID, "null argument");
+    *out = r->r.stats;
+  });
+}
+
+int dl_renderer_set_taps(dl_renderer *r, int enabled) {
+  return guard([&] {
+    if (!r) fail(DL_ERR_INVALID, "null renderer");
+    r->r.keep_taps = enabled != 0;
+  });
+}
+
+int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *written) {
+  return guard([&] {
+    if (!r) fail(DL_ERR_INVALID, "null renderer");
+    if (which < 0 || which >= (int)r->r.taps.size()) fail(DL_ERR_INVALID, "unknown tap");
+    const auto &t = r->r.taps[which];
+    if (written) *written = t.size();
+    if (!dst) return;
+    if (bytes < t.size()) fail(DL_ERR_CAPACITY, "tap buffer too small");
+    if (!t.empty()) std::memcpy(dst, t.data(), t.size());
+  });
+}
+
+int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t key_bits) {
+  return guard([&] {
+    if (!ctx) fail(DL_ERR_INVALID, "null context");
+    if (n == 0) return;
+    if (!keys || !vals) fail(DL_ERR_INVALID, "null keys/vals");
+    if (key_bits == 0 || key_bits > 32) fail(DL_ERR_INVALID, "key_bits must be 1..32");
+    Context &c = ctx->c;
+    c.make_current();
+    hipStream_t s = c.stream;
+    const size_t bytes = (size_t)n * 4;
+    c.sort_keys.reserve(bytes);
+    c.sort_vals.reserve(bytes);
+    c.sort_tmp_keys.reserve(bytes);
+    c.sort_tmp_vals.reserve(bytes);
+    c.sort_ws.reserve(radix_workspace_bytes(n, (int)key_bits));
+    DL_HIP(hipMemcpyAsync(c.sort_keys.get(), keys, bytes, hipMemcpyHostToDevice, s));
+    DL_HIP(hipMemcpyAsync(c.sort_vals.get(), vals, bytes, hipMemcpyHostToDevice, s));
+    const bool in_tmp = radix_sort_pairs(s, c.sort_keys.as<uint32_t>(), c.sort_vals.as<uint32_t>(),
+                                         c.sort_tmp_keys.as<uint32_t>(), c.sort_tmp_vals.as<uint32_t>(), n,
+                                         (int)key_bits, c.sort_ws.get());
+    DL_HIP(hipMemcpyAsync(keys, in_tmp ? c.sort_tmp_keys.get() : c.sort_keys.get(), bytes, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(vals, in_tmp ? c.sort_tmp_vals.get() : c.sort_vals.get(), bytes, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dl_debug_math(dl_context *ctx, int op, const float *a, const float *b, float *out, uint32_t n) {
+  return guard([&] {
+    if (!ctx || !a || !out) fail(DL_ERR_INVALID, "null argument");
+    if ((op == 2 || op == 3) && !b) fail(DL_ERR_INVALID, "op needs b");
+    if (op < 0 || op > 4) fail(DL_ERR_INVALID, "unknown op");
+    if (n == 0) return;
+    Context &c = ctx->c;
+    c.make_current();
+    hipStream_t s = c.stream;
+    DeviceBuffer da, db, dout;
+    da.reserve((size_t)n * 4);
+    dout.reserve((size_t)n * 4);
+    DL_HIP(hipMemcpyAsync(da.get(), a, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    if (b) {
+      db.reserve((size_t)n * 4);
+      DL_HIP(hipMemcpyAsync(db.get(), b, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    }
+    k_debug_math<<<div_up(n, 256), 256, 0, s>>>(op, da.as<float>(), db.as<float>(), dout.as<float>(), n);
+    DL_HIP(hipGetLastError());
+    DL_HIP(hipMemcpyAsync(out, dout.get(), (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Internal declarations shared by the translation units of libdroplet_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/droplet_hip.h"
+
+namespace dl {
+
+// ---------------------------------------------------------------------------
+// errors: every entry point converts failures into a status code plus a
+// thread-local message (dl_last_error).
+// ---------------------------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_last_error(const std::string &msg);
+[[noreturn]] void fail(int code, const std::string &msg);
+void check_hip(hipError_t e, const char *what);
+
+#define DL_HIP(call) ::dl::check_hip((call), #call)
+
+// Runs f() and maps exceptions to status codes.
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return DL_OK;
+  } catch (const Error &e) {
+    set_last_error(e.msg);
+    return e.code;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return DL_ERR_INTERNAL;
+  } catch (...) {
+    set_last_error("unknown error");
+    return DL_ERR_INTERNAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device memory: a grow-only buffer owned by the library
+// ---------------------------------------------------------------------------
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  // ensure at least `bytes` of capacity; contents are not preserved on growth
+  void reserve(size_t bytes);
+  void release();
+  void *get() const { return ptr_; }
+  template <class T>
+  T *as() const { return static_cast<T *>(ptr_); }
+  size_t capacity() const { return cap_; }
+
+ private:
+  void *ptr_ = nullptr;
+  size_t cap_ = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  std::string arch;
+  // scratch for the standalone sort entry point
+  DeviceBuffer sort_keys, sort_vals, sort_tmp_keys, sort_tmp_vals, sort_ws;
+  void make_current() const;
+};
+
+// ---------------------------------------------------------------------------
+// device-wide primitives (scan_sort.hip)
+// ---------------------------------------------------------------------------
+// Exclusive scan of n u32 values; writes the total to *d_total (device) when
+// d_total is not null. `ws` must hold scan_workspace_bytes(n).
+size_t scan_workspace_bytes(size_t n);
+void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n,
+                        uint32_t *d_total, void *ws);
+
+// Stable LSD radix sort of (key, value) pairs on bits [0, key_bits) of the
+// key. Results end in either (keys, vals) or (tmp_keys, tmp_vals); the
+// return value is true when they are in the tmp buffers.
+size_t radix_workspace_bytes(size_t n, int key_bits);
+bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
+                      uint32_t *tmp_vals, size_t n, int key_bits, void *ws);
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace dl

@@ -1,0 +1,76 @@
+// Renderer state and kernel parameter blocks (host + device visible).
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "dl_internal.h"
+
+namespace dl {
+
+// Device-resident SplatData (src/types.ts:23-32), same array layouts.
+struct SceneView {
+  const float *pos;
+  const float *rot;
+  const float *ls;
+  const float *op;
+  const float *dc;
+  const float *sh;
+  uint32_t count;
+  uint32_t bands;
+};
+
+// Project uniforms (pipeline.ts:475-506): every value as the f32 the reference
+// writes into its uniform buffer.
+struct ProjUniforms {
+  float view[16];
+  float width, height, focal, cx, cy, near_, far_;
+  uint32_t tiles_x, tiles_y, max_cov, proj;
+};
+
+struct RastParams {
+  uint32_t tiles_x, width, height;
+  uint32_t has_pairs, resume, finalize, write_accum;
+  float bg_r, bg_g, bg_b, bg_a;
+};
+
+struct Scene {
+  Context *ctx = nullptr;
+  uint32_t count = 0, bands = 0;
+  DeviceBuffer pos, rot, ls, op, dc, sh;
+  void upload(const dl_splats &s);
+  SceneView view() const;
+};
+
+struct Renderer {
+  Context *ctx = nullptr;
+  dl_renderer_opts opts{};
+  uint32_t max_cov = 16384;
+  Scene own_scene;
+  // depth sort
+  DeviceBuffer keys, ids, tkeys, tids, sort_ws;
+  // per chunk
+  DeviceBuffer rec, counts, offsets, scan_ws, ranges;
+  // pairs
+  DeviceBuffer pkeys, pvals, ptkeys, ptvals, psort_ws;
+  // image
+  DeviceBuffer accum, rgba, small;
+  uint32_t width = 0, height = 0;
+  dl_render_stats stats{};
+  bool timing = false;
+  bool keep_taps = false;
+  std::array<std::vector<uint8_t>, 8> taps;
+  std::vector<hipEvent_t> events;
+  std::vector<int> ev_stage;
+  size_t ev_used = 0;
+
+  ~Renderer();
+  void render(const SceneView &sc, const dl_camera &cam, const dl_background &bg);
+  void read_image(uint8_t *out);
+
+ private:
+  void mark(int stage);
+  void ensure_events(size_t n);
+};
+
+}  // namespace dl

@@ -1,0 +1,674 @@
+// Forward splat renderer on gfx950: the device path behind dl_render.
+//
+// Stages (reference citations are relative to the droplet repository):
+//   k_cull_key   f64 near/far (or radius) cull + sortable f32 depth key
+//                 (src/render/camera.ts:56-99, src/spatial/radix-sort.ts:20-24)
+//   depth sort   stable LSD radix over (key, splat id)  (radix-sort.ts:65-117)
+//   k_project    EWA projection, SH colour, tile AABB count (project.ts:73-383)
+//   scan         exclusive scan of counts (prefix-sum.ts:22-86)
+//   k_emit       (tile, splat) pairs in row-major tile order (emit-pairs.ts:27-56)
+//   pair sort    stable LSD radix by tile id (pipeline.ts:544-578)
+//   k_ranges     per-tile [start, end) (find-boundaries.ts:17-99)
+//   k_rasterize  front-to-back compositing + finalize (rasterize.ts:36-82,
+//                 finalize.ts:29-42)
+// All sizes stay on the device except the visible count and the pair count,
+// which the host reads to size launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dl_common.h"
+#include "dl_internal.h"
+#include "render.h"
+
+namespace dl {
+
+namespace {
+
+constexpr int kCullBlock = 256;
+constexpr int kProjBlock = 128;
+
+struct ViewF32 {
+  float m[16];
+};
+
+// ---------------------------------------------------------------------------
+// cull + depth key (camera.ts:56-99): f64, no contraction, JS evaluation order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kCullBlock) void k_cull_key(const float *__restrict__ pos, uint32_t n, ViewF32 v,
+                                                         double near_plane, double far_plane, uint32_t proj,
+                                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
+                                                         uint32_t *__restrict__ d_nvis) {
+  const uint32_t i = blockIdx.x * kCullBlock + threadIdx.x;
+  bool vis = false;
+  if (i < n) {
+    const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
+    float depth_f = 0.0f;
+    if (proj == DL_PINHOLE) {
+      double cz = (double)v.m[2] * x + (double)v.m[6] * y;
+      cz = cz + (double)v.m[10] * z;
+      cz = cz + (double)v.m[14];
+      const double depth = -cz;
+      vis = depth > near_plane && depth < far_plane;
+      depth_f = (float)depth;
+    } else {
+      double cx = (double)v.m[0] * x + (double)v.m[4] * y;
+      cx = cx + (double)v.m[8] * z;
+      cx = cx + (double)v.m[12];
+      double cy = (double)v.m[1] * x + (double)v.m[5] * y;
+      cy = cy + (double)v.m[9] * z;
+      cy = cy + (double)v.m[13];
+      double cz = (double)v.m[2] * x + (double)v.m[6] * y;
+      cz = cz + (double)v.m[10] * z;
+      cz = cz + (double)v.m[14];
+      double r2 = cx * cx + cy * cy;
+      r2 = r2 + cz * cz;
+      vis = r2 > near_plane * near_plane;
+      depth_f = (float)__builtin_sqrt(r2);
+    }
+    keys[i] = vis ? sortable_key(depth_f) : kInvisibleKey;
+    ids[i] = i;
+  }
+  const uint64_t bal = __ballot(vis);
+  if (__lane_id() == 0 && bal) atomicAdd(d_nvis, (uint32_t)__popcll(bal));
+}
+
+// ---------------------------------------------------------------------------
+// project (project.ts:73-383) for splat id = sorted[j]
+// ---------------------------------------------------------------------------
+__device__ inline void quat_rot(const float q[4], float v0, float v1, float v2, float o[3]) {
+  const float c0 = q[2] * v2 - q[3] * v1;
+  const float c1 = q[3] * v0 - q[1] * v2;
+  const float c2 = q[1] * v1 - q[2] * v0;
+  const float t0 = 2.0f * c0, t1 = 2.0f * c1, t2 = 2.0f * c2;
+  const float d0 = q[2] * t2 - q[3] * t1;
+  const float d1 = q[3] * t0 - q[1] * t2;
+  const float d2 = q[1] * t1 - q[2] * t0;
+  o[0] = (v0 + q[0] * t0) + d0;
+  o[1] = (v1 + q[0] * t1) + d1;
+  o[2] = (v2 + q[0] * t2) + d2;
+}
+
+template <int BANDS>
+__device__ inline void eval_sh(const float *__restrict__ sh, float x, float y, float z, float out[3]) {
+  constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+  float c[3 * M];
+#pragma unroll
+  for (int k = 0; k < 3 * M; k++) c[k] = sh[k];
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  const float SH1 = (float)0.4886025119029199;
+  const float e0 = (-SH1) * y, e1 = SH1 * z, e2 = (-SH1) * x;
+  r = r + e0 * c[0]; r = r + e1 * c[1]; r = r + e2 * c[2];
+  g = g + e0 * c[M + 0]; g = g + e1 * c[M + 1]; g = g + e2 * c[M + 2];
+  b = b + e0 * c[2 * M + 0]; b = b + e1 * c[2 * M + 1]; b = b + e2 * c[2 * M + 2];
+  if constexpr (BANDS >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    float b2[5];
+    b2[0] = (float)1.0925484305920792 * xy;
+    b2[1] = (float)-1.0925484305920792 * yz;
+    b2[2] = (float)0.31539156525252005 * ((2.0f * zz - xx) - yy);
+    b2[3] = (float)-1.0925484305920792 * xz;
+    b2[4] = (float)0.5462742152960396 * (xx - yy);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      r = r + b2[k] * c[3 + k];
+      g = g + b2[k] * c[M + 3 + k];
+      b = b + b2[k] * c[2 * M + 3 + k];
+    }
+    if constexpr (BANDS >= 3) {
+      float b3[7];
+      b3[0] = ((float)-0.5900435899266435 * y) * (3.0f * xx - yy);
+      b3[1] = ((float)2.890611442640554 * xy) * z;
+      b3[2] = ((float)-0.4570457994644658 * y) * ((4.0f * zz - xx) - yy);
+      b3[3] = ((float)0.3731763325901154 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+      b3[4] = ((float)-0.4570457994644658 * x) * ((4.0f * zz - xx) - yy);
+      b3[5] = ((float)1.445305721320277 * z) * (xx - yy);
+      b3[6] = ((float)-0.5900435899266435 * x) * (xx - 3.0f * yy);
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        r = r + b3[k] * c[8 + k];
+        g = g + b3[k] * c[M + 8 + k];
+        b = b + b3[k] * c[2 * M + 8 + k];
+      }
+    }
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = b;
+}
+
+template <int BANDS>
+__global__ __launch_bounds__(kProjBlock) void k_project(SceneView sc, const uint32_t *__restrict__ sorted,
+                                                        uint32_t j0, uint32_t cn, ProjUniforms u,
+                                                        float4 *__restrict__ rec, uint32_t *__restrict__ counts) {
+  const uint32_t jj = blockIdx.x * kProjBlock + threadIdx.x;
+  if (jj >= cn) return;
+  const uint32_t id = sorted[j0 + jj];
+  const float *m = u.view;
+  const float px = sc.pos[3 * (size_t)id], py = sc.pos[3 * (size_t)id + 1], pz = sc.pos[3 * (size_t)id + 2];
+  const float4 qv = reinterpret_cast<const float4 *>(sc.rot)[id];
+  const float q[4] = {qv.x, qv.y, qv.z, qv.w};
+  const float s0 = dl_expf(sc.ls[3 * (size_t)id]);
+  const float s1 = dl_expf(sc.ls[3 * (size_t)id + 1]);
+  const float s2 = dl_expf(sc.ls[3 * (size_t)id + 2]);
+  const float logit = sc.op[id];
+
+  const float pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
+  const float pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
+  const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
+  const float depth = -pcz;
+  float sx, sy;
+  bool culled = false;
+  if (u.proj == DL_PINHOLE) {
+    culled = depth <= u.near_ || depth >= u.far_;
+    const float proj_x = (pcx * u.focal) / (-pcz);
+    const float proj_y = ((-pcy) * u.focal) / (-pcz);
+    sx = proj_x + u.cx;
+    sy = proj_y + u.cy;
+  } else {
+    const float rr = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+    culled = rr <= u.near_;
+    const float lon = (float)atan2((double)pcx, (double)(-pcz));
+    const float lat = (float)asin((double)(pcy / rr));
+    sx = (lon / kTwoPi + 0.5f) * u.width;
+    sy = (0.5f - lat / kPi) * u.height;
+  }
+  float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+  float4 o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t count = 0;
+  if (!culled) {
+    float rx[3], ry[3], rz[3];
+    quat_rot(q, 1.0f, 0.0f, 0.0f, rx);
+    quat_rot(q, 0.0f, 1.0f, 0.0f, ry);
+    quat_rot(q, 0.0f, 0.0f, 1.0f, rz);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      rx[k] = rx[k] * s0;
+      ry[k] = ry[k] * s1;
+      rz[k] = rz[k] * s2;
+    }
+    const float c00 = (rx[0] * rx[0] + ry[0] * ry[0]) + rz[0] * rz[0];
+    const float c01 = (rx[0] * rx[1] + ry[0] * ry[1]) + rz[0] * rz[1];
+    const float c02 = (rx[0] * rx[2] + ry[0] * ry[2]) + rz[0] * rz[2];
+    const float c11 = (rx[1] * rx[1] + ry[1] * ry[1]) + rz[1] * rz[1];
+    const float c12 = (rx[1] * rx[2] + ry[1] * ry[2]) + rz[1] * rz[2];
+    const float c22 = (rx[2] * rx[2] + ry[2] * ry[2]) + rz[2] * rz[2];
+    const float v00 = m[0], v01 = m[4], v02 = m[8];
+    const float v10 = m[1], v11 = m[5], v12 = m[9];
+    const float v20 = m[2], v21 = m[6], v22 = m[10];
+    const float a00 = (v00 * c00 + v01 * c01) + v02 * c02;
+    const float a01 = (v00 * c01 + v01 * c11) + v02 * c12;
+    const float a02 = (v00 * c02 + v01 * c12) + v02 * c22;
+    const float a10 = (v10 * c00 + v11 * c01) + v12 * c02;
+    const float a11 = (v10 * c01 + v11 * c11) + v12 * c12;
+    const float a12 = (v10 * c02 + v11 * c12) + v12 * c22;
+    const float a20 = (v20 * c00 + v21 * c01) + v22 * c02;
+    const float a21 = (v20 * c01 + v21 * c11) + v22 * c12;
+    const float a22 = (v20 * c02 + v21 * c12) + v22 * c22;
+    const float cc00 = (a00 * v00 + a01 * v01) + a02 * v02;
+    const float cc01 = (a00 * v10 + a01 * v11) + a02 * v12;
+    const float cc02 = (a00 * v20 + a01 * v21) + a02 * v22;
+    const float cc11 = (a10 * v10 + a11 * v11) + a12 * v12;
+    const float cc12 = (a10 * v20 + a11 * v21) + a12 * v22;
+    const float cc22 = (a20 * v20 + a21 * v21) + a22 * v22;
+    const float inv_z = 1.0f / pcz;
+    const float j00 = u.focal * inv_z;
+    const float j02 = (((-u.focal) * pcx) * inv_z) * inv_z;
+    const float j11 = u.focal * inv_z;
+    const float j12 = (((-u.focal) * pcy) * inv_z) * inv_z;
+    const float b00 = j00 * cc00 + j02 * cc02;
+    const float b01 = j00 * cc01 + j02 * cc12;
+    const float b02 = j00 * cc02 + j02 * cc22;
+    const float b11 = j11 * cc11 + j12 * cc12;
+    const float b12 = j11 * cc12 + j12 * cc22;
+    float cov00 = b00 * j00 + b02 * j02;
+    const float cov01 = b01 * j11 + b02 * j12;
+    float cov11 = b11 * j11 + b12 * j12;
+    const float det_orig = cov00 * cov11 - cov01 * cov01;
+    cov00 = cov00 + kAA;
+    cov11 = cov11 + kAA;
+    const float det = cov00 * cov11 - cov01 * cov01;
+    const float aa = __builtin_sqrtf(wmax(det_orig / det, 0.0f));
+    if (det <= kDetMin) {
+      culled = true;
+    } else {
+      const float inv_det = 1.0f / det;
+      const float conic_a = cov11 * inv_det;
+      const float conic_b = (-cov01) * inv_det;
+      const float conic_c = cov00 * inv_det;
+      const float mid = 0.5f * (cov00 + cov11);
+      const float r_eig = __builtin_sqrtf(wmax(kEigMin, mid * mid - det));
+      const float lambda1 = mid + r_eig;
+      const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
+      const float fade_s = kFadeStart * u.height;
+      const float fade_e = kFadeEnd * u.height;
+      const float fade = wclamp((fade_e - radius_raw) / (fade_e - fade_s), 0.0f, 1.0f);
+      if (fade <= 0.0f) {
+        culled = true;
+      } else {
+        const float radius = __builtin_ceilf(wmin(radius_raw, fade_e));
+        float col[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
+        if constexpr (BANDS > 0) {
+          constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+          const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+          float sh[3];
+          eval_sh<BANDS>(sc.sh + (size_t)id * 3 * M, pcx / len, pcy / len, pcz / len, sh);
+          col[0] = col[0] + sh[0];
+          col[1] = col[1] + sh[1];
+          col[2] = col[2] + sh[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
+        const float alpha = (dl_sigmoidf(logit) * fade) * aa;
+        o0 = make_float4(sx, sy, radius, depth);
+        o1 = make_float4(conic_a, conic_b, conic_c, alpha);
+        o2 = make_float4(col[0], col[1], col[2], 0.0f);
+        const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
+        const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
+        const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
+        const uint32_t max_ty = u32sat(wmin((float)u.tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
+        if (max_tx >= min_tx && max_ty >= min_ty) {
+          count = (max_tx - min_tx + 1u) * (max_ty - min_ty + 1u);
+          if (count > u.max_cov) count = u.max_cov;
+        }
+      }
+    }
+  }
+  if (culled) {
+    o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+    o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    count = 0;
+  }
+  rec[3 * (size_t)jj + 0] = o0;
+  rec[3 * (size_t)jj + 1] = o1;
+  rec[3 * (size_t)jj + 2] = o2;
+  counts[jj] = count;
+}
+
+// ---------------------------------------------------------------------------
+// emit (emit-pairs.ts:27-56)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, const uint32_t *__restrict__ offs,
+                                              uint32_t cn, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov,
+                                              uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
+  const uint32_t jj = blockIdx.x * 128 + threadIdx.x;
+  if (jj >= cn) return;
+  const float4 v0 = rec[3 * (size_t)jj];
+  const float sx = v0.x, sy = v0.y, radius = v0.z;
+  if (radius <= 0.0f) return;
+  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
+  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
+  const uint32_t max_tx = u32sat(wmin((float)tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
+  const uint32_t max_ty = u32sat(wmin((float)tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
+  if (max_tx < min_tx || max_ty < min_ty) return;
+  uint32_t off = offs[jj];
+  uint32_t written = 0;
+  for (uint32_t ty = min_ty; ty <= max_ty; ty++) {
+    for (uint32_t tx = min_tx; tx <= max_tx; tx++) {
+      if (written >= max_cov) return;
+      pkeys[off] = ty * tiles_x + tx;
+      pvals[off] = jj;
+      off++;
+      written++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tile ranges: off[t] = lower_bound(keys, t), off[num_tiles] = n
+// (the result of find-boundaries.ts:17-99; every entry written once)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ keys, uint32_t n, uint32_t num_tiles,
+                                                uint32_t *__restrict__ off) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cur = keys[i];
+  if (i == 0) {
+    for (uint32_t t = 0; t <= cur; t++) off[t] = 0;
+  } else {
+    const uint32_t prev = keys[i - 1];
+    for (uint32_t t = prev + 1; t <= cur; t++) off[t] = i;
+  }
+  if (i == n - 1)
+    for (uint32_t t = cur + 1; t <= num_tiles; t++) off[t] = n;
+}
+
+// ---------------------------------------------------------------------------
+// rasterize (rasterize.ts:36-82) + finalize (finalize.ts:29-42)
+//
+// One wave64 per 16x16 tile. Lane l owns pixel column l%16 and the four rows
+// 4*(l/16) .. 4*(l/16)+3, so the wave covers the whole tile and needs no
+// workgroup barrier. The tile's splats are staged 64 at a time in LDS (one
+// record per lane) and read back as broadcasts. Every pixel runs exactly the
+// reference's loop body in the reference's order; a pixel stops when T < 1e-4
+// and the wave stops when all its pixels have.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, const RastParams &p) {
+  const float r = wclamp(cr + T * p.bg_r, 0.0f, 1.0f);
+  const float g = wclamp(cg + T * p.bg_g, 0.0f, 1.0f);
+  const float b = wclamp(cb + T * p.bg_b, 0.0f, 1.0f);
+  const float a = wclamp((1.0f - T) + T * p.bg_a, 0.0f, 1.0f);
+  const uint32_t R = u32sat(r * 255.0f + 0.5f);
+  const uint32_t G = u32sat(g * 255.0f + 0.5f);
+  const uint32_t B = u32sat(b * 255.0f + 0.5f);
+  const uint32_t A = u32sat(a * 255.0f + 0.5f);
+  return R | (G << 8) | (B << 16) | (A << 24);
+}
+
+__global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
+                                                  const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
+                                                  uint32_t *__restrict__ out_rgba, RastParams p) {
+  __shared__ float4 s_rec[64 * 3];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+  const int lane = threadIdx.x;
+  const uint32_t px = tx * kTile + (lane & 15);
+  const uint32_t py0 = ty * kTile + (lane >> 4) * 4;
+  const float pxf = (float)px + 0.5f;
+  float cr[4], cg[4], cb[4], T[4], pyf[4];
+  bool live[4], inb[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t y = py0 + k;
+    inb[k] = px < p.width && y < p.height;
+    pyf[k] = (float)y + 0.5f;
+    cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = 1.0f;
+    if (inb[k] && p.resume) {
+      const float4 a = accum[(size_t)y * p.width + px];
+      cr[k] = a.x; cg[k] = a.y; cb[k] = a.z; T[k] = a.w;
+    }
+    live[k] = inb[k] && !(T[k] < kMinT);
+  }
+  if (p.has_pairs) {
+    const uint32_t start = off[tile], end = off[tile + 1];
+    for (uint32_t base = start; base < end; base += 64) {
+      const bool any_live = live[0] || live[1] || live[2] || live[3];
+      if (__ballot(any_live) == 0) break;
+      const uint32_t i = base + lane;
+      if (i < end) {
+        const uint32_t j = pvals[i];
+        s_rec[3 * lane + 0] = rec[3 * (size_t)j + 0];
+        s_rec[3 * lane + 1] = rec[3 * (size_t)j + 1];
+        s_rec[3 * lane + 2] = rec[3 * (size_t)j + 2];
+      }
+      __syncthreads();
+      const uint32_t cnt = min(64u, end - base);
+      for (uint32_t c = 0; c < cnt; c++) {
+        const float4 a = s_rec[3 * c + 0];
+        const float4 b = s_rec[3 * c + 1];
+        const float4 col = s_rec[3 * c + 2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!live[k]) continue;
+          const float dx = pxf - a.x;
+          const float dy = pyf[k] - a.y;
+          const float r = a.z;
+          if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
+          const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
+          if (power > 0.0f) continue;
+          const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf(power) - kGaussFloor));
+          if (alpha < kMinAlpha) continue;
+          const float w = T[k] * alpha;
+          cr[k] = cr[k] + w * col.x;
+          cg[k] = cg[k] + w * col.y;
+          cb[k] = cb[k] + w * col.z;
+          T[k] = T[k] * (1.0f - alpha);
+          live[k] = !(T[k] < kMinT);
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!inb[k]) continue;
+    const size_t pix = (size_t)(py0 + k) * p.width + px;
+    if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
+    if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static uint32_t bit_width_u32(uint32_t v) {
+  uint32_t b = 0;
+  while (v) {
+    b++;
+    v >>= 1;
+  }
+  return b;
+}
+
+void Renderer::ensure_events(size_t n) {
+  while (events.size() < n) {
+    hipEvent_t e;
+    DL_HIP(hipEventCreate(&e));
+    events.push_back(e);
+  }
+}
+
+void Renderer::mark(int stage) {
+  if (!timing) return;
+  ensure_events(ev_used + 1);
+  DL_HIP(hipEventRecord(events[ev_used], ctx->stream));
+  ev_stage.push_back(stage);
+  ev_used++;
+}
+
+template <class T>
+static void tap_copy(std::vector<uint8_t> &dst, const T *dev, size_t count, hipStream_t s) {
+  const size_t old = dst.size();
+  dst.resize(old + count * sizeof(T));
+  if (count) DL_HIP(hipMemcpyAsync(dst.data() + old, dev, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  DL_HIP(hipStreamSynchronize(s));
+}
+
+void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_background &bg) {
+  if (cam.width == 0 || cam.height == 0) fail(DL_ERR_INVALID, "camera width and height must be > 0");
+  if (sc.bands > 3) fail(DL_ERR_INVALID, "shBands must be 0..3");
+  if (cam.projection != DL_PINHOLE && cam.projection != DL_EQUIRECT) fail(DL_ERR_INVALID, "unknown projection");
+  hipStream_t s = ctx->stream;
+  ctx->make_current();
+  ev_used = 0;
+  ev_stage.clear();
+  for (auto &t : taps) t.clear();
+
+  const uint32_t n = sc.count;
+  const uint32_t W = cam.width, H = cam.height;
+  const uint32_t tiles_x = div_up(W, kTile), tiles_y = div_up(H, kTile);
+  const uint32_t num_tiles = tiles_x * tiles_y;
+  const size_t npx = (size_t)W * H;
+  stats = dl_render_stats{};
+  stats.num_tiles = num_tiles;
+  stats.tiles_x = tiles_x;
+  stats.tiles_y = tiles_y;
+  width = W;
+  height = H;
+
+  rgba.reserve(npx * 4);
+  small.reserve(256);
+  uint32_t *d_nvis = small.as<uint32_t>();
+  uint32_t *d_total = d_nvis + 1;
+
+  RastParams rp{};
+  rp.tiles_x = tiles_x;
+  rp.width = W;
+  rp.height = H;
+  rp.bg_r = bg.r;
+  rp.bg_g = bg.g;
+  rp.bg_b = bg.b;
+  rp.bg_a = bg.a;
+
+  ViewF32 vf;
+  std::memcpy(vf.m, cam.view_matrix, sizeof(vf.m));
+
+  // ---- cull + depth keys + depth sort ----
+  uint32_t nvis = 0;
+  const uint32_t *sorted_ids = nullptr;
+  if (n > 0) {
+    keys.reserve((size_t)n * 4);
+    ids.reserve((size_t)n * 4);
+    tkeys.reserve((size_t)n * 4);
+    tids.reserve((size_t)n * 4);
+    sort_ws.reserve(radix_workspace_bytes(n, 32));
+    mark(DL_STAGE_CULL);
+    DL_HIP(hipMemsetAsync(d_nvis, 0, sizeof(uint32_t), s));
+    k_cull_key<<<div_up(n, kCullBlock), kCullBlock, 0, s>>>(sc.pos, n, vf, cam.near_plane, cam.far_plane,
+                                                             cam.projection, keys.as<uint32_t>(), ids.as<uint32_t>(),
+                                                             d_nvis);
+    DL_HIP(hipGetLastError());
+    mark(DL_STAGE_DEPTH_SORT);
+    const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
+                                         tids.as<uint32_t>(), n, 32, sort_ws.get());
+    sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
+    const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
+    DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    if (keep_taps) {
+      tap_copy(taps[DL_TAP_SORTED_INDICES], sorted_ids, nvis, s);
+      tap_copy(taps[DL_TAP_DEPTH_KEYS], sorted_keys, nvis, s);
+    }
+  }
+  stats.num_visible = nvis;
+
+  const bool need_accum = keep_taps;
+  bool accum_valid = false;
+  if (nvis == 0) {
+    // pipeline.ts:210-212: background only
+    rp.has_pairs = 0;
+    rp.resume = 0;
+    rp.finalize = 1;
+    rp.write_accum = need_accum;
+    if (need_accum) accum.reserve(npx * 16);
+    mark(DL_STAGE_RASTERIZE);
+    k_rasterize<<<num_tiles, 64, 0, s>>>(nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp);
+    DL_HIP(hipGetLastError());
+  } else {
+    const uint32_t chunk_cap = (opts.chunk_cap && opts.chunk_cap < nvis) ? opts.chunk_cap : nvis;
+    rec.reserve((size_t)chunk_cap * 48);
+    counts.reserve((size_t)chunk_cap * 4);
+    offsets.reserve((size_t)chunk_cap * 4);
+    scan_ws.reserve(scan_workspace_bytes(chunk_cap));
+    ranges.reserve(((size_t)num_tiles + 1) * 4);
+    const uint32_t chunks = div_up(nvis, chunk_cap);
+    if (chunks > 1 || need_accum) accum.reserve(npx * 16);
+    ProjUniforms u{};
+    std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
+    u.width = (float)W;
+    u.height = (float)H;
+    u.focal = (float)((double)H / (2.0 * std::tan(cam.fov_y / 2.0)));
+    u.cx = (float)((double)W / 2.0);
+    u.cy = (float)((double)H / 2.0);
+    u.near_ = (float)cam.near_plane;
+    u.far_ = (float)cam.far_plane;
+    u.tiles_x = tiles_x;
+    u.tiles_y = tiles_y;
+    u.max_cov = max_cov;
+    u.proj = cam.projection;
+    const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(num_tiles - 1));
+    stats.pair_key_bits = key_bits;
+    for (uint32_t c0 = 0; c0 < nvis; c0 += chunk_cap) {
+      const uint32_t cn = std::min(chunk_cap, nvis - c0);
+      const bool last = c0 + cn == nvis;
+      stats.num_chunks++;
+      mark(DL_STAGE_PROJECT);
+      switch (sc.bands) {
+        case 0: k_project<0><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
+        case 1: k_project<1><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
+        case 2: k_project<2><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
+        default: k_project<3><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
+      }
+      DL_HIP(hipGetLastError());
+      mark(DL_STAGE_SCAN);
+      exclusive_scan_u32(s, counts.as<uint32_t>(), offsets.as<uint32_t>(), cn, d_total, scan_ws.get());
+      uint32_t npairs = 0;
+      DL_HIP(hipMemcpyAsync(&npairs, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      DL_HIP(hipStreamSynchronize(s));
+      if (keep_taps) {
+        const size_t old = taps[DL_TAP_PROJECTED].size();
+        taps[DL_TAP_PROJECTED].resize(old + (size_t)cn * 48);
+        DL_HIP(hipMemcpyAsync(taps[DL_TAP_PROJECTED].data() + old, rec.get(), (size_t)cn * 48, hipMemcpyDeviceToHost, s));
+        tap_copy(taps[DL_TAP_TILE_COUNTS], counts.as<uint32_t>(), cn, s);
+      }
+      stats.num_pairs += npairs;
+      rp.resume = accum_valid;
+      rp.finalize = last;
+      rp.write_accum = !last || need_accum;
+      if (npairs == 0) {
+        // pipeline.ts:306: nothing to rasterize in this chunk
+        if (keep_taps) {
+          std::vector<uint32_t> z(num_tiles + 1, 0u);
+          auto &t = taps[DL_TAP_TILE_OFFSETS];
+          t.insert(t.end(), reinterpret_cast<uint8_t *>(z.data()), reinterpret_cast<uint8_t *>(z.data() + z.size()));
+        }
+        if (last) {
+          rp.has_pairs = 0;
+          mark(DL_STAGE_RASTERIZE);
+          k_rasterize<<<num_tiles, 64, 0, s>>>(nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp);
+          DL_HIP(hipGetLastError());
+        }
+        continue;
+      }
+      pkeys.reserve((size_t)npairs * 4);
+      pvals.reserve((size_t)npairs * 4);
+      ptkeys.reserve((size_t)npairs * 4);
+      ptvals.reserve((size_t)npairs * 4);
+      psort_ws.reserve(radix_workspace_bytes(npairs, key_bits));
+      mark(DL_STAGE_EMIT);
+      k_emit<<<div_up(cn, 128), 128, 0, s>>>(rec.as<float4>(), offsets.as<uint32_t>(), cn, tiles_x, tiles_y, max_cov,
+                                             pkeys.as<uint32_t>(), pvals.as<uint32_t>());
+      DL_HIP(hipGetLastError());
+      mark(DL_STAGE_PAIR_SORT);
+      const bool in_tmp = radix_sort_pairs(s, pkeys.as<uint32_t>(), pvals.as<uint32_t>(), ptkeys.as<uint32_t>(),
+                                           ptvals.as<uint32_t>(), npairs, (int)key_bits, psort_ws.get());
+      const uint32_t *skeys = in_tmp ? ptkeys.as<uint32_t>() : pkeys.as<uint32_t>();
+      const uint32_t *svals = in_tmp ? ptvals.as<uint32_t>() : pvals.as<uint32_t>();
+      mark(DL_STAGE_RANGES);
+      k_ranges<<<div_up(npairs, 256), 256, 0, s>>>(skeys, npairs, num_tiles, ranges.as<uint32_t>());
+      DL_HIP(hipGetLastError());
+      if (keep_taps) {
+        tap_copy(taps[DL_TAP_PAIR_TILES], skeys, npairs, s);
+        tap_copy(taps[DL_TAP_PAIR_SPLATS], svals, npairs, s);
+        tap_copy(taps[DL_TAP_TILE_OFFSETS], ranges.as<uint32_t>(), (size_t)num_tiles + 1, s);
+      }
+      rp.has_pairs = 1;
+      mark(DL_STAGE_RASTERIZE);
+      k_rasterize<<<num_tiles, 64, 0, s>>>(rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
+                                           rgba.as<uint32_t>(), rp);
+      DL_HIP(hipGetLastError());
+      accum_valid = true;
+    }
+  }
+  mark(-1);
+  DL_HIP(hipStreamSynchronize(s));
+  if (keep_taps) tap_copy(taps[DL_TAP_ACCUM], accum.as<float>(), npx * 4, s);
+  if (timing && ev_used > 1) {
+    for (size_t k = 0; k + 1 < ev_used; k++) {
+      float ms = 0.0f;
+      DL_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
+      if (ev_stage[k] >= 0 && ev_stage[k] < DL_NUM_STAGES) stats.stage_ms[ev_stage[k]] += ms;
+      stats.total_ms += ms;
+    }
+  }
+}
+
+void Renderer::read_image(uint8_t *out) {
+  if (!width) fail(DL_ERR_INVALID, "no image has been rendered yet");
+  DL_HIP(hipMemcpyAsync(out, rgba.get(), (size_t)width * height * 4, hipMemcpyDeviceToHost, ctx->stream));
+  DL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+Renderer::~Renderer() {
+  for (auto e : events) (void)hipEventDestroy(e);
+}
+
+}  // namespace dl

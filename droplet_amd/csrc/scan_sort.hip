@@ -1,0 +1,336 @@
+// Device-wide exclusive scan and stable LSD radix sort (u32 key / u32 value)
+// for gfx950.
+//
+// Both are tiled 256-thread workgroups (4 wave64) over 4096-element tiles.
+// The sort is the replacement for src/gpu/radix-sort.ts:91-216, whose scatter
+// takes slots with atomicAdd and is therefore unstable; this one ranks every
+// element inside its wave with ballots in input order, so equal keys keep
+// their input order across passes (stable).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "dl_common.h"
+#include "dl_internal.h"
+
+namespace dl {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTileN = kBlock * kItems;  // 4096
+constexpr int kWaves = kBlock / 64;
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total to *total (all threads).
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave, uint32_t *total) {
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  uint32_t inc = wave_inclusive_scan(v);
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  uint32_t wave_off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    uint32_t c = s_wave[w];
+    if (w < wave) wave_off += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + inc - v;
+}
+
+// -------------------------------- scan ------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in, size_t n,
+                                                        uint32_t *__restrict__ partials) {
+  __shared__ uint32_t s_wave[kWaves];
+  const size_t base = (size_t)blockIdx.x * kTileN;
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    size_t i = base + (size_t)k * kBlock + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  acc = wave_sum(acc);
+  if (__lane_id() == 0) s_wave[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kWaves; w++) t += s_wave[w];
+    partials[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of one 4096-element tile plus the tile's prefix (from the
+// scanned partials). The last tile also writes the grand total.
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                      size_t n, const uint32_t *__restrict__ prefixes,
+                                                      uint32_t *__restrict__ d_total) {
+  __shared__ uint32_t s_data[kTileN];
+  __shared__ uint32_t s_wave[kWaves];
+  const size_t base = (size_t)blockIdx.x * kTileN;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    int li = k * kBlock + threadIdx.x;
+    size_t i = base + li;
+    s_data[li] = i < n ? in[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t v[kItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    v[k] = s_data[threadIdx.x * kItems + k];
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, s_wave, &total);
+  const uint32_t prefix = prefixes ? prefixes[blockIdx.x] : 0u;
+  run += prefix;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    s_data[threadIdx.x * kItems + k] = run;
+    run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    int li = k * kBlock + threadIdx.x;
+    size_t i = base + li;
+    if (i < n) out[i] = s_data[li];
+  }
+  if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_total = prefix + total;
+}
+
+// ------------------------------- radix ------------------------------------
+// Tile layout: wave w owns elements [w*1024, (w+1)*1024) of the tile; item k
+// of lane l is element w*1024 + k*64 + l. Loads are coalesced and the
+// (k, lane) processing order is the input order, which makes ranks stable.
+__global__ __launch_bounds__(kBlock) void k_radix_upsweep(const uint32_t *__restrict__ keys, size_t n, int shift,
+                                                          int bits, uint32_t *__restrict__ counts,
+                                                          uint32_t nblocks) {
+  __shared__ uint32_t s_hist[256];
+  const uint32_t mask = (1u << bits) - 1u;
+  s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * kTileN;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    size_t i = base + (size_t)k * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&s_hist[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= mask) counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = s_hist[threadIdx.x];
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
+                                                            const uint32_t *__restrict__ vals_in,
+                                                            uint32_t *__restrict__ keys_out,
+                                                            uint32_t *__restrict__ vals_out, size_t n, int shift,
+                                                            const uint32_t *__restrict__ offsets,
+                                                            uint32_t nblocks) {
+  constexpr uint32_t kDigits = 1u << BITS;
+  constexpr uint32_t kMask = kDigits - 1u;
+  __shared__ uint32_t s_whist[kWaves][256];
+  __shared__ uint32_t s_keys[kTileN];
+  __shared__ uint32_t s_vals[kTileN];
+  __shared__ uint32_t s_start[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_wave[kWaves];
+
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const size_t tile0 = (size_t)blockIdx.x * kTileN;
+  const uint32_t tile_n = (uint32_t)std::min<size_t>(kTileN, n - tile0);
+
+  for (int d = threadIdx.x; d < kWaves * 256; d += kBlock) (&s_whist[0][0])[d] = 0;
+  __syncthreads();
+
+  uint32_t key[kItems], val[kItems], rank[kItems];
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
+    const bool valid = li < tile_n;
+    key[k] = valid ? keys_in[tile0 + li] : 0u;
+    val[k] = valid ? vals_in[tile0 + li] : 0u;
+    const uint32_t d = (key[k] >> shift) & kMask;
+    uint64_t peers = __ballot(valid);
+    if (!valid) peers = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+      const int set = (d >> b) & 1u;
+      const uint64_t bal = __ballot(set);
+      peers &= set ? bal : ~bal;
+    }
+    const int leader = peers ? (__ffsll((long long)peers) - 1) : 0;
+    uint32_t old = 0;
+    if (valid && lane == leader) {
+      old = s_whist[wave][d];
+      s_whist[wave][d] = old + (uint32_t)__popcll(peers);
+    }
+    old = (uint32_t)__shfl((int)old, leader, 64);
+    rank[k] = old + (uint32_t)__popcll(peers & lt);
+  }
+  __syncthreads();
+  // per digit: wave offsets and block count, then block-wide digit starts
+  {
+    const uint32_t d = threadIdx.x;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      uint32_t c = s_whist[w][d];
+      s_whist[w][d] = cnt;
+      cnt += c;
+    }
+    if (d >= kDigits) cnt = 0;
+    uint32_t total;
+    uint32_t start = block_exclusive_scan(cnt, s_wave, &total);
+    s_start[d] = start;
+    if (d < kDigits) s_gbase[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
+    if (li < tile_n) {
+      const uint32_t d = (key[k] >> shift) & kMask;
+      const uint32_t pos = s_start[d] + s_whist[wave][d] + rank[k];
+      s_keys[pos] = key[k];
+      s_vals[pos] = val[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    const uint32_t p = (uint32_t)k * kBlock + threadIdx.x;
+    if (p < tile_n) {
+      const uint32_t kk = s_keys[p];
+      const uint32_t d = (kk >> shift) & kMask;
+      const size_t dst = (size_t)s_gbase[d] + (p - s_start[d]);
+      keys_out[dst] = kk;
+      vals_out[dst] = s_vals[p];
+    }
+  }
+}
+
+using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
+                        const uint32_t *, uint32_t);
+
+DownFn downsweep_for(int bits) {
+  switch (bits) {
+    case 1: return k_radix_downsweep<1>;
+    case 2: return k_radix_downsweep<2>;
+    case 3: return k_radix_downsweep<3>;
+    case 4: return k_radix_downsweep<4>;
+    case 5: return k_radix_downsweep<5>;
+    case 6: return k_radix_downsweep<6>;
+    case 7: return k_radix_downsweep<7>;
+    default: return k_radix_downsweep<8>;
+  }
+}
+
+size_t scan_levels_bytes(size_t n) {
+  size_t bytes = 0;
+  while (n > (size_t)kTileN) {
+    n = div_up(n, kTileN);
+    bytes += ((n * sizeof(uint32_t) + 255) / 256) * 256;
+  }
+  return bytes + 256;
+}
+
+void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, uint8_t *ws) {
+  if (n == 0) {
+    if (d_total) DL_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+    return;
+  }
+  if (n <= (size_t)kTileN) {
+    k_scan_down<<<1, kBlock, 0, s>>>(in, out, n, nullptr, d_total);
+    DL_HIP(hipGetLastError());
+    return;
+  }
+  const uint32_t nb = div_up(n, kTileN);
+  uint32_t *partials = reinterpret_cast<uint32_t *>(ws);
+  uint8_t *next = ws + ((nb * sizeof(uint32_t) + 255) / 256) * 256;
+  k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, partials);
+  DL_HIP(hipGetLastError());
+  scan_rec(s, partials, partials, nb, nullptr, next);
+  k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total);
+  DL_HIP(hipGetLastError());
+}
+
+void pass_layout(int key_bits, int *passes, int bits[8]) {
+  int p = std::max(1, (key_bits + 7) / 8);
+  int per = (key_bits + p - 1) / p;
+  int left = key_bits;
+  for (int i = 0; i < p; i++) {
+    bits[i] = std::min(per, left);
+    left -= bits[i];
+  }
+  *passes = p;
+}
+
+}  // namespace
+
+size_t scan_workspace_bytes(size_t n) { return scan_levels_bytes(n); }
+
+void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n, uint32_t *d_total,
+                        void *ws) {
+  scan_rec(s, d_in, d_out, n, d_total, static_cast<uint8_t *>(ws));
+}
+
+size_t radix_workspace_bytes(size_t n, int key_bits) {
+  (void)key_bits;
+  const size_t nb = std::max<size_t>(1, div_up(n, kTileN));
+  const size_t counts = ((256 * nb * sizeof(uint32_t) + 255) / 256) * 256;
+  return counts * 2 + scan_levels_bytes(256 * nb);
+}
+
+bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
+                      size_t n, int key_bits, void *ws) {
+  if (n <= 1 || key_bits <= 0) return false;
+  if (key_bits > 32) key_bits = 32;
+  const uint32_t nb = div_up(n, kTileN);
+  uint8_t *w = static_cast<uint8_t *>(ws);
+  const size_t counts_bytes = ((256 * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
+  uint32_t *counts = reinterpret_cast<uint32_t *>(w);
+  uint32_t *offsets = reinterpret_cast<uint32_t *>(w + counts_bytes);
+  uint8_t *scan_ws = w + 2 * counts_bytes;
+  int passes, bits[8];
+  pass_layout(key_bits, &passes, bits);
+  uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
+  int shift = 0;
+  for (int p = 0; p < passes; p++) {
+    const size_t ndig = (size_t)1 << bits[p];
+    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb);
+    DL_HIP(hipGetLastError());
+    scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
+    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb);
+    DL_HIP(hipGetLastError());
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+    shift += bits[p];
+  }
+  return ki == tmp_keys;
+}
+
+}  // namespace dl

@@ -1,0 +1,203 @@
+"""`Device`, `Renderer` and `Scene`: the Python mirror of droplet's render API.
+
+`create_device` follows `createDevice` (src/gpu/device.ts:27-41), `Renderer`
+follows `Renderer` (src/render/pipeline.ts:43-592): same option names, same
+argument meaning, same errors raised as exceptions. Everything runs in
+libdroplet_hip.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .types import BackgroundRGBA, RenderCamera, SplatData
+
+
+@dataclass
+class DeviceOptions:
+    """`DeviceOptions` (src/gpu/device.ts:11-20); only the ordinal applies."""
+
+    device_index: int = 0
+
+
+@dataclass
+class RendererOptions:
+    """`RendererOptions` (src/render/pipeline.ts:43-48). `chunk_cap` None keeps
+    every visible splat in one device chunk (images are identical to the
+    reference's 200k chunks); `max_coverage_per_splat` defaults to 16384."""
+
+    chunk_cap: Optional[int] = None
+    max_coverage_per_splat: Optional[int] = None
+
+
+class Device:
+    """A HIP device context (the GPUDevice stand-in)."""
+
+    def __init__(self, opts: Optional[DeviceOptions] = None):
+        lib = L.load()
+        o = L.DlDeviceOpts()
+        o.device_index = (opts or DeviceOptions()).device_index
+        h = C.c_void_p()
+        L.check(lib.dl_context_create(C.byref(o), C.byref(h)))
+        self._h = h
+        buf = C.create_string_buffer(64)
+        L.check(lib.dl_context_device_name(self._h, buf, 64))
+        self.name = buf.value.decode()
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("device destroyed")
+        return self._h
+
+    def destroy(self) -> None:
+        if self._h is not None:
+            L.load().dl_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def create_device(opts: Optional[DeviceOptions] = None) -> Device:
+    """`createDevice` (src/gpu/device.ts:27-41): raises when no HIP device."""
+    return Device(opts)
+
+
+class Scene:
+    """Device-resident SplatData: uploaded once, rendered many times."""
+
+    def __init__(self, device: Device, data: SplatData):
+        self.device = device
+        self.count = data.count
+        self.sh_bands = data.sh_bands
+        h = C.c_void_p()
+        s = L.splats_struct(data)
+        L.check(L.load().dl_scene_create(device.handle, C.byref(s), C.byref(h)))
+        self._h = h
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            L.load().dl_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Renderer:
+    """Tile-binned splat rasterizer (src/render/pipeline.ts:50-592)."""
+
+    def __init__(self, device: Device, opts: Optional[RendererOptions] = None):
+        self.device = device
+        opts = opts or RendererOptions()
+        o = L.DlRendererOpts()
+        o.chunk_cap = int(opts.chunk_cap or 0)
+        o.max_coverage_per_splat = int(opts.max_coverage_per_splat or 0)
+        h = C.c_void_p()
+        L.check(L.load().dl_renderer_create(device.handle, C.byref(o), C.byref(h)))
+        self._h = h
+        self._last_size = (0, 0)
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("renderer destroyed")
+        return self._h
+
+    def render(self, data: SplatData, camera: RenderCamera,
+               background: BackgroundRGBA = BackgroundRGBA()) -> np.ndarray:
+        """`Renderer.render` (pipeline.ts:181-185): W*H*4 RGBA8 bytes."""
+        out = np.empty(camera.width * camera.height * 4, np.uint8)
+        s = L.splats_struct(data)
+        c = L.camera_struct(camera)
+        b = L.background_struct(background)
+        L.check(L.load().dl_render(self.handle, C.byref(s), C.byref(c), C.byref(b), L.ptr(out, L.u8p)))
+        self._last_size = (camera.width, camera.height)
+        return out
+
+    def render_scene(self, scene: Scene, camera: RenderCamera,
+                     background: BackgroundRGBA = BackgroundRGBA(), readback: bool = True):
+        """Render a device-resident scene; with readback=False the image stays
+        on the device (see read_image)."""
+        c = L.camera_struct(camera)
+        b = L.background_struct(background)
+        out = np.empty(camera.width * camera.height * 4, np.uint8) if readback else None
+        L.check(L.load().dl_render_scene(self.handle, scene._h, C.byref(c), C.byref(b),
+                                         L.ptr(out, L.u8p) if readback else None))
+        self._last_size = (camera.width, camera.height)
+        return out
+
+    def read_image(self) -> np.ndarray:
+        w, h = self._last_size
+        out = np.empty(w * h * 4, np.uint8)
+        L.check(L.load().dl_renderer_read_image(self.handle, L.ptr(out, L.u8p)))
+        return out
+
+    def set_timing(self, enabled: bool) -> None:
+        L.check(L.load().dl_renderer_set_timing(self.handle, int(enabled)))
+
+    def set_taps(self, enabled: bool) -> None:
+        L.check(L.load().dl_renderer_set_taps(self.handle, int(enabled)))
+
+    def stats(self) -> dict:
+        st = L.DlRenderStats()
+        L.check(L.load().dl_renderer_stats(self.handle, C.byref(st)))
+        return {
+            "num_visible": st.num_visible, "num_chunks": st.num_chunks, "num_tiles": st.num_tiles,
+            "tiles_x": st.tiles_x, "tiles_y": st.tiles_y, "pair_key_bits": st.pair_key_bits,
+            "num_pairs": int(st.num_pairs),
+            "stage_ms": {name: float(st.stage_ms[i]) for i, name in enumerate(L.STAGES)},
+            "total_ms": float(st.total_ms),
+        }
+
+    def tap(self, name: str) -> np.ndarray:
+        which = L.TAPS.index(name)
+        n = C.c_size_t()
+        L.check(L.load().dl_renderer_tap(self.handle, which, None, 0, C.byref(n)))
+        dtype = np.float32 if name in ("projected", "accum") else np.uint32
+        out = np.empty(n.value // 4, dtype)
+        L.check(L.load().dl_renderer_tap(self.handle, which, out.ctypes.data_as(C.c_void_p), n.value, None))
+        return out
+
+    def destroy(self) -> None:
+        """`Renderer.destroy` (pipeline.ts:581-591)."""
+        if getattr(self, "_h", None) is not None:
+            L.load().dl_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def sort_pairs_u32(device: Device, keys: np.ndarray, vals: np.ndarray, key_bits: int = 32):
+    """Stable GPU sort of u32 (key, value) pairs on the low key_bits bits;
+    the stable replacement of `GpuRadixSorter.sort` (src/gpu/radix-sort.ts:142-215)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    v = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if k.size != v.size:
+        raise ValueError("keys and vals differ in length")
+    L.check(L.load().dl_sort_pairs_u32(device.handle, L.ptr(k, L.u32p), L.ptr(v, L.u32p), k.size, key_bits))
+    return k, v
+
+
+def debug_math(device: Device, op: str, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:
+    ops = {"exp": 0, "sqrt": 1, "div": 2, "depth_key": 3, "sigmoid": 4}
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+    out = np.empty_like(a)
+    L.check(L.load().dl_debug_math(device.handle, ops[op], L.ptr(a), L.ptr(bb), L.ptr(out), a.size))
+    return out

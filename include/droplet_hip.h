@@ -1,0 +1,184 @@
+/*
+ * libdroplet_hip.so -- C ABI of the MI355X (gfx950) forward splat renderer and
+ * cooperative voxelizer.
+ *
+ * This ABI replaces the reference's WebGPU device layer and GPU kernels
+ * (citations are relative to the droplet repository):
+ *   src/gpu/device.ts        createDevice / uploadBuffer / readbackBuffer
+ *   src/gpu/radix-sort.ts    GpuRadixSorter.sort
+ *   src/render/pipeline.ts   Renderer (constructor, render, destroy)
+ *   src/voxel/voxelize.ts    voxelize
+ *   src/render/shaders/*, src/voxel/shaders/voxelize.ts   (every WGSL kernel)
+ *
+ * Conventions:
+ *   - Every call returns DL_OK (0) or an error code; dl_last_error() gives a
+ *     thread-local message for the last failure on the calling thread.
+ *   - Host pointers are borrowed for the duration of the call; device memory
+ *     is owned by the library. Calls are synchronous unless stated.
+ *   - A dl_renderer is not re-entrant: its scratch memory is shared by its
+ *     calls (the reference's Renderer has the same rule).
+ *   - Structs hold plain scalars and pointers only, so bun:ffi, ctypes, JNI
+ *     or cgo can bind them directly.
+ */
+#ifndef DROPLET_HIP_H
+#define DROPLET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define DL_OK 0
+#define DL_ERR_INVALID 1     /* bad argument (reference: TypeError / GPU validation error) */
+#define DL_ERR_DEVICE 2      /* HIP runtime failure or no device (reference: 'No WebGPU adapter') */
+#define DL_ERR_OOM 3         /* device allocation failed */
+#define DL_ERR_CAPACITY 4    /* a documented size limit was exceeded */
+#define DL_ERR_INTERNAL 5
+
+#define DL_PINHOLE 0u
+#define DL_EQUIRECT 1u
+
+typedef struct dl_context dl_context;
+typedef struct dl_renderer dl_renderer;
+typedef struct dl_scene dl_scene;
+typedef struct dl_voxel_result dl_voxel_result;
+
+/* SplatData (src/types.ts:23-32): struct of float32 arrays, N = count.
+ * positions 3N, rotations 4N [w,x,y,z], log_scales 3N, opacity_logits N,
+ * colors_dc 3N, colors_sh 3*M*N channel-major per splat (M = 0/3/8/15 for
+ * sh_bands 0..3; may be NULL when sh_bands == 0). */
+typedef struct {
+    uint32_t count;
+    const float *positions;
+    const float *rotations;
+    const float *log_scales;
+    const float *opacity_logits;
+    const float *colors_dc;
+    const float *colors_sh;
+    uint32_t sh_bands;
+} dl_splats;
+
+/* RenderCamera (src/types.ts:74-82). fov_y/near/far are f64 (JS numbers) so
+ * the f64 cull of src/render/camera.ts:56-99 is kept. */
+typedef struct {
+    uint32_t width, height;
+    float view_matrix[16];       /* column-major world->camera */
+    double fov_y, near_plane, far_plane;
+    uint32_t projection;         /* DL_PINHOLE / DL_EQUIRECT */
+} dl_camera;
+
+/* BackgroundRGBA (src/types.ts:60-65); default (0,0,0,1) when NULL is passed */
+typedef struct { float r, g, b, a; } dl_background;
+
+/* DeviceOptions (src/gpu/device.ts:11-20): only the device ordinal applies. */
+typedef struct { int device_index; } dl_device_opts;
+
+/* RendererOptions (src/render/pipeline.ts:43-48). 0 selects the reference
+ * default: chunk_cap = unchunked on the device (the image is bit-identical to
+ * the reference's 200000-splat chunks), max_coverage_per_splat = 16384. */
+typedef struct {
+    uint32_t chunk_cap;
+    uint32_t max_coverage_per_splat;
+} dl_renderer_opts;
+
+/* ---- device (replaces createDevice, src/gpu/device.ts:27-41) ---- */
+int dl_context_create(const dl_device_opts *opts, dl_context **out);
+void dl_context_destroy(dl_context *ctx);
+const char *dl_last_error(void);
+/* name of the device, e.g. "gfx950"; writes at most cap bytes */
+int dl_context_device_name(dl_context *ctx, char *buf, size_t cap);
+
+/* ---- renderer (replaces Renderer, src/render/pipeline.ts:50-592) ---- */
+int dl_renderer_create(dl_context *ctx, const dl_renderer_opts *opts, dl_renderer **out);
+void dl_renderer_destroy(dl_renderer *r);
+
+/* Renderer.render (pipeline.ts:181-426): uploads the splats, renders and
+ * writes W*H*4 bytes of RGBA8 (R,G,B,A byte order) into out_rgba. */
+int dl_render(dl_renderer *r, const dl_splats *splats, const dl_camera *cam,
+              const dl_background *bg, uint8_t *out_rgba);
+
+/* Device-resident scene: upload once, render many frames. */
+int dl_scene_create(dl_context *ctx, const dl_splats *splats, dl_scene **out);
+void dl_scene_destroy(dl_scene *scene);
+/* Render a resident scene. out_rgba may be NULL: the image then stays on the
+ * device (dl_renderer_read_image copies it later). */
+int dl_render_scene(dl_renderer *r, const dl_scene *scene, const dl_camera *cam,
+                    const dl_background *bg, uint8_t *out_rgba);
+int dl_renderer_read_image(dl_renderer *r, uint8_t *out_rgba);
+
+/* Stage timing and sizes of the last render. */
+#define DL_STAGE_CULL 0
+#define DL_STAGE_DEPTH_SORT 1
+#define DL_STAGE_PROJECT 2
+#define DL_STAGE_SCAN 3
+#define DL_STAGE_EMIT 4
+#define DL_STAGE_PAIR_SORT 5
+#define DL_STAGE_RANGES 6
+#define DL_STAGE_RASTERIZE 7
+#define DL_NUM_STAGES 8
+typedef struct {
+    uint32_t num_visible;
+    uint32_t num_chunks;
+    uint32_t num_tiles;
+    uint32_t tiles_x, tiles_y;
+    uint32_t pair_key_bits;
+    uint64_t num_pairs;          /* total over chunks */
+    float stage_ms[DL_NUM_STAGES]; /* hipEvent times on the renderer's stream (timing enabled) */
+    float total_ms;
+} dl_render_stats;
+int dl_renderer_set_timing(dl_renderer *r, int enabled);
+int dl_renderer_stats(dl_renderer *r, dl_render_stats *out);
+
+/* Debug taps for parity tests: when enabled, the next render keeps copies of
+ * its intermediate arrays (concatenated over chunks). */
+#define DL_TAP_SORTED_INDICES 0  /* u32[num_visible]  visible splat ids, front to back */
+#define DL_TAP_DEPTH_KEYS 1      /* u32[num_visible]  sortable depth keys, sorted */
+#define DL_TAP_PROJECTED 2       /* f32[num_visible*12] project() records in sorted order */
+#define DL_TAP_TILE_COUNTS 3     /* u32[num_visible]  per-splat tile counts */
+#define DL_TAP_PAIR_TILES 4      /* u32[num_pairs]    sorted tile ids */
+#define DL_TAP_PAIR_SPLATS 5     /* u32[num_pairs]    chunk-local splat index */
+#define DL_TAP_TILE_OFFSETS 6    /* u32[num_chunks*(num_tiles+1)] */
+#define DL_TAP_ACCUM 7           /* f32[W*H*4] final (r, g, b, T) */
+int dl_renderer_set_taps(dl_renderer *r, int enabled);
+/* bytes: capacity of dst; returns DL_ERR_CAPACITY if too small */
+int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *written);
+
+/* ---- stable u32 key/value sort (replaces GpuRadixSorter.sort,
+ * src/gpu/radix-sort.ts:142-215, which is unstable; this one is stable).
+ * Host arrays in/out; only the low key_bits bits of each key are compared. */
+int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t key_bits);
+
+/* ---- voxelizer (replaces voxelize, src/voxel/voxelize.ts:59-248) ---- */
+typedef struct {
+    double voxel_resolution;     /* 0 -> 0.05 */
+    double opacity_cutoff;       /* <0 -> 0.1 (0 is a valid cutoff) */
+    int has_bounds;              /* 1: use bounds_min/max (VoxelizeOptions.bounds) */
+    double bounds_min[3], bounds_max[3];
+} dl_voxel_opts;
+
+int dl_voxelize(dl_context *ctx, const dl_splats *splats, const dl_voxel_opts *opts,
+                dl_voxel_result **out);
+/* number of non-empty blocks (BlockMaskBuffer.count) */
+uint64_t dl_voxel_result_count(const dl_voxel_result *res);
+/* blocks in the reference's insertion order (BlockMaskBuffer.blocks) */
+int dl_voxel_result_blocks(const dl_voxel_result *res, uint32_t *block_idx, uint32_t *mask_lo,
+                           uint32_t *mask_hi);
+/* bounds and numBlocksX/Y/Z (JS numbers; -Infinity for an empty scene) */
+int dl_voxel_result_bounds(const dl_voxel_result *res, double bounds_min[3], double bounds_max[3],
+                           double num_blocks[3]);
+/* GPU time of the voxelizer's device stages of the last call, in ms */
+float dl_voxel_result_device_ms(const dl_voxel_result *res);
+void dl_voxel_result_free(dl_voxel_result *res);
+
+/* ---- numerics self-check: evaluates the device's transcendental and IEEE
+ * primitives on n inputs (used by the parity tests).
+ * op: 0 exp, 1 sqrt, 2 div (a/b), 3 f64->f32 depth key, 4 sigmoid. */
+int dl_debug_math(dl_context *ctx, int op, const float *a, const float *b, float *out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+/*
+ * CPU oracle for droplet's forward splat renderer and voxelizer.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP product
+ * path (libdroplet_hip.so) is compared against. Only tests, the smoke check
+ * and the bench's CPU-baseline leg load it; the product never links it.
+ *
+ * It restates, in plain C with f32/f64 arithmetic and no FMA contraction, the
+ * reference's TypeScript host code and WGSL kernels (file:line citations are
+ * relative to the droplet repository):
+ *   camera / cull ......... src/render/camera.ts:32-99
+ *   depth sort ............ src/spatial/radix-sort.ts:20-117
+ *   pack .................. src/render/preprocess.ts:90-138
+ *   project ............... src/render/shaders/project.ts:73-383
+ *   scan .................. src/render/shaders/prefix-sum.ts:22-86, src/render/pipeline.ts:277-286
+ *   emit .................. src/render/shaders/emit-pairs.ts:27-56
+ *   pair sort ............. src/render/pipeline.ts:544-578
+ *   tile ranges ........... src/render/shaders/find-boundaries.ts:17-99
+ *   rasterize ............. src/render/shaders/rasterize.ts:36-82
+ *   finalize .............. src/render/shaders/finalize.ts:29-42
+ *   render orchestration .. src/render/pipeline.ts:181-426
+ *   AABB .................. src/spatial/aabb.ts:22-62
+ *   BVH ................... src/spatial/bvh.ts:104-239
+ *   voxelize .............. src/voxel/voxelize.ts:59-306, src/voxel/shaders/voxelize.ts:360-453
+ *   block mask buffer ..... src/voxel/block-mask-buffer.ts:23-58
+ *
+ * Transcendentals. WGSL leaves exp() precision to the adapter, so the oracle
+ * fixes one function, orc_expf: Cody-Waite reduction by ln2 (two-constant
+ * split) and the degree-7 Taylor polynomial of e^r on |r| <= ln2/2, evaluated
+ * with fmaf (Horner), scaled by 2^k in two exact steps. It is pinned against
+ * libm (<= 1 ulp). The product implements the same specification. atan2/asin
+ * (equirect only) are evaluated in f64 and rounded to f32. JS f64 arithmetic
+ * (cull, AABB, bounds, BVH) is restated in C double without contraction.
+ *
+ * Pinning status of each part is listed in the test suite (tests/test_oracle_*.py).
+ */
+#ifndef DROPLET_ORACLE_H
+#define DROPLET_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_PINHOLE 0u
+#define ORC_EQUIRECT 1u
+
+typedef struct {
+    uint32_t count;
+    const float *positions;      /* 3N */
+    const float *rotations;      /* 4N, [w,x,y,z] */
+    const float *log_scales;     /* 3N */
+    const float *opacity_logits; /* N */
+    const float *colors_dc;      /* 3N */
+    const float *colors_sh;      /* 3*M*N channel-major per splat, may be NULL when bands == 0 */
+    uint32_t sh_bands;           /* 0..3 */
+} orc_splats;
+
+typedef struct {
+    uint32_t width, height;
+    float view[16];              /* column-major world->camera */
+    double fov_y, near_plane, far_plane;
+    uint32_t projection;         /* ORC_PINHOLE / ORC_EQUIRECT */
+} orc_camera;
+
+/* Optional outputs of orc_render. Any pointer may be NULL. Capacities are in
+ * elements; when a capacity is too small nothing is written to that array and
+ * the needed size is still reported. Arrays that are per chunk are the
+ * concatenation over chunks in chunk order. */
+typedef struct {
+    /* sizes (always written) */
+    uint32_t num_visible;
+    uint32_t num_chunks;
+    uint64_t num_pairs;          /* total over chunks */
+    uint32_t num_tiles;
+    /* stats (always written) */
+    uint64_t pixel_iters;        /* loop iterations executed across all pixels (before the T break) */
+    uint64_t pixel_evals;        /* iterations that passed the r-box test */
+    uint64_t pixel_blends;       /* iterations that composited (alpha >= 1/255) */
+    uint64_t tile_iters;         /* per tile: max over its pixels of iterations, summed over tiles */
+    /* arrays */
+    uint32_t *sorted_indices;    /* [num_visible] visible splat ids, front to back */
+    uint32_t *depth_keys;        /* [num_visible] sortable f32 depth keys in sorted order */
+    float *projected;            /* [num_visible * 12] project() records (3 vec4 each) */
+    uint32_t *tile_counts;       /* [num_visible] per-splat tile counts (pre-scan) */
+    uint32_t *pair_tiles;        /* [num_pairs] sorted tile ids */
+    uint32_t *pair_splats;       /* [num_pairs] chunk-local splat index */
+    uint32_t *tile_offsets;      /* [num_chunks * (num_tiles + 1)] */
+    uint32_t *chunk_pairs;       /* [num_chunks] pairs in each chunk */
+    float *accum;                /* [W*H*4] final (r, g, b, T) */
+    uint64_t cap_visible;
+    uint64_t cap_pairs;
+    uint64_t cap_chunks;
+} orc_render_taps;
+
+/* --- primitives --- */
+float orc_expf(float x);
+void orc_expf_array(const float *in, float *out, size_t n);
+float orc_sigmoidf(float x);
+uint32_t orc_float_to_sortable(float f);
+uint32_t orc_u32sat(float f);
+
+/* --- camera / cull / depth sort --- */
+double orc_focal(const orc_camera *cam);
+uint32_t orc_cull(const float *positions, uint32_t n, const orc_camera *cam,
+                  uint32_t *out_indices, float *out_depths);
+void orc_depth_sort(uint32_t *indices, const float *depths, uint32_t n);
+
+/* --- project one splat from its packed record (stride 14+3M floats) ---
+ * writes 12 floats into out and returns the tile count */
+uint32_t orc_project_packed(const float *rec, uint32_t bands, const orc_camera *cam,
+                            uint32_t max_coverage, float out[12]);
+
+/* --- full render (pipeline.ts:181-473). chunk_cap 0 = reference default
+ * (200000); max_coverage 0 = reference default (16384). bg may be NULL
+ * (reference default (0,0,0,1)). out_rgba: W*H*4 bytes or NULL. */
+int orc_render(const orc_splats *s, const orc_camera *cam, const float *bg,
+               uint32_t chunk_cap, uint32_t max_coverage, uint8_t *out_rgba,
+               orc_render_taps *taps, int num_threads);
+
+/* Stable counting sort of (tile, value) pairs by tile (pipeline.ts:544-578). */
+void orc_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n);
+
+/* Time-bounded sample of the render: runs every CPU stage on the whole scene
+ * and rasterizes only tile rows r with r % row_step == 0; returns seconds for
+ * the stages in stage_seconds[0..6] = cull, depth sort, pack+project,
+ * scan+emit, pair sort, ranges, rasterize(sampled rows)+finalize. */
+int orc_render_timed(const orc_splats *s, const orc_camera *cam, uint32_t row_step,
+                     int num_threads, double *stage_seconds, uint64_t *num_pairs);
+
+/* --- voxelizer (voxelize.ts:59-306) --- */
+typedef struct {
+    double voxel_resolution;     /* default 0.05 */
+    double opacity_cutoff;       /* default 0.1 */
+    int has_bounds;
+    double bounds_min[3], bounds_max[3];
+} orc_voxel_opts;
+
+typedef struct {
+    /* always written */
+    uint64_t num_blocks;         /* non-empty blocks */
+    double bounds_min[3], bounds_max[3];
+    double num_blocks_xyz[3];    /* JS numbers: may be -Infinity for an empty scene */
+    uint64_t num_batches;        /* batches with >= 1 candidate */
+    uint64_t num_candidates;     /* sum of batch candidate counts */
+    uint64_t contrib_pairs;      /* (voxel, Gaussian) evaluations that passed the AABB test */
+    uint64_t num_near;           /* voxels with |alpha - cutoff| <= near_eps */
+    /* arrays (may be NULL; capacities in elements) */
+    uint32_t *block_idx, *mask_lo, *mask_hi;   /* [num_blocks] insertion order */
+    uint64_t cap_blocks;
+    uint32_t *near_block, *near_bit;           /* [num_near] */
+    float *near_alpha;
+    uint64_t cap_near;
+    double near_eps;
+} orc_voxel_result;
+
+int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_result *res,
+                 int num_threads);
+void orc_aabb_extents(const orc_splats *s, float *out_ext);
+/* BVH DFS leaf order (right child first) of all Gaussians, written to out_perm[N] */
+int orc_bvh_order(const float *positions, const float *extents, uint32_t n, uint32_t *out_perm);
+
+/* Morton helpers (morton.ts:11-22) */
+uint32_t orc_morton_encode3(uint32_t x, uint32_t y, uint32_t z);
+void orc_morton_decode3(uint32_t m, uint32_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+"""GPU parity of the render path: libdroplet_hip.so (through the C ABI) against
+the CPU oracle on identical seeded inputs.
+
+Bar: sort keys, pair lists, tile ranges, projected records, the FP32
+accumulator and the RGBA8 image are bit-exact (the kernels and the oracle use
+one exp specification and IEEE-exact +,-,*,/,sqrt without contraction).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from droplet_amd import synthetic as S
+from droplet_amd.renderer import Renderer, RendererOptions, Scene, debug_math, sort_pairs_u32
+from droplet_amd.types import BackgroundRGBA
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_taps(renderer, data, cam, bg=None):
+    renderer.set_taps(True)
+    try:
+        rgba = renderer.render(data, cam, bg or BackgroundRGBA())
+        taps = {name: renderer.tap(name) for name in
+                ("sorted_indices", "depth_keys", "projected", "tile_counts", "pair_tiles", "pair_splats",
+                 "tile_offsets", "accum")}
+        st = renderer.stats()
+    finally:
+        renderer.set_taps(False)
+    return rgba, taps, st
+
+
+def assert_render_parity(renderer, data, cam, bg=None, chunk_cap=None, max_coverage=0):
+    rgba, t, st = _gpu_taps(renderer, data, cam, bg)
+    ref = oracle.render(data, cam, background=bg, chunk_cap=chunk_cap or 0xFFFFFFFF, max_coverage=max_coverage)
+    print(f"case: n={data.count} {cam.width}x{cam.height} visible={ref['num_visible']} pairs={ref['num_pairs']}",
+          flush=True)
+    assert st["num_visible"] == ref["num_visible"]
+    assert st["num_pairs"] == ref["num_pairs"]
+    np.testing.assert_array_equal(t["sorted_indices"], ref["sorted_indices"])
+    np.testing.assert_array_equal(t["depth_keys"], ref["depth_keys"])
+    proj = t["projected"].reshape(-1, 12)
+    rp = ref["projected"]
+    culled = rp[:, 2] < 0
+    # culled splats: only slot 0 is defined (project.ts:217-219)
+    np.testing.assert_array_equal(proj[culled, :4].view(np.uint32), rp[culled, :4].view(np.uint32))
+    np.testing.assert_array_equal(proj[~culled].view(np.uint32), rp[~culled].view(np.uint32))
+    np.testing.assert_array_equal(t["tile_counts"], ref["tile_counts"])
+    np.testing.assert_array_equal(t["pair_tiles"], ref["pair_tiles"])
+    np.testing.assert_array_equal(t["pair_splats"], ref["pair_splats"])
+    np.testing.assert_array_equal(t["tile_offsets"], ref["tile_offsets"].reshape(-1))
+    np.testing.assert_array_equal(t["accum"].view(np.uint32), ref["accum"].reshape(-1).view(np.uint32))
+    np.testing.assert_array_equal(rgba, ref["rgba"])
+    return rgba, ref
+
+
+def test_exp_matches_oracle_bitexact(device):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-110, 95, 200_000), rng.uniform(-5, 0, 200_000),
+                        np.array([0.0, -0.0, 1.0, -1.0, -4.5, 88.7, 88.8, -87.4, -103.9, -104.5, 90, -200])]
+                       ).astype(np.float32)
+    got = debug_math(device, "exp", x)
+    np.testing.assert_array_equal(got.view(np.uint32), oracle.expf(x).view(np.uint32))
+
+
+def test_ieee_primitives_match_host(device):
+    rng = np.random.default_rng(1)
+    a = (rng.standard_normal(300_000) * 10.0 ** rng.uniform(-20, 20, 300_000)).astype(np.float32)
+    b = (rng.standard_normal(300_000) * 10.0 ** rng.uniform(-20, 20, 300_000)).astype(np.float32)
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(debug_math(device, "div", a, b).view(np.uint32), (a / b).view(np.uint32))
+        aa = np.abs(a)
+        np.testing.assert_array_equal(debug_math(device, "sqrt", aa).view(np.uint32), np.sqrt(aa).view(np.uint32))
+
+
+def test_sort_pairs_stable(device):
+    rng = np.random.default_rng(2)
+    for n, bits in [(1, 32), (17, 4), (5000, 15), (1_000_003, 32), (300_000, 7)]:
+        keys = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+        vals = np.arange(n, dtype=np.uint32)
+        k, v = sort_pairs_u32(device, keys, vals, bits)
+        masked = keys & np.uint32((1 << bits) - 1 if bits < 32 else 0xFFFFFFFF)
+        order = np.argsort(masked, kind="stable")
+        np.testing.assert_array_equal(v, vals[order])
+        np.testing.assert_array_equal(k, keys[order])
+
+
+def test_render_plain_workload_parity(renderer):
+    data, cam = S.plain_render_workload(seed=0)
+    assert_render_parity(renderer, data, cam)
+
+
+def test_render_small_scenes_parity(renderer):
+    cam128 = S.default_camera(128, 128)
+    assert_render_parity(renderer, S.one_splat((0, 0, 0)), cam128)
+    assert_render_parity(renderer, S.many_splats(5000, "sphere"), S.default_camera(256, 256))
+    assert_render_parity(renderer, S.gaussian_cloud(20_000, seed=3, sh_bands=1), S.default_camera(333, 251))
+    assert_render_parity(renderer, S.gaussian_cloud(20_000, seed=4, sh_bands=2), S.default_camera(200, 100))
+    assert_render_parity(renderer, S.gaussian_cloud(20_000, seed=5, sh_bands=3, pos_std=0.75, pos_clip=3.0),
+                         S.default_camera(640, 360), bg=BackgroundRGBA(0.2, 0.4, 0.6, 0.5))
+
+
+def test_render_equirect_parity(renderer):
+    data = S.gaussian_cloud(30_000, seed=6, pos_std=1.5, pos_clip=3.0, sh_bands=1)
+    assert_render_parity(renderer, data, S.equirect_camera(512, 256))
+
+
+def test_render_empty_scene(renderer):
+    rgba = renderer.render(S.empty_scene(), S.default_camera(128, 128))
+    img = rgba.reshape(128, 128, 4)
+    assert (img[..., :3] <= 5).all()
+    assert (img[..., 3] == 255).all()
+
+
+def test_chunked_matches_unchunked(device):
+    data = S.many_splats(200_000, "random")
+    cam = S.default_camera(64, 64)
+    single = Renderer(device)
+    multi = Renderer(device, RendererOptions(chunk_cap=50_000))
+    try:
+        a = single.render(data, cam)
+        b = multi.render(data, cam)
+        np.testing.assert_array_equal(a, b)
+        assert multi.stats()["num_chunks"] == 4
+        ref = oracle.render(data, cam, chunk_cap=50_000, taps=False)
+        np.testing.assert_array_equal(b, ref["rgba"])
+    finally:
+        single.destroy()
+        multi.destroy()
+
+
+def test_chunked_taps_match_oracle(device):
+    data = S.gaussian_cloud(30_000, seed=8)
+    cam = S.default_camera(320, 180)
+    r = Renderer(device, RendererOptions(chunk_cap=7_000))
+    try:
+        rgba, t, st = _gpu_taps(r, data, cam)
+        ref = oracle.render(data, cam, chunk_cap=7_000)
+        assert st["num_chunks"] == ref["num_chunks"] == 5
+        np.testing.assert_array_equal(t["pair_tiles"], ref["pair_tiles"])
+        np.testing.assert_array_equal(t["pair_splats"], ref["pair_splats"])
+        np.testing.assert_array_equal(t["tile_offsets"], ref["tile_offsets"].reshape(-1))
+        np.testing.assert_array_equal(rgba, ref["rgba"])
+    finally:
+        r.destroy()
+
+
+def test_scene_handle_matches_render(device, renderer):
+    data, cam = S.plain_render_workload(seed=11)
+    scene = Scene(device, data)
+    try:
+        a = renderer.render_scene(scene, cam)
+        b = renderer.render(data, cam)
+        np.testing.assert_array_equal(a, b)
+        renderer.render_scene(scene, cam, readback=False)
+        np.testing.assert_array_equal(renderer.read_image(), a)
+    finally:
+        scene.destroy()
+
+
+def test_max_coverage_truncation_parity(device):
+    data = S.dense_cluster(20_000)
+    cam = S.default_camera(256, 256)
+    r = Renderer(device, RendererOptions(max_coverage_per_splat=7))
+    try:
+        assert_render_parity(r, data, cam, max_coverage=7)
+    finally:
+        r.destroy()
