@@ -1,0 +1,178 @@
+"""Benchmark of the forward splat renderer on one MI355X.
+
+Workload: 6,000,000 Gaussians (positions ~N(0, 0.75^2) clipped to [-3, 3],
+SH bands 3), 3840x2160 pinhole from (0,0,4), scene resident in HBM. A step is
+one full frame: f64 cull + depth keys, depth sort, project, scan, emit, pair
+sort, tile ranges, rasterize + finalize to RGBA8 in device memory. The RGBA8
+readback is timed separately and is not part of `value`.
+
+Prints one JSON line (metric/value in ms per frame, lower is better) with:
+  roofline      dominant kernel (rasterize) against the MI355X FP32 peak,
+                duration from hipEvents on the renderer's stream
+  cpu_baseline  the CPU oracle on the same scene: every stage on all 6M
+                splats, rasterize on every ROW_STEP-th tile row scaled up
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/frame @6M splats 4K; batched frames/s vs one-by-one; % HBM/VALU peak"
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (published)
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E (published)
+# Algorithmic work per rasterize loop iteration: the reference's per-pixel
+# body (rasterize.ts:62-79) is 31 f32 operations counting each add, mul,
+# compare, min/max and the exp as one.
+RAST_OPS_PER_ITER = 31
+ROW_STEP = 16                # CPU baseline: rasterize every 16th tile row
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def _load_traffic():
+    path = os.path.join(ROOT, "profiles", "rasterize_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(data, cam):
+    import oracle  # CPU checker only; never the measured product
+
+    threads = min(16, oracle.cpu_threads())
+    t0 = time.perf_counter()
+    secs, npairs = oracle.render_timed(data, cam, row_step=ROW_STEP, threads=threads)
+    wall = time.perf_counter() - t0
+    # stage 6 holds rasterize over 1/ROW_STEP of the tile rows (+ nothing else)
+    frame_s = sum(secs[:6]) + secs[6] * ROW_STEP
+    return {
+        "value": round(frame_s * 1e3, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
+        "sample": (f"C oracle on the same 6M-splat 4K frame: cull, depth sort, project, scan+emit, "
+                   f"counting pair sort ({npairs} pairs) and ranges on the whole scene; rasterize on every "
+                   f"{ROW_STEP}th tile row, scaled x{ROW_STEP}; {wall:.1f} s of CPU wall time; "
+                   f"{_cpu_model()}"),
+        "stage_s": [round(s, 3) for s in secs],
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--splats", type=int, default=6_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the renderer runs on a single "
+              "device and multi-GPU sharding is not part of this build.", file=sys.stderr)
+        return 2
+
+    from droplet_amd import synthetic as S
+    from droplet_amd.renderer import Renderer, Scene, create_device
+
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    note(f"generating {args.splats} splats")
+    data, cam = S.large_render_workload(n=args.splats)
+    dev = create_device()
+    scene = Scene(dev, data)
+    r = Renderer(dev)
+    note(f"scene resident on {dev.name}")
+
+    # one untimed instrumented frame: rasterize work units (pixel loop iterations)
+    r.set_counting(True)
+    r.render_scene(scene, cam, readback=False)
+    iters = r.stats()["pixel_iters"]
+    r.set_counting(False)
+    note(f"instrumented frame: {iters} pixel iterations")
+
+    for _ in range(args.warmup):
+        r.render_scene(scene, cam, readback=False)
+
+    r.set_timing(True)
+    stage_sum = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render_scene(scene, cam, readback=False)
+        st = r.stats()
+        for k, v in st["stage_ms"].items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    wall = time.perf_counter() - t0
+    r.set_timing(False)
+    ms_per_frame = wall * 1e3 / args.steps
+    stage_ms = {k: round(v / args.steps, 4) for k, v in stage_sum.items()}
+
+    t1 = time.perf_counter()
+    img = r.read_image()
+    readback_ms = (time.perf_counter() - t1) * 1e3
+    st = r.stats()
+    note(f"timed {args.steps} frames: {ms_per_frame:.3f} ms/frame")
+
+    rast_ms = stage_ms["rasterize"]
+    achieved = iters * RAST_OPS_PER_ITER / (rast_ms * 1e-3) / 1e12
+    traffic = _load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": round(ms_per_frame, 3),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_frame, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "large forward render: 6M Gaussians, SH bands 3, 3840x2160 pinhole, scene resident in HBM",
+            "splats": data.count, "sh_bands": data.sh_bands, "width": cam.width, "height": cam.height,
+            "projection": cam.projection, "visible": st["num_visible"], "pairs": st["num_pairs"],
+            "tiles": st["num_tiles"],
+        },
+        "roofline": {
+            "kernel": "k_rasterize", "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "work": f"{iters} pixel loop iterations x {RAST_OPS_PER_ITER} ops",
+            "launch_ms": rast_ms,
+        },
+        "stage_ms": stage_ms,
+        "readback_ms": round(readback_ms, 3),
+        "image_nonblack_px": int(((img.reshape(-1, 4)[:, :3] > 5).any(1)).sum()),
+    }
+    if not args.no_cpu:
+        note("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(data, cam)
+    print(json.dumps(line))
+    r.destroy()
+    scene.destroy()
+    dev.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

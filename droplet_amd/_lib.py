@@ -51,13 +51,14 @@ class DlDeviceOpts(C.Structure):
 
 
 class DlRendererOpts(C.Structure):
-    _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32)]
+    _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32), ("binning", C.c_uint32)]
 
 
 class DlRenderStats(C.Structure):
     _fields_ = [("num_visible", C.c_uint32), ("num_chunks", C.c_uint32), ("num_tiles", C.c_uint32),
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("pair_key_bits", C.c_uint32),
-                ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float)]
+                ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
+                ("pixel_iters", C.c_uint64)]
 
 
 class DlVoxelOpts(C.Structure):
@@ -79,6 +80,7 @@ SIGNATURES = [
     ("dl_render_scene", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlBackground), u8p]),
     ("dl_renderer_read_image", C.c_int, [C.c_void_p, u8p]),
     ("dl_renderer_set_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("dl_renderer_set_counting", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_stats", C.c_int, [C.c_void_p, C.POINTER(DlRenderStats)]),
     ("dl_renderer_set_taps", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_tap", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),

@@ -11,9 +11,6 @@
 #include "dl_internal.h"
 #include "render.h"
 
-struct dl_context {
-  dl::Context c;
-};
 struct dl_renderer {
   dl::Renderer r;
 };
@@ -240,6 +237,13 @@ int dl_renderer_set_timing(dl_renderer *r, int enabled) {
   return guard([&] {
     if (!r) fail(DL_ERR_INVALID, "null renderer");
     r->r.timing = enabled != 0;
+  });
+}
+
+int dl_renderer_set_counting(dl_renderer *r, int enabled) {
+  return guard([&] {
+    if (!r) fail(DL_ERR_INVALID, "null renderer");
+    r->r.counting = enabled != 0;
   });
 }
 

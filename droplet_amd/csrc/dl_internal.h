@@ -93,6 +93,24 @@ size_t radix_workspace_bytes(size_t n, int key_bits);
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
                       uint32_t *tmp_vals, size_t n, int key_bits, void *ws);
 
+// off[b] = lower_bound(sorted keys, b) for b < num_bins and off[num_bins] = n
+// (all zeros when n == 0): the result of find-boundaries.ts:17-99.
+void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_bins, uint32_t *d_off);
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+inline uint32_t bit_width_u32(uint32_t v) {
+  uint32_t b = 0;
+  while (v) {
+    b++;
+    v >>= 1;
+  }
+  return b;
+}
+
 }  // namespace dl
+
+// opaque handle of include/droplet_hip.h
+struct dl_context {
+  dl::Context c;
+};

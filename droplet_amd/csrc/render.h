@@ -4,6 +4,7 @@
 #include <array>
 #include <vector>
 
+#include "binning.h"
 #include "dl_internal.h"
 
 namespace dl {
@@ -47,19 +48,23 @@ struct Renderer {
   dl_renderer_opts opts{};
   uint32_t max_cov = 16384;
   Scene own_scene;
-  // depth sort
-  DeviceBuffer keys, ids, tkeys, tids, sort_ws;
+  // per splat (id order): depth keys/ids, projected records, tile counts
+  DeviceBuffer keys, ids, tkeys, tids, sort_ws, rec, counts, parts;
   // per chunk
-  DeviceBuffer rec, counts, offsets, scan_ws, ranges;
+  DeviceBuffer chunk_counts, offsets, scan_ws, ranges;
   // pairs
   DeviceBuffer pkeys, pvals, ptkeys, ptvals, psort_ws;
+  // span binning
+  BinWorkspace bin_ws;
   // image
   DeviceBuffer accum, rgba, small;
   uint32_t width = 0, height = 0;
   dl_render_stats stats{};
   bool timing = false;
   bool keep_taps = false;
+  bool counting = false;
   std::array<std::vector<uint8_t>, 8> taps;
+  std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)
   std::vector<hipEvent_t> events;
   std::vector<int> ev_stage;
   size_t ev_used = 0;

@@ -1,18 +1,20 @@
 // Forward splat renderer on gfx950: the device path behind dl_render.
 //
 // Stages (reference citations are relative to the droplet repository):
-//   k_cull_key   f64 near/far (or radius) cull + sortable f32 depth key
-//                 (src/render/camera.ts:56-99, src/spatial/radix-sort.ts:20-24)
-//   depth sort   stable LSD radix over (key, splat id)  (radix-sort.ts:65-117)
-//   k_project    EWA projection, SH colour, tile AABB count (project.ts:73-383)
-//   scan         exclusive scan of counts (prefix-sum.ts:22-86)
-//   k_emit       (tile, splat) pairs in row-major tile order (emit-pairs.ts:27-56)
-//   pair sort    stable LSD radix by tile id (pipeline.ts:544-578)
-//   k_ranges     per-tile [start, end) (find-boundaries.ts:17-99)
-//   k_rasterize  front-to-back compositing + finalize (rasterize.ts:36-82,
-//                 finalize.ts:29-42)
-// All sizes stay on the device except the visible count and the pair count,
-// which the host reads to size launches.
+//   k_cull_project  one coalesced pass over the scene in splat order: the
+//                   f64 near/far (or radius) cull and sortable f32 depth key
+//                   (src/render/camera.ts:56-99, src/spatial/radix-sort.ts:20-24)
+//                   and, for visible splats, the EWA projection, SH colour and
+//                   tile count (src/render/shaders/project.ts:73-383)
+//   depth sort      stable LSD radix over (key, splat id) (radix-sort.ts:65-117)
+//   binning         per-tile lists of splat ids in depth order: span_bin
+//                   (binning.hip), or k_emit + stable radix sort + ranges
+//                   (emit-pairs.ts:27-56, pipeline.ts:544-578, find-boundaries.ts:17-99)
+//   k_rasterize     front-to-back compositing + finalize (rasterize.ts:36-82,
+//                   finalize.ts:29-42)
+// Projected records stay in splat-id order; the per-tile lists carry splat
+// ids, whose order inside a tile is the depth order. The reference's pair
+// values (chunk-local depth ranks) are the inverse permutation of these ids.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +22,7 @@
 #include <cstring>
 #include <vector>
 
+#include "binning.h"
 #include "dl_common.h"
 #include "dl_internal.h"
 #include "render.h"
@@ -28,56 +31,14 @@ namespace dl {
 
 namespace {
 
-constexpr int kCullBlock = 256;
-constexpr int kProjBlock = 128;
+constexpr int kProjBlock = 256;
 
 struct ViewF32 {
   float m[16];
 };
 
 // ---------------------------------------------------------------------------
-// cull + depth key (camera.ts:56-99): f64, no contraction, JS evaluation order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kCullBlock) void k_cull_key(const float *__restrict__ pos, uint32_t n, ViewF32 v,
-                                                         double near_plane, double far_plane, uint32_t proj,
-                                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
-                                                         uint32_t *__restrict__ d_nvis) {
-  const uint32_t i = blockIdx.x * kCullBlock + threadIdx.x;
-  bool vis = false;
-  if (i < n) {
-    const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
-    float depth_f = 0.0f;
-    if (proj == DL_PINHOLE) {
-      double cz = (double)v.m[2] * x + (double)v.m[6] * y;
-      cz = cz + (double)v.m[10] * z;
-      cz = cz + (double)v.m[14];
-      const double depth = -cz;
-      vis = depth > near_plane && depth < far_plane;
-      depth_f = (float)depth;
-    } else {
-      double cx = (double)v.m[0] * x + (double)v.m[4] * y;
-      cx = cx + (double)v.m[8] * z;
-      cx = cx + (double)v.m[12];
-      double cy = (double)v.m[1] * x + (double)v.m[5] * y;
-      cy = cy + (double)v.m[9] * z;
-      cy = cy + (double)v.m[13];
-      double cz = (double)v.m[2] * x + (double)v.m[6] * y;
-      cz = cz + (double)v.m[10] * z;
-      cz = cz + (double)v.m[14];
-      double r2 = cx * cx + cy * cy;
-      r2 = r2 + cz * cz;
-      vis = r2 > near_plane * near_plane;
-      depth_f = (float)__builtin_sqrt(r2);
-    }
-    keys[i] = vis ? sortable_key(depth_f) : kInvisibleKey;
-    ids[i] = i;
-  }
-  const uint64_t bal = __ballot(vis);
-  if (__lane_id() == 0 && bal) atomicAdd(d_nvis, (uint32_t)__popcll(bal));
-}
-
-// ---------------------------------------------------------------------------
-// project (project.ts:73-383) for splat id = sorted[j]
+// project (project.ts:73-383)
 // ---------------------------------------------------------------------------
 __device__ inline void quat_rot(const float q[4], float v0, float v1, float v2, float o[3]) {
   const float c0 = q[2] * v2 - q[3] * v1;
@@ -141,14 +102,15 @@ __device__ inline void eval_sh(const float *__restrict__ sh, float x, float y, f
   out[2] = b;
 }
 
+// One splat through project.ts:111-285. Returns the tile count; a culled
+// splat gets slot 0 = (0, 0, -1, 0) and count 0 (project.ts:216-219, 247-250).
 template <int BANDS>
-__global__ __launch_bounds__(kProjBlock) void k_project(SceneView sc, const uint32_t *__restrict__ sorted,
-                                                        uint32_t j0, uint32_t cn, ProjUniforms u,
-                                                        float4 *__restrict__ rec, uint32_t *__restrict__ counts) {
-  const uint32_t jj = blockIdx.x * kProjBlock + threadIdx.x;
-  if (jj >= cn) return;
-  const uint32_t id = sorted[j0 + jj];
+__device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const ProjUniforms &u, float4 &o0,
+                                         float4 &o1, float4 &o2) {
   const float *m = u.view;
+  o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+  o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float px = sc.pos[3 * (size_t)id], py = sc.pos[3 * (size_t)id + 1], pz = sc.pos[3 * (size_t)id + 2];
   const float4 qv = reinterpret_cast<const float4 *>(sc.rot)[id];
   const float q[4] = {qv.x, qv.y, qv.z, qv.w};
@@ -162,144 +124,243 @@ __global__ __launch_bounds__(kProjBlock) void k_project(SceneView sc, const uint
   const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
   const float depth = -pcz;
   float sx, sy;
-  bool culled = false;
   if (u.proj == DL_PINHOLE) {
-    culled = depth <= u.near_ || depth >= u.far_;
+    if (depth <= u.near_ || depth >= u.far_) return 0;
     const float proj_x = (pcx * u.focal) / (-pcz);
     const float proj_y = ((-pcy) * u.focal) / (-pcz);
     sx = proj_x + u.cx;
     sy = proj_y + u.cy;
   } else {
     const float rr = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
-    culled = rr <= u.near_;
+    if (rr <= u.near_) return 0;
     const float lon = (float)atan2((double)pcx, (double)(-pcz));
     const float lat = (float)asin((double)(pcy / rr));
     sx = (lon / kTwoPi + 0.5f) * u.width;
     sy = (0.5f - lat / kPi) * u.height;
   }
-  float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
-  float4 o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float4 o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float rx[3], ry[3], rz[3];
+  quat_rot(q, 1.0f, 0.0f, 0.0f, rx);
+  quat_rot(q, 0.0f, 1.0f, 0.0f, ry);
+  quat_rot(q, 0.0f, 0.0f, 1.0f, rz);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    rx[k] = rx[k] * s0;
+    ry[k] = ry[k] * s1;
+    rz[k] = rz[k] * s2;
+  }
+  const float c00 = (rx[0] * rx[0] + ry[0] * ry[0]) + rz[0] * rz[0];
+  const float c01 = (rx[0] * rx[1] + ry[0] * ry[1]) + rz[0] * rz[1];
+  const float c02 = (rx[0] * rx[2] + ry[0] * ry[2]) + rz[0] * rz[2];
+  const float c11 = (rx[1] * rx[1] + ry[1] * ry[1]) + rz[1] * rz[1];
+  const float c12 = (rx[1] * rx[2] + ry[1] * ry[2]) + rz[1] * rz[2];
+  const float c22 = (rx[2] * rx[2] + ry[2] * ry[2]) + rz[2] * rz[2];
+  const float v00 = m[0], v01 = m[4], v02 = m[8];
+  const float v10 = m[1], v11 = m[5], v12 = m[9];
+  const float v20 = m[2], v21 = m[6], v22 = m[10];
+  const float a00 = (v00 * c00 + v01 * c01) + v02 * c02;
+  const float a01 = (v00 * c01 + v01 * c11) + v02 * c12;
+  const float a02 = (v00 * c02 + v01 * c12) + v02 * c22;
+  const float a10 = (v10 * c00 + v11 * c01) + v12 * c02;
+  const float a11 = (v10 * c01 + v11 * c11) + v12 * c12;
+  const float a12 = (v10 * c02 + v11 * c12) + v12 * c22;
+  const float a20 = (v20 * c00 + v21 * c01) + v22 * c02;
+  const float a21 = (v20 * c01 + v21 * c11) + v22 * c12;
+  const float a22 = (v20 * c02 + v21 * c12) + v22 * c22;
+  const float cc00 = (a00 * v00 + a01 * v01) + a02 * v02;
+  const float cc01 = (a00 * v10 + a01 * v11) + a02 * v12;
+  const float cc02 = (a00 * v20 + a01 * v21) + a02 * v22;
+  const float cc11 = (a10 * v10 + a11 * v11) + a12 * v12;
+  const float cc12 = (a10 * v20 + a11 * v21) + a12 * v22;
+  const float cc22 = (a20 * v20 + a21 * v21) + a22 * v22;
+  const float inv_z = 1.0f / pcz;
+  const float j00 = u.focal * inv_z;
+  const float j02 = (((-u.focal) * pcx) * inv_z) * inv_z;
+  const float j11 = u.focal * inv_z;
+  const float j12 = (((-u.focal) * pcy) * inv_z) * inv_z;
+  const float b00 = j00 * cc00 + j02 * cc02;
+  const float b01 = j00 * cc01 + j02 * cc12;
+  const float b02 = j00 * cc02 + j02 * cc22;
+  const float b11 = j11 * cc11 + j12 * cc12;
+  const float b12 = j11 * cc12 + j12 * cc22;
+  float cov00 = b00 * j00 + b02 * j02;
+  const float cov01 = b01 * j11 + b02 * j12;
+  float cov11 = b11 * j11 + b12 * j12;
+  const float det_orig = cov00 * cov11 - cov01 * cov01;
+  cov00 = cov00 + kAA;
+  cov11 = cov11 + kAA;
+  const float det = cov00 * cov11 - cov01 * cov01;
+  const float aa = __builtin_sqrtf(wmax(det_orig / det, 0.0f));
+  if (det <= kDetMin) return 0;
+  const float inv_det = 1.0f / det;
+  const float conic_a = cov11 * inv_det;
+  const float conic_b = (-cov01) * inv_det;
+  const float conic_c = cov00 * inv_det;
+  const float mid = 0.5f * (cov00 + cov11);
+  const float r_eig = __builtin_sqrtf(wmax(kEigMin, mid * mid - det));
+  const float lambda1 = mid + r_eig;
+  const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
+  const float fade_s = kFadeStart * u.height;
+  const float fade_e = kFadeEnd * u.height;
+  const float fade = wclamp((fade_e - radius_raw) / (fade_e - fade_s), 0.0f, 1.0f);
+  if (fade <= 0.0f) return 0;
+  const float radius = __builtin_ceilf(wmin(radius_raw, fade_e));
+  float col[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
+  if constexpr (BANDS > 0) {
+    constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+    const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+    float sh[3];
+    eval_sh<BANDS>(sc.sh + (size_t)id * 3 * M, pcx / len, pcy / len, pcz / len, sh);
+    col[0] = col[0] + sh[0];
+    col[1] = col[1] + sh[1];
+    col[2] = col[2] + sh[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
+  const float alpha = (dl_sigmoidf(logit) * fade) * aa;
+  o0 = make_float4(sx, sy, radius, depth);
+  o1 = make_float4(conic_a, conic_b, conic_c, alpha);
+  o2 = make_float4(col[0], col[1], col[2], 0.0f);
+  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
+  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
+  const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
+  const uint32_t max_ty = u32sat(wmin((float)u.tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
   uint32_t count = 0;
-  if (!culled) {
-    float rx[3], ry[3], rz[3];
-    quat_rot(q, 1.0f, 0.0f, 0.0f, rx);
-    quat_rot(q, 0.0f, 1.0f, 0.0f, ry);
-    quat_rot(q, 0.0f, 0.0f, 1.0f, rz);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      rx[k] = rx[k] * s0;
-      ry[k] = ry[k] * s1;
-      rz[k] = rz[k] * s2;
-    }
-    const float c00 = (rx[0] * rx[0] + ry[0] * ry[0]) + rz[0] * rz[0];
-    const float c01 = (rx[0] * rx[1] + ry[0] * ry[1]) + rz[0] * rz[1];
-    const float c02 = (rx[0] * rx[2] + ry[0] * ry[2]) + rz[0] * rz[2];
-    const float c11 = (rx[1] * rx[1] + ry[1] * ry[1]) + rz[1] * rz[1];
-    const float c12 = (rx[1] * rx[2] + ry[1] * ry[2]) + rz[1] * rz[2];
-    const float c22 = (rx[2] * rx[2] + ry[2] * ry[2]) + rz[2] * rz[2];
-    const float v00 = m[0], v01 = m[4], v02 = m[8];
-    const float v10 = m[1], v11 = m[5], v12 = m[9];
-    const float v20 = m[2], v21 = m[6], v22 = m[10];
-    const float a00 = (v00 * c00 + v01 * c01) + v02 * c02;
-    const float a01 = (v00 * c01 + v01 * c11) + v02 * c12;
-    const float a02 = (v00 * c02 + v01 * c12) + v02 * c22;
-    const float a10 = (v10 * c00 + v11 * c01) + v12 * c02;
-    const float a11 = (v10 * c01 + v11 * c11) + v12 * c12;
-    const float a12 = (v10 * c02 + v11 * c12) + v12 * c22;
-    const float a20 = (v20 * c00 + v21 * c01) + v22 * c02;
-    const float a21 = (v20 * c01 + v21 * c11) + v22 * c12;
-    const float a22 = (v20 * c02 + v21 * c12) + v22 * c22;
-    const float cc00 = (a00 * v00 + a01 * v01) + a02 * v02;
-    const float cc01 = (a00 * v10 + a01 * v11) + a02 * v12;
-    const float cc02 = (a00 * v20 + a01 * v21) + a02 * v22;
-    const float cc11 = (a10 * v10 + a11 * v11) + a12 * v12;
-    const float cc12 = (a10 * v20 + a11 * v21) + a12 * v22;
-    const float cc22 = (a20 * v20 + a21 * v21) + a22 * v22;
-    const float inv_z = 1.0f / pcz;
-    const float j00 = u.focal * inv_z;
-    const float j02 = (((-u.focal) * pcx) * inv_z) * inv_z;
-    const float j11 = u.focal * inv_z;
-    const float j12 = (((-u.focal) * pcy) * inv_z) * inv_z;
-    const float b00 = j00 * cc00 + j02 * cc02;
-    const float b01 = j00 * cc01 + j02 * cc12;
-    const float b02 = j00 * cc02 + j02 * cc22;
-    const float b11 = j11 * cc11 + j12 * cc12;
-    const float b12 = j11 * cc12 + j12 * cc22;
-    float cov00 = b00 * j00 + b02 * j02;
-    const float cov01 = b01 * j11 + b02 * j12;
-    float cov11 = b11 * j11 + b12 * j12;
-    const float det_orig = cov00 * cov11 - cov01 * cov01;
-    cov00 = cov00 + kAA;
-    cov11 = cov11 + kAA;
-    const float det = cov00 * cov11 - cov01 * cov01;
-    const float aa = __builtin_sqrtf(wmax(det_orig / det, 0.0f));
-    if (det <= kDetMin) {
-      culled = true;
+  if (max_tx >= min_tx && max_ty >= min_ty) {
+    count = (max_tx - min_tx + 1u) * (max_ty - min_ty + 1u);
+    if (count > u.max_cov) count = u.max_cov;
+  }
+  return count;
+}
+
+// f64 cull (camera.ts:56-99) + key, then project() for visible splats; one
+// thread per splat in id order, so every scene array is read coalesced.
+// Per-block partial sums of visible splats and tile counts replace any
+// same-address atomics.
+template <int BANDS>
+__global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjUniforms u, double near_plane,
+                                                             double far_plane, uint32_t *__restrict__ keys,
+                                                             uint32_t *__restrict__ ids, float4 *__restrict__ rec,
+                                                             uint32_t *__restrict__ counts,
+                                                             uint32_t *__restrict__ part_vis,
+                                                             unsigned long long *__restrict__ part_pairs) {
+  __shared__ uint32_t s_vis[kProjBlock / 64];
+  __shared__ unsigned long long s_pairs[kProjBlock / 64];
+  const uint32_t i = blockIdx.x * kProjBlock + threadIdx.x;
+  bool vis = false;
+  uint32_t count = 0;
+  if (i < sc.count) {
+    const float *m = u.view;
+    const double x = sc.pos[3 * (size_t)i], y = sc.pos[3 * (size_t)i + 1], z = sc.pos[3 * (size_t)i + 2];
+    float depth_f;
+    if (u.proj == DL_PINHOLE) {
+      double cz = (double)m[2] * x + (double)m[6] * y;
+      cz = cz + (double)m[10] * z;
+      cz = cz + (double)m[14];
+      const double depth = -cz;
+      vis = depth > near_plane && depth < far_plane;
+      depth_f = (float)depth;
     } else {
-      const float inv_det = 1.0f / det;
-      const float conic_a = cov11 * inv_det;
-      const float conic_b = (-cov01) * inv_det;
-      const float conic_c = cov00 * inv_det;
-      const float mid = 0.5f * (cov00 + cov11);
-      const float r_eig = __builtin_sqrtf(wmax(kEigMin, mid * mid - det));
-      const float lambda1 = mid + r_eig;
-      const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
-      const float fade_s = kFadeStart * u.height;
-      const float fade_e = kFadeEnd * u.height;
-      const float fade = wclamp((fade_e - radius_raw) / (fade_e - fade_s), 0.0f, 1.0f);
-      if (fade <= 0.0f) {
-        culled = true;
-      } else {
-        const float radius = __builtin_ceilf(wmin(radius_raw, fade_e));
-        float col[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
-        if constexpr (BANDS > 0) {
-          constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
-          const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
-          float sh[3];
-          eval_sh<BANDS>(sc.sh + (size_t)id * 3 * M, pcx / len, pcy / len, pcz / len, sh);
-          col[0] = col[0] + sh[0];
-          col[1] = col[1] + sh[1];
-          col[2] = col[2] + sh[2];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
-        const float alpha = (dl_sigmoidf(logit) * fade) * aa;
-        o0 = make_float4(sx, sy, radius, depth);
-        o1 = make_float4(conic_a, conic_b, conic_c, alpha);
-        o2 = make_float4(col[0], col[1], col[2], 0.0f);
-        const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
-        const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
-        const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
-        const uint32_t max_ty = u32sat(wmin((float)u.tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
-        if (max_tx >= min_tx && max_ty >= min_ty) {
-          count = (max_tx - min_tx + 1u) * (max_ty - min_ty + 1u);
-          if (count > u.max_cov) count = u.max_cov;
-        }
-      }
+      double cx = (double)m[0] * x + (double)m[4] * y;
+      cx = cx + (double)m[8] * z;
+      cx = cx + (double)m[12];
+      double cy = (double)m[1] * x + (double)m[5] * y;
+      cy = cy + (double)m[9] * z;
+      cy = cy + (double)m[13];
+      double cz = (double)m[2] * x + (double)m[6] * y;
+      cz = cz + (double)m[10] * z;
+      cz = cz + (double)m[14];
+      double r2 = cx * cx + cy * cy;
+      r2 = r2 + cz * cz;
+      vis = r2 > near_plane * near_plane;
+      depth_f = (float)__builtin_sqrt(r2);
     }
+    keys[i] = vis ? sortable_key(depth_f) : kInvisibleKey;
+    ids[i] = i;
+    float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f), o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o2 = o1;
+    if (vis) count = project_splat<BANDS>(sc, i, u, o0, o1, o2);
+    rec[3 * (size_t)i + 0] = o0;
+    rec[3 * (size_t)i + 1] = o1;
+    rec[3 * (size_t)i + 2] = o2;
+    counts[i] = count;
   }
-  if (culled) {
-    o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
-    o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    count = 0;
+  uint32_t v = vis ? 1u : 0u;
+  unsigned long long pc = count;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    v += __shfl_xor(v, d, 64);
+    pc += __shfl_xor(pc, d, 64);
   }
-  rec[3 * (size_t)jj + 0] = o0;
-  rec[3 * (size_t)jj + 1] = o1;
-  rec[3 * (size_t)jj + 2] = o2;
-  counts[jj] = count;
+  if (__lane_id() == 0) {
+    s_vis[threadIdx.x >> 6] = v;
+    s_pairs[threadIdx.x >> 6] = pc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tv = 0;
+    unsigned long long tp = 0;
+#pragma unroll
+    for (int w = 0; w < kProjBlock / 64; w++) {
+      tv += s_vis[w];
+      tp += s_pairs[w];
+    }
+    part_vis[blockIdx.x] = tv;
+    part_pairs[blockIdx.x] = tp;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_totals(const uint32_t *__restrict__ part_vis,
+                                                        const unsigned long long *__restrict__ part_pairs,
+                                                        uint32_t nparts, uint32_t *__restrict__ d_nvis,
+                                                        unsigned long long *__restrict__ d_pairs) {
+  __shared__ uint32_t s_v[16];
+  __shared__ unsigned long long s_p[16];
+  uint32_t v = 0;
+  unsigned long long p = 0;
+  for (uint32_t k = threadIdx.x; k < nparts; k += 1024) {
+    v += part_vis[k];
+    p += part_pairs[k];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    v += __shfl_xor(v, d, 64);
+    p += __shfl_xor(p, d, 64);
+  }
+  if (__lane_id() == 0) {
+    s_v[threadIdx.x >> 6] = v;
+    s_p[threadIdx.x >> 6] = p;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tv = 0;
+    unsigned long long tp = 0;
+    for (int w = 0; w < 16; w++) {
+      tv += s_v[w];
+      tp += s_p[w];
+    }
+    *d_nvis = tv;
+    *d_pairs = tp;
+  }
+}
+
+// tile counts of one chunk in depth order
+__global__ __launch_bounds__(256) void k_gather_counts(const uint32_t *__restrict__ counts,
+                                                       const uint32_t *__restrict__ ids, uint32_t cn,
+                                                       uint32_t *__restrict__ out) {
+  const uint32_t jj = blockIdx.x * 256 + threadIdx.x;
+  if (jj < cn) out[jj] = counts[ids[jj]];
 }
 
 // ---------------------------------------------------------------------------
-// emit (emit-pairs.ts:27-56)
+// emit (emit-pairs.ts:27-56); values are splat ids
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, const uint32_t *__restrict__ offs,
-                                              uint32_t cn, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov,
-                                              uint32_t *__restrict__ pkeys, uint32_t *__restrict__ pvals) {
+__global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
+                                              const uint32_t *__restrict__ offs, uint32_t cn, uint32_t tiles_x,
+                                              uint32_t tiles_y, uint32_t max_cov, uint32_t *__restrict__ pkeys,
+                                              uint32_t *__restrict__ pvals) {
   const uint32_t jj = blockIdx.x * 128 + threadIdx.x;
   if (jj >= cn) return;
-  const float4 v0 = rec[3 * (size_t)jj];
+  const uint32_t id = ids[jj];
+  const float4 v0 = rec[3 * (size_t)id];
   const float sx = v0.x, sy = v0.y, radius = v0.z;
   if (radius <= 0.0f) return;
   const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
@@ -313,30 +374,11 @@ __global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, co
     for (uint32_t tx = min_tx; tx <= max_tx; tx++) {
       if (written >= max_cov) return;
       pkeys[off] = ty * tiles_x + tx;
-      pvals[off] = jj;
+      pvals[off] = id;
       off++;
       written++;
     }
   }
-}
-
-// ---------------------------------------------------------------------------
-// tile ranges: off[t] = lower_bound(keys, t), off[num_tiles] = n
-// (the result of find-boundaries.ts:17-99; every entry written once)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ keys, uint32_t n, uint32_t num_tiles,
-                                                uint32_t *__restrict__ off) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t cur = keys[i];
-  if (i == 0) {
-    for (uint32_t t = 0; t <= cur; t++) off[t] = 0;
-  } else {
-    const uint32_t prev = keys[i - 1];
-    for (uint32_t t = prev + 1; t <= cur; t++) off[t] = i;
-  }
-  if (i == n - 1)
-    for (uint32_t t = cur + 1; t <= num_tiles; t++) off[t] = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -361,10 +403,13 @@ __device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, cons
   return R | (G << 8) | (B << 16) | (A << 24);
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
                                                   const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
-                                                  uint32_t *__restrict__ out_rgba, RastParams p) {
+                                                  uint32_t *__restrict__ out_rgba, RastParams p,
+                                                  unsigned long long *__restrict__ d_iters) {
   __shared__ float4 s_rec[64 * 3];
+  uint32_t iters = 0;
   const uint32_t tile = blockIdx.x;
   const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
   const int lane = threadIdx.x;
@@ -406,6 +451,7 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           if (!live[k]) continue;
+          if (COUNT) iters++;
           const float dx = pxf - a.x;
           const float dy = pyf[k] - a.y;
           const float r = a.z;
@@ -432,6 +478,34 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
     if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
     if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }
+  if (COUNT) {
+    uint32_t s = iters;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) atomicAdd(d_iters, (unsigned long long)s);
+  }
+}
+
+void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float4 *rec, const uint32_t *off,
+                      const uint32_t *pvals, float4 *accum, uint32_t *out_rgba, const RastParams &p,
+                      unsigned long long *d_iters) {
+  if (count)
+    k_rasterize<true><<<num_tiles, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
+  else
+    k_rasterize<false><<<num_tiles, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
+  DL_HIP(hipGetLastError());
+}
+
+using CullProjFn = void (*)(SceneView, ProjUniforms, double, double, uint32_t *, uint32_t *, float4 *, uint32_t *,
+                            uint32_t *, unsigned long long *);
+
+CullProjFn cull_project_for(uint32_t bands) {
+  switch (bands) {
+    case 0: return k_cull_project<0>;
+    case 1: return k_cull_project<1>;
+    case 2: return k_cull_project<2>;
+    default: return k_cull_project<3>;
+  }
 }
 
 }  // namespace
@@ -439,15 +513,6 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static uint32_t bit_width_u32(uint32_t v) {
-  uint32_t b = 0;
-  while (v) {
-    b++;
-    v >>= 1;
-  }
-  return b;
-}
-
 void Renderer::ensure_events(size_t n) {
   while (events.size() < n) {
     hipEvent_t e;
@@ -465,11 +530,17 @@ void Renderer::mark(int stage) {
 }
 
 template <class T>
-static void tap_copy(std::vector<uint8_t> &dst, const T *dev, size_t count, hipStream_t s) {
-  const size_t old = dst.size();
-  dst.resize(old + count * sizeof(T));
-  if (count) DL_HIP(hipMemcpyAsync(dst.data() + old, dev, count * sizeof(T), hipMemcpyDeviceToHost, s));
+static void tap_append(std::vector<uint8_t> &dst, const T *src, size_t count) {
+  const uint8_t *b = reinterpret_cast<const uint8_t *>(src);
+  dst.insert(dst.end(), b, b + count * sizeof(T));
+}
+
+template <class T>
+static std::vector<T> to_host(const T *dev, size_t count, hipStream_t s) {
+  std::vector<T> h(count);
+  if (count) DL_HIP(hipMemcpyAsync(h.data(), dev, count * sizeof(T), hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
+  return h;
 }
 
 void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_background &bg) {
@@ -496,8 +567,14 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
 
   rgba.reserve(npx * 4);
   small.reserve(256);
+  // device scalars: [0] visible, [1] chunk pairs, [2..3] u64 loop iterations,
+  // [4..6] span-binning scalars, [8..9] u64 total pairs
   uint32_t *d_nvis = small.as<uint32_t>();
   uint32_t *d_total = d_nvis + 1;
+  unsigned long long *d_iters = reinterpret_cast<unsigned long long *>(d_nvis + 2);
+  uint32_t *d_bin = d_nvis + 4;
+  unsigned long long *d_pairs_all = reinterpret_cast<unsigned long long *>(d_nvis + 8);
+  if (counting) DL_HIP(hipMemsetAsync(d_iters, 0, sizeof(unsigned long long), s));
 
   RastParams rp{};
   rp.tiles_x = tiles_x;
@@ -508,23 +585,43 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rp.bg_b = bg.b;
   rp.bg_a = bg.a;
 
-  ViewF32 vf;
-  std::memcpy(vf.m, cam.view_matrix, sizeof(vf.m));
+  ProjUniforms u{};
+  std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
+  u.width = (float)W;
+  u.height = (float)H;
+  u.focal = (float)((double)H / (2.0 * std::tan(cam.fov_y / 2.0)));
+  u.cx = (float)((double)W / 2.0);
+  u.cy = (float)((double)H / 2.0);
+  u.near_ = (float)cam.near_plane;
+  u.far_ = (float)cam.far_plane;
+  u.tiles_x = tiles_x;
+  u.tiles_y = tiles_y;
+  u.max_cov = max_cov;
+  u.proj = cam.projection;
 
-  // ---- cull + depth keys + depth sort ----
+  // ---- cull + project (splat order), depth sort ----
   uint32_t nvis = 0;
+  unsigned long long pairs_all = 0;
   const uint32_t *sorted_ids = nullptr;
   if (n > 0) {
+    const uint32_t nparts = div_up(n, kProjBlock);
     keys.reserve((size_t)n * 4);
     ids.reserve((size_t)n * 4);
     tkeys.reserve((size_t)n * 4);
     tids.reserve((size_t)n * 4);
+    rec.reserve((size_t)n * 48);
+    counts.reserve((size_t)n * 4);
+    parts.reserve((size_t)nparts * 12 + 256);
     sort_ws.reserve(radix_workspace_bytes(n, 32));
+    uint32_t *part_vis = parts.as<uint32_t>();
+    unsigned long long *part_pairs =
+        reinterpret_cast<unsigned long long *>(parts.as<uint8_t>() + (((size_t)nparts * 4 + 255) & ~(size_t)255));
     mark(DL_STAGE_CULL);
-    DL_HIP(hipMemsetAsync(d_nvis, 0, sizeof(uint32_t), s));
-    k_cull_key<<<div_up(n, kCullBlock), kCullBlock, 0, s>>>(sc.pos, n, vf, cam.near_plane, cam.far_plane,
-                                                             cam.projection, keys.as<uint32_t>(), ids.as<uint32_t>(),
-                                                             d_nvis);
+    cull_project_for(sc.bands)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
+                                                             keys.as<uint32_t>(), ids.as<uint32_t>(), rec.as<float4>(),
+                                                             counts.as<uint32_t>(), part_vis, part_pairs);
+    DL_HIP(hipGetLastError());
+    k_reduce_totals<<<1, 1024, 0, s>>>(part_vis, part_pairs, nparts, d_nvis, d_pairs_all);
     DL_HIP(hipGetLastError());
     mark(DL_STAGE_DEPTH_SORT);
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
@@ -532,10 +629,27 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
     const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
     DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
+    // every per-chunk u32 scan below sums a subset of these pairs
+    if (pairs_all > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (tile, splat) pairs in one frame");
     if (keep_taps) {
-      tap_copy(taps[DL_TAP_SORTED_INDICES], sorted_ids, nvis, s);
-      tap_copy(taps[DL_TAP_DEPTH_KEYS], sorted_keys, nvis, s);
+      auto sid = to_host(sorted_ids, nvis, s);
+      tap_append(taps[DL_TAP_SORTED_INDICES], sid.data(), nvis);
+      auto sk = to_host(sorted_keys, nvis, s);
+      tap_append(taps[DL_TAP_DEPTH_KEYS], sk.data(), nvis);
+      auto rh = to_host(rec.as<float>(), (size_t)n * 12, s);
+      auto ch = to_host(counts.as<uint32_t>(), n, s);
+      std::vector<float> pr((size_t)nvis * 12);
+      std::vector<uint32_t> pc(nvis);
+      for (uint32_t j = 0; j < nvis; j++) {
+        std::memcpy(&pr[(size_t)j * 12], &rh[(size_t)sid[j] * 12], 48);
+        pc[j] = ch[sid[j]];
+      }
+      tap_append(taps[DL_TAP_PROJECTED], pr.data(), pr.size());
+      tap_append(taps[DL_TAP_TILE_COUNTS], pc.data(), pc.size());
+      tap_rank.assign(n, 0xFFFFFFFFu);
+      for (uint32_t j = 0; j < nvis; j++) tap_rank[sid[j]] = j;
     }
   }
   stats.num_visible = nvis;
@@ -550,55 +664,38 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     rp.write_accum = need_accum;
     if (need_accum) accum.reserve(npx * 16);
     mark(DL_STAGE_RASTERIZE);
-    k_rasterize<<<num_tiles, 64, 0, s>>>(nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp);
-    DL_HIP(hipGetLastError());
+    launch_rasterize(counting, num_tiles, s, nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp,
+                     d_iters);
   } else {
     const uint32_t chunk_cap = (opts.chunk_cap && opts.chunk_cap < nvis) ? opts.chunk_cap : nvis;
-    rec.reserve((size_t)chunk_cap * 48);
-    counts.reserve((size_t)chunk_cap * 4);
-    offsets.reserve((size_t)chunk_cap * 4);
-    scan_ws.reserve(scan_workspace_bytes(chunk_cap));
-    ranges.reserve(((size_t)num_tiles + 1) * 4);
     const uint32_t chunks = div_up(nvis, chunk_cap);
+    const bool span = opts.binning == 0 && span_binning_supported(tiles_x, tiles_y);
+    ranges.reserve(((size_t)num_tiles + 1) * 4);
     if (chunks > 1 || need_accum) accum.reserve(npx * 16);
-    ProjUniforms u{};
-    std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
-    u.width = (float)W;
-    u.height = (float)H;
-    u.focal = (float)((double)H / (2.0 * std::tan(cam.fov_y / 2.0)));
-    u.cx = (float)((double)W / 2.0);
-    u.cy = (float)((double)H / 2.0);
-    u.near_ = (float)cam.near_plane;
-    u.far_ = (float)cam.far_plane;
-    u.tiles_x = tiles_x;
-    u.tiles_y = tiles_y;
-    u.max_cov = max_cov;
-    u.proj = cam.projection;
     const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(num_tiles - 1));
     stats.pair_key_bits = key_bits;
     for (uint32_t c0 = 0; c0 < nvis; c0 += chunk_cap) {
       const uint32_t cn = std::min(chunk_cap, nvis - c0);
       const bool last = c0 + cn == nvis;
+      const uint32_t *cids = sorted_ids + c0;
       stats.num_chunks++;
-      mark(DL_STAGE_PROJECT);
-      switch (sc.bands) {
-        case 0: k_project<0><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
-        case 1: k_project<1><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
-        case 2: k_project<2><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
-        default: k_project<3><<<div_up(cn, kProjBlock), kProjBlock, 0, s>>>(sc, sorted_ids, c0, cn, u, rec.as<float4>(), counts.as<uint32_t>()); break;
-      }
-      DL_HIP(hipGetLastError());
+      // ---- pair count of this chunk (and scanned offsets for emit) ----
       mark(DL_STAGE_SCAN);
-      exclusive_scan_u32(s, counts.as<uint32_t>(), offsets.as<uint32_t>(), cn, d_total, scan_ws.get());
-      uint32_t npairs = 0;
-      DL_HIP(hipMemcpyAsync(&npairs, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-      DL_HIP(hipStreamSynchronize(s));
-      if (keep_taps) {
-        const size_t old = taps[DL_TAP_PROJECTED].size();
-        taps[DL_TAP_PROJECTED].resize(old + (size_t)cn * 48);
-        DL_HIP(hipMemcpyAsync(taps[DL_TAP_PROJECTED].data() + old, rec.get(), (size_t)cn * 48, hipMemcpyDeviceToHost, s));
-        tap_copy(taps[DL_TAP_TILE_COUNTS], counts.as<uint32_t>(), cn, s);
+      uint64_t npairs64 = pairs_all;
+      if (chunks > 1 || !span) {
+        chunk_counts.reserve((size_t)cn * 4);
+        offsets.reserve((size_t)cn * 4);
+        scan_ws.reserve(scan_workspace_bytes(cn));
+        k_gather_counts<<<div_up(cn, 256), 256, 0, s>>>(counts.as<uint32_t>(), cids, cn, chunk_counts.as<uint32_t>());
+        DL_HIP(hipGetLastError());
+        exclusive_scan_u32(s, chunk_counts.as<uint32_t>(), offsets.as<uint32_t>(), cn, d_total, scan_ws.get());
+        uint32_t t32 = 0;
+        DL_HIP(hipMemcpyAsync(&t32, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        DL_HIP(hipStreamSynchronize(s));
+        npairs64 = t32;
       }
+      if (npairs64 > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (tile, splat) pairs in one chunk");
+      const uint32_t npairs = (uint32_t)npairs64;
       stats.num_pairs += npairs;
       rp.resume = accum_valid;
       rp.finalize = last;
@@ -607,50 +704,66 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         // pipeline.ts:306: nothing to rasterize in this chunk
         if (keep_taps) {
           std::vector<uint32_t> z(num_tiles + 1, 0u);
-          auto &t = taps[DL_TAP_TILE_OFFSETS];
-          t.insert(t.end(), reinterpret_cast<uint8_t *>(z.data()), reinterpret_cast<uint8_t *>(z.data() + z.size()));
+          tap_append(taps[DL_TAP_TILE_OFFSETS], z.data(), z.size());
         }
         if (last) {
           rp.has_pairs = 0;
           mark(DL_STAGE_RASTERIZE);
-          k_rasterize<<<num_tiles, 64, 0, s>>>(nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp);
-          DL_HIP(hipGetLastError());
+          launch_rasterize(counting, num_tiles, s, nullptr, nullptr, nullptr, accum.as<float4>(),
+                           rgba.as<uint32_t>(), rp, d_iters);
         }
         continue;
       }
-      pkeys.reserve((size_t)npairs * 4);
       pvals.reserve((size_t)npairs * 4);
-      ptkeys.reserve((size_t)npairs * 4);
-      ptvals.reserve((size_t)npairs * 4);
-      psort_ws.reserve(radix_workspace_bytes(npairs, key_bits));
-      mark(DL_STAGE_EMIT);
-      k_emit<<<div_up(cn, 128), 128, 0, s>>>(rec.as<float4>(), offsets.as<uint32_t>(), cn, tiles_x, tiles_y, max_cov,
-                                             pkeys.as<uint32_t>(), pvals.as<uint32_t>());
-      DL_HIP(hipGetLastError());
-      mark(DL_STAGE_PAIR_SORT);
-      const bool in_tmp = radix_sort_pairs(s, pkeys.as<uint32_t>(), pvals.as<uint32_t>(), ptkeys.as<uint32_t>(),
-                                           ptvals.as<uint32_t>(), npairs, (int)key_bits, psort_ws.get());
-      const uint32_t *skeys = in_tmp ? ptkeys.as<uint32_t>() : pkeys.as<uint32_t>();
-      const uint32_t *svals = in_tmp ? ptvals.as<uint32_t>() : pvals.as<uint32_t>();
-      mark(DL_STAGE_RANGES);
-      k_ranges<<<div_up(npairs, 256), 256, 0, s>>>(skeys, npairs, num_tiles, ranges.as<uint32_t>());
-      DL_HIP(hipGetLastError());
+      const uint32_t *svals = pvals.as<uint32_t>();
+      if (span) {
+        // stable two-level bucketing straight into per-tile lists (binning.hip)
+        mark(DL_STAGE_EMIT);
+        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, bin_ws,
+                 ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin);
+      } else {
+        pkeys.reserve((size_t)npairs * 4);
+        ptkeys.reserve((size_t)npairs * 4);
+        ptvals.reserve((size_t)npairs * 4);
+        psort_ws.reserve(radix_workspace_bytes(npairs, key_bits));
+        mark(DL_STAGE_EMIT);
+        k_emit<<<div_up(cn, 128), 128, 0, s>>>(rec.as<float4>(), cids, offsets.as<uint32_t>(), cn, tiles_x, tiles_y,
+                                               max_cov, pkeys.as<uint32_t>(), pvals.as<uint32_t>());
+        DL_HIP(hipGetLastError());
+        mark(DL_STAGE_PAIR_SORT);
+        const bool in_tmp = radix_sort_pairs(s, pkeys.as<uint32_t>(), pvals.as<uint32_t>(), ptkeys.as<uint32_t>(),
+                                             ptvals.as<uint32_t>(), npairs, (int)key_bits, psort_ws.get());
+        const uint32_t *skeys = in_tmp ? ptkeys.as<uint32_t>() : pkeys.as<uint32_t>();
+        svals = in_tmp ? ptvals.as<uint32_t>() : pvals.as<uint32_t>();
+        mark(DL_STAGE_RANGES);
+        bin_ranges(s, skeys, npairs, num_tiles, ranges.as<uint32_t>());
+      }
       if (keep_taps) {
-        tap_copy(taps[DL_TAP_PAIR_TILES], skeys, npairs, s);
-        tap_copy(taps[DL_TAP_PAIR_SPLATS], svals, npairs, s);
-        tap_copy(taps[DL_TAP_TILE_OFFSETS], ranges.as<uint32_t>(), (size_t)num_tiles + 1, s);
+        // reference view: sorted tile ids and chunk-local depth ranks
+        auto off = to_host(ranges.as<uint32_t>(), (size_t)num_tiles + 1, s);
+        auto vals = to_host(svals, npairs, s);
+        std::vector<uint32_t> keys_h(npairs);
+        for (uint32_t t = 0; t < num_tiles; t++)
+          for (uint32_t k = off[t]; k < off[t + 1]; k++) keys_h[k] = t;
+        for (auto &v : vals) v = tap_rank[v] - c0;
+        tap_append(taps[DL_TAP_PAIR_TILES], keys_h.data(), keys_h.size());
+        tap_append(taps[DL_TAP_PAIR_SPLATS], vals.data(), vals.size());
+        tap_append(taps[DL_TAP_TILE_OFFSETS], off.data(), off.size());
       }
       rp.has_pairs = 1;
       mark(DL_STAGE_RASTERIZE);
-      k_rasterize<<<num_tiles, 64, 0, s>>>(rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
-                                           rgba.as<uint32_t>(), rp);
-      DL_HIP(hipGetLastError());
+      launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
+                       rgba.as<uint32_t>(), rp, d_iters);
       accum_valid = true;
     }
   }
   mark(-1);
+  if (counting) DL_HIP(hipMemcpyAsync(&stats.pixel_iters, d_iters, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
-  if (keep_taps) tap_copy(taps[DL_TAP_ACCUM], accum.as<float>(), npx * 4, s);
+  if (keep_taps) {
+    auto acc = to_host(accum.as<float>(), npx * 4, s);
+    tap_append(taps[DL_TAP_ACCUM], acc.data(), acc.size());
+  }
   if (timing && ev_used > 1) {
     for (size_t k = 0; k + 1 < ev_used; k++) {
       float ms = 0.0f;

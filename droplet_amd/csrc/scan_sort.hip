@@ -233,6 +233,28 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   }
 }
 
+// off[b] = lower_bound(keys, b) for b < num_bins, off[num_bins] = n; every
+// entry is written exactly once (keys sorted, all < num_bins, n >= 1).
+__global__ __launch_bounds__(256) void k_bin_ranges(const uint32_t *__restrict__ keys, uint32_t n,
+                                                    uint32_t num_bins, uint32_t *__restrict__ off) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t cur = keys[i];
+  if (i == 0) {
+    for (uint32_t t = 0; t <= cur; t++) off[t] = 0;
+  } else {
+    const uint32_t prev = keys[i - 1];
+    for (uint32_t t = prev + 1; t <= cur; t++) off[t] = i;
+  }
+  if (i == n - 1)
+    for (uint32_t t = cur + 1; t <= num_bins; t++) off[t] = n;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
                         const uint32_t *, uint32_t);
 
@@ -296,6 +318,15 @@ size_t scan_workspace_bytes(size_t n) { return scan_levels_bytes(n); }
 void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n, uint32_t *d_total,
                         void *ws) {
   scan_rec(s, d_in, d_out, n, d_total, static_cast<uint8_t *>(ws));
+}
+
+void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_bins, uint32_t *d_off) {
+  if (n == 0) {
+    k_fill_u32<<<div_up((size_t)num_bins + 1, 256), 256, 0, s>>>(d_off, (size_t)num_bins + 1, 0u);
+  } else {
+    k_bin_ranges<<<div_up(n, 256), 256, 0, s>>>(d_keys, n, num_bins, d_off);
+  }
+  DL_HIP(hipGetLastError());
 }
 
 size_t radix_workspace_bytes(size_t n, int key_bits) {

@@ -32,6 +32,7 @@ class RendererOptions:
 
     chunk_cap: Optional[int] = None
     max_coverage_per_splat: Optional[int] = None
+    binning: str = "span"   # "span" (two-level bucketing) or "sort" (emit + radix sort)
 
 
 class Device:
@@ -104,6 +105,9 @@ class Renderer:
         o = L.DlRendererOpts()
         o.chunk_cap = int(opts.chunk_cap or 0)
         o.max_coverage_per_splat = int(opts.max_coverage_per_splat or 0)
+        if opts.binning not in ("span", "sort"):
+            raise ValueError("binning must be 'span' or 'sort'")
+        o.binning = 0 if opts.binning == "span" else 1
         h = C.c_void_p()
         L.check(L.load().dl_renderer_create(device.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -147,6 +151,9 @@ class Renderer:
     def set_timing(self, enabled: bool) -> None:
         L.check(L.load().dl_renderer_set_timing(self.handle, int(enabled)))
 
+    def set_counting(self, enabled: bool) -> None:
+        L.check(L.load().dl_renderer_set_counting(self.handle, int(enabled)))
+
     def set_taps(self, enabled: bool) -> None:
         L.check(L.load().dl_renderer_set_taps(self.handle, int(enabled)))
 
@@ -158,7 +165,7 @@ class Renderer:
             "tiles_x": st.tiles_x, "tiles_y": st.tiles_y, "pair_key_bits": st.pair_key_bits,
             "num_pairs": int(st.num_pairs),
             "stage_ms": {name: float(st.stage_ms[i]) for i, name in enumerate(L.STAGES)},
-            "total_ms": float(st.total_ms),
+            "total_ms": float(st.total_ms), "pixel_iters": int(st.pixel_iters),
         }
 
     def tap(self, name: str) -> np.ndarray:

@@ -82,6 +82,10 @@ typedef struct { int device_index; } dl_device_opts;
 typedef struct {
     uint32_t chunk_cap;
     uint32_t max_coverage_per_splat;
+    /* 0: two-level span binning straight into per-tile lists (grids up to
+     * 256x1024 tiles; wider grids fall back to 1); 1: emit + stable radix
+     * sort of (tile, splat) pairs. Both give identical lists and images. */
+    uint32_t binning;
 } dl_renderer_opts;
 
 /* ---- device (replaces createDevice, src/gpu/device.ts:27-41) ---- */
@@ -128,8 +132,12 @@ typedef struct {
     uint64_t num_pairs;          /* total over chunks */
     float stage_ms[DL_NUM_STAGES]; /* hipEvent times on the renderer's stream (timing enabled) */
     float total_ms;
+    uint64_t pixel_iters;        /* rasterize loop iterations over all pixels (counting enabled) */
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
+/* Counts rasterize loop iterations (the work unit of the rasterize roofline)
+ * with an instrumented kernel variant; meant for one untimed frame. */
+int dl_renderer_set_counting(dl_renderer *r, int enabled);
 int dl_renderer_stats(dl_renderer *r, dl_render_stats *out);
 
 /* Debug taps for parity tests: when enabled, the next render keeps copies of

@@ -167,3 +167,27 @@ def test_max_coverage_truncation_parity(device):
         assert_render_parity(r, data, cam, max_coverage=7)
     finally:
         r.destroy()
+
+
+@pytest.mark.parametrize("binning", ["span", "sort"])
+def test_binning_paths_match_oracle(device, binning):
+    r = Renderer(device, RendererOptions(binning=binning))
+    try:
+        data, cam = S.plain_render_workload(seed=12)
+        assert_render_parity(r, data, cam)
+        assert_render_parity(r, S.dense_cluster(20_000), S.default_camera(256, 256))
+        # maxCoverage truncation leaves a partial last row of tiles
+        r2 = Renderer(device, RendererOptions(binning=binning, max_coverage_per_splat=5))
+        try:
+            assert_render_parity(r2, S.gaussian_cloud(5000, seed=13, log_scale_range=(-4.0, -2.0)),
+                                 S.default_camera(300, 200), max_coverage=5)
+        finally:
+            r2.destroy()
+    finally:
+        r.destroy()
+
+
+def test_wide_grid_falls_back_to_sort(renderer):
+    # 4400 px = 275 tile columns: above the span path's 256-column LDS limit
+    data = S.gaussian_cloud(20_000, seed=14)
+    assert_render_parity(renderer, data, S.default_camera(4400, 64))

@@ -1,0 +1,124 @@
+"""Pins the oracle's voxelizer on the CPU.
+
+* a single isotropic Gaussian: the solid voxels are exactly those where
+  1 - exp(-o e^{-d^2/2s^2}) >= cutoff, d = distance from the centre to the
+  closest point of the voxel cube (f64 closed form, apart from voxels whose
+  alpha sits within 1e-5 of the cutoff);
+* AABB extents against R diag(s) in f64 (aabb.ts:22-62);
+* BVH candidate lists are complete: the sum of batch list lengths equals the
+  brute-force count of (batch box, Gaussian AABB) overlaps;
+* the BVH order is a permutation of all Gaussians;
+* empty scene: 0 blocks, numBlocks = -Infinity (voxelize.ts:74-76 on
+  +/-Infinity bounds);
+* the browser test page voxel cases (examples/browser/test.ts:253-274).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from droplet_amd import synthetic as S
+from droplet_amd.types import SplatData
+
+
+def _sig(x):
+    return 1 / (1 + math.exp(-x))
+
+
+def _decode(res, nbx, nby):
+    out = {}
+    for b, lo, hi in zip(res["block_idx"], res["mask_lo"], res["mask_hi"]):
+        out[int(b)] = (int(lo) | (int(hi) << 32))
+    return out
+
+
+@pytest.mark.parametrize("ls,logit,cut", [(-2.5, 2.0, 0.1), (-3.0, 0.0, 0.3), (-2.0, -1.0, 0.05)])
+def test_single_gaussian_ball_closed_form(ls, logit, cut):
+    p = np.array([0.113, -0.071, 0.052], np.float32)
+    data = SplatData(1, p, [1, 0, 0, 0], [ls] * 3, [logit], [0, 0, 0])
+    res_v = 0.02
+    r = oracle.voxelize(data, voxel_resolution=res_v, opacity_cutoff=cut, near_eps=1e-5)
+    nb = [int(v) for v in r["num_blocks_xyz"]]
+    bmn = np.array(r["bounds_min"])
+    got = _decode(r, nb[0], nb[1])
+    s = math.exp(float(np.float32(ls)))
+    ext = float(np.float32(3 * s))
+    o = _sig(float(np.float32(logit)))
+    half = res_v / 2
+    near = set(zip(r["near_block"].tolist(), r["near_bit"].tolist()))
+    mism = 0
+    for bz in range(nb[2]):
+        for by in range(nb[1]):
+            for bx in range(nb[0]):
+                flat = bx + by * nb[0] + bz * nb[0] * nb[1]
+                mask = got.get(flat, 0)
+                for lin in range(64):
+                    lx, ly, lz = lin & 3, (lin >> 2) & 3, lin >> 4
+                    vc = bmn + (np.array([bx, by, bz]) * 4 + np.array([lx, ly, lz]) + 0.5) * res_v
+                    inside = np.all(np.abs(vc - p) <= ext + half)
+                    closest = np.clip(p, vc - half, vc + half)
+                    d2 = float(np.sum((closest - p) ** 2)) / (s * s)
+                    alpha = 1 - math.exp(-o * math.exp(-0.5 * d2)) if inside else 0.0
+                    want = alpha >= cut
+                    have = bool((mask >> lin) & 1)
+                    if want != have:
+                        assert (flat, lin) in near or abs(alpha - cut) < 1e-5, (flat, lin, alpha)
+                        mism += 1
+    assert mism <= 2
+    assert r["block_idx"].size > 0
+
+
+def test_aabb_extents_against_rotation_matrix():
+    data = S.gaussian_cloud(2000, seed=4)
+    ext = oracle.aabb_extents(data).reshape(-1, 3).astype(np.float64)
+    q = data.rotations.reshape(-1, 4).astype(np.float64)
+    s = np.exp(data.log_scales.reshape(-1, 3).astype(np.float64))
+    w, x, y, z = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+    want = 3 * np.sqrt(np.einsum("nij,nj->ni", R ** 2, s ** 2))
+    np.testing.assert_allclose(ext, want, rtol=1e-6)
+    # identity rotation: ext = 3 s exactly
+    one = SplatData(1, [0, 0, 0], [1, 0, 0, 0], [-2.0, -3.0, -1.5], [0], [0, 0, 0])
+    np.testing.assert_array_equal(oracle.aabb_extents(one), np.float32(3 * np.exp([-2.0, -3.0, -1.5])))
+
+
+def test_bvh_order_is_permutation_and_lists_are_complete():
+    data = S.gaussian_cloud(3000, seed=5, pos_std=0.4, log_scale_range=(math.log(0.005), math.log(0.04)))
+    ext = oracle.aabb_extents(data)
+    perm = oracle.bvh_order(data.positions, ext)
+    assert sorted(perm.tolist()) == list(range(data.count))
+    r = oracle.voxelize(data, voxel_resolution=0.02, opacity_cutoff=0.1)
+    bmn = np.array(r["bounds_min"])
+    nb = [int(v) for v in r["num_blocks_xyz"]]
+    bs = 4 * 0.02
+    p = data.positions.reshape(-1, 3).astype(np.float64)
+    e = ext.reshape(-1, 3).astype(np.float64)
+    total, batches = 0, 0
+    for bz in range(0, nb[2], 16):
+        for by in range(0, nb[1], 16):
+            for bx in range(0, nb[0], 16):
+                c = [min(16, nb[0] - bx), min(16, nb[1] - by), min(16, nb[2] - bz)]
+                qmn = np.array([bmn[0] + bx * bs, bmn[1] + by * bs, bmn[2] + bz * bs])
+                qmx = qmn + np.array(c) * bs
+                ok = ~(((p + e) < qmn) | ((p - e) > qmx)).any(1)
+                total += int(ok.sum())
+                batches += int(ok.any())
+    assert r["num_candidates"] == total
+    assert r["num_batches"] == batches
+
+
+def test_empty_scene():
+    r = oracle.voxelize(S.empty_scene(), voxel_resolution=0.05, opacity_cutoff=0.1)
+    assert r["block_idx"].size == 0
+    assert all(v == -math.inf for v in r["num_blocks_xyz"])
+
+
+def test_browser_sphere_voxelizes():
+    r = oracle.voxelize(S.many_splats(5000, "sphere"), voxel_resolution=0.05, opacity_cutoff=0.1)
+    assert r["block_idx"].size > 0
+    # insertion order: block ids inside one batch ascend in (lz, ly, lx) order
+    assert len(set(r["block_idx"].tolist())) == r["block_idx"].size
