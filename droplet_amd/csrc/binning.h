@@ -16,7 +16,12 @@ struct BinGrid {
 };
 
 struct BinWorkspace {
-  DeviceBuffer c1, base1, spans, row_start, block_base, row_of, c2, base2, totals, scan_ws;
+  // level 1
+  DeviceBuffer c1, base1, spans, row_start;
+  // level 2a / 2b chunk tables, counts and scanned bases
+  DeviceBuffer tmp, chunk_base_a, seg_of_a, c2, base2, gspans;
+  DeviceBuffer seg_start_b, chunk_base_b, seg_of_b, c3, base3;
+  DeviceBuffer scan_ws;
 };
 
 bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y);

@@ -1,24 +1,23 @@
 // Tile binning without a pair sort (gfx950).
 //
-// Produces exactly what emit (src/render/shaders/emit-pairs.ts:27-56) followed
-// by the stable pair sort (src/render/pipeline.ts:544-578) and the tile
-// ranges (src/render/shaders/find-boundaries.ts:17-99) produce: for every tile
-// the list of chunk-local splat indices covering it, ascending, plus the
+// Produces exactly what emit (src/render/shaders/emit-pairs.ts:27-56)
+// followed by the stable pair sort (src/render/pipeline.ts:544-578) and the
+// tile ranges (src/render/shaders/find-boundaries.ts:17-99) produce: for every
+// tile the list of the chunk's splats covering it, in depth order, plus the
 // per-tile offsets. Pair keys are never materialised.
 //
-// A splat's tiles are a rectangle (a full-row prefix plus one partial row when
-// maxCoverage truncates it), so the pair list factors into two stable
-// bucketings with few buckets each:
-//   level 1  splats -> rows: span (j, x0, x1) for every tile row a splat covers,
-//            grouped by row, ascending j inside a row;
-//   level 2  spans of one row -> columns: j for every column a span covers,
-//            grouped by column, ascending j inside a tile.
-// Each level splits its input into fixed blocks (256 splats, 1024 spans). A
-// block builds, in LDS, one bitmask per bucket over its elements; popcounts
-// give per-(bucket, block) counts, one flat exclusive scan over (bucket,
-// block) gives every block's base per bucket, and the rank of an element in
-// a bucket is a popcount of the bits below it. Every pair is then written once
-// to its final position.
+// A splat's tiles are a rectangle (a full-row prefix plus one partial row
+// when maxCoverage truncates it), so the stable pair order factors into
+// stable bucketings with few buckets each:
+//   level 1  splats -> tile rows: one span (id, x0, x1) per covered row,
+//            grouped by row, depth order inside a row (LDS bitmask ranks);
+//   level 2a row spans -> 16-column groups of the row;
+//   level 2b group spans -> the group's columns: the splat id per tile.
+// Levels 2a and 2b are one primitive, a multi-bucket stable scatter (MBS):
+// every item covers a contiguous range of at most 16 buckets; a wave ranks its
+// 64 items in a bucket with one ballot per bucket, so ranks follow item order;
+// per-(segment, bucket, chunk) counts and one flat exclusive scan place every
+// chunk's run of each bucket; each item is then written once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,10 +32,8 @@ namespace {
 
 constexpr int kB1 = 256;                 // splats per level-1 block (one per thread)
 constexpr int kW1 = kB1 / 32;            // mask words per row
-constexpr int kB2Threads = 256;
-constexpr int kSpansPerThread = 4;
-constexpr int kB2 = kB2Threads * kSpansPerThread;  // spans per level-2 block
-constexpr int kW2 = kB2 / 32;            // mask words per column
+constexpr int kChunk = 1024;             // items per MBS chunk: 4 waves x 4 steps x 64 lanes
+constexpr int kK = 16;                   // buckets per MBS segment
 
 struct Rect {
   uint32_t x0, x1, y0;
@@ -133,152 +130,204 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const float4 *__restrict__
   }
 }
 
-// --------------------------- row blocks -----------------------------------
-// one workgroup: per row, the number of level-2 blocks and their prefix;
-// writes row_start[0..tiles_y], block_base[0..tiles_y] and row_of[block]
-__global__ __launch_bounds__(1024) void k_row_blocks(const uint32_t *__restrict__ base1, uint32_t nb1,
-                                                     uint32_t tiles_y, const uint32_t *__restrict__ d_num_spans,
-                                                     uint32_t *__restrict__ row_start,
-                                                     uint32_t *__restrict__ block_base,
-                                                     uint32_t *__restrict__ row_of, uint32_t max_blocks,
-                                                     uint32_t *__restrict__ d_nb2) {
-  __shared__ uint32_t s_nb[1024];
-  __shared__ uint32_t s_total;
-  const uint32_t t = threadIdx.x;
-  const uint32_t num_spans = *d_num_spans;
-  // tiles_y <= 1024 is checked on the host
-  uint32_t start = 0, nb = 0;
-  if (t < tiles_y) {
-    start = base1[(size_t)t * nb1];
-    const uint32_t end = (t + 1 < tiles_y) ? base1[(size_t)(t + 1) * nb1] : num_spans;
-    nb = (end - start + kB2 - 1) / kB2;
-    row_start[t] = start;
-  }
-  if (t == 0) row_start[tiles_y] = num_spans;
-  s_nb[t] = nb;
-  __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 entries
-  for (uint32_t d = 1; d < 1024; d <<= 1) {
-    const uint32_t v = (t >= d) ? s_nb[t - d] : 0u;
-    __syncthreads();
-    s_nb[t] += v;
-    __syncthreads();
-  }
-  const uint32_t excl = s_nb[t] - nb;
-  if (t < tiles_y) block_base[t] = excl;
-  if (t == 1023) {
-    s_total = s_nb[1023];
-    block_base[tiles_y] = s_total;
-    *d_nb2 = s_total;
-  }
-  __syncthreads();
-  if (t < tiles_y)
-    for (uint32_t k = 0; k < nb && excl + k < max_blocks; k++) row_of[excl + k] = t;
+// row_start[r] = spans before row r (the (row r, block 0) entry of the
+// level-1 scan), row_start[tiles_y] = all spans
+__global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, uint32_t nb1,
+                                                   uint32_t tiles_y, const uint32_t *__restrict__ d_num_spans,
+                                                   uint32_t *__restrict__ row_start) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r < tiles_y) row_start[r] = base1[(size_t)r * nb1];
+  if (r == tiles_y) row_start[r] = *d_num_spans;
 }
 
-// ---------------------------- level 2 -------------------------------------
-struct L2Block {
-  uint32_t row, k, nbr, span0, nspan;
-  size_t cbase;  // index of (column 0, block k) in the counts array
+// ----------------------- MBS chunk tables ---------------------------------
+__global__ __launch_bounds__(256) void k_seg_chunks(const uint32_t *__restrict__ seg_start, uint32_t nseg,
+                                                    uint32_t *__restrict__ nchunks) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s < nseg) nchunks[s] = (seg_start[s + 1] - seg_start[s] + kChunk - 1) / kChunk;
+}
+
+__global__ __launch_bounds__(256) void k_seg_of(const uint32_t *__restrict__ chunk_base, uint32_t nseg,
+                                                uint32_t *__restrict__ seg_of) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nseg) return;
+  for (uint32_t c = chunk_base[s]; c < chunk_base[s + 1]; c++) seg_of[c] = s;
+}
+
+// ---------------------------- MBS -----------------------------------------
+struct MbsGeom {
+  const uint32_t *seg_of;      // chunk -> segment
+  const uint32_t *chunk_base;  // segment -> first chunk (nseg + 1 entries)
+  const uint32_t *seg_start;   // segment -> first item (nseg + 1 entries)
+  uint32_t groups;             // 16-column groups per row (level 2b: segment = row * groups + group)
 };
 
-__device__ inline L2Block l2_block(uint32_t g2, const uint32_t *row_of, const uint32_t *row_start,
-                                   const uint32_t *block_base, uint32_t tiles_x) {
-  L2Block b;
-  b.row = row_of[g2];
-  const uint32_t bb = block_base[b.row];
-  b.k = g2 - bb;
-  b.nbr = block_base[b.row + 1] - bb;
-  const uint32_t rs = row_start[b.row], re = row_start[b.row + 1];
-  b.span0 = rs + b.k * kB2;
-  b.nspan = min((uint32_t)kB2, re - b.span0);
-  b.cbase = (size_t)tiles_x * bb + b.k;
-  return b;
+// buckets [lo, hi] of an item; lo > hi: none
+template <int LEVEL>
+__device__ inline void item_buckets(uint2 it, uint32_t seg, uint32_t groups, uint32_t &lo, uint32_t &hi) {
+  const uint32_t xa = it.y & 0xFFFFu, xb = it.y >> 16;
+  if (LEVEL == 0) {
+    lo = xa >> 4;
+    hi = xb >> 4;
+  } else {
+    const uint32_t g0 = (seg % groups) * 16u;
+    lo = max(xa, g0) - g0;
+    hi = min(xb, g0 + 15u) - g0;
+  }
 }
 
-__device__ inline void build_col_masks(uint32_t *s_mask, const uint2 *spans, const L2Block &b, uint32_t tiles_x,
-                                       uint2 sp[kSpansPerThread]) {
-  const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < tiles_x * kW2; i += kB2Threads) s_mask[i] = 0;
-  __syncthreads();
+struct ChunkSpan {
+  uint32_t seg, cb, nch, k, i0, iend;
+};
+
+__device__ inline ChunkSpan chunk_span(const MbsGeom &G, uint32_t c) {
+  ChunkSpan s;
+  s.seg = G.seg_of[c];
+  s.cb = G.chunk_base[s.seg];
+  s.nch = G.chunk_base[s.seg + 1] - s.cb;
+  s.k = c - s.cb;
+  s.i0 = G.seg_start[s.seg] + s.k * kChunk;
+  s.iend = min(G.seg_start[s.seg + 1], s.i0 + kChunk);
+  return s;
+}
+
+// counts[(16 * cb + b) * ... ] laid out per segment as [bucket][chunk]:
+// index 16*cb + b*nch + k, so a flat exclusive scan orders (segment, bucket, chunk)
+template <int LEVEL>
+__global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ items, MbsGeom G,
+                                                   uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_cnt[4][kK];
+  const ChunkSpan cs = chunk_span(G, blockIdx.x);
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  uint32_t cnt[kK];
 #pragma unroll
-  for (int q = 0; q < kSpansPerThread; q++) {
-    const uint32_t s = t + q * kB2Threads;
-    sp[q] = make_uint2(0xFFFFFFFFu, 0u);
-    if (s < b.nspan) {
-      sp[q] = spans[b.span0 + s];
-      const uint32_t xa = sp[q].y & 0xFFFFu, xb = sp[q].y >> 16;
-      for (uint32_t x = xa; x <= xb; x++) atomicOr(&s_mask[x * kW2 + (s >> 5)], 1u << (s & 31));
-    }
+  for (int b = 0; b < kK; b++) cnt[b] = 0;
+#pragma unroll
+  for (int step = 0; step < 4; step++) {
+    const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
+    uint32_t lo = 1, hi = 0;
+    if (i < cs.iend) item_buckets<LEVEL>(items[i], cs.seg, G.groups, lo, hi);
+#pragma unroll
+    for (int b = 0; b < kK; b++) cnt[b] += (uint32_t)__popcll(__ballot(lo <= (uint32_t)b && (uint32_t)b <= hi));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int b = 0; b < kK; b++) s_cnt[w][b] = cnt[b];
   }
   __syncthreads();
-}
-
-__global__ __launch_bounds__(kB2Threads) void k_cols_count(const uint2 *__restrict__ spans,
-                                                           const uint32_t *__restrict__ row_of,
-                                                           const uint32_t *__restrict__ row_start,
-                                                           const uint32_t *__restrict__ block_base, BinGrid g,
-                                                           uint32_t *__restrict__ c2) {
-  extern __shared__ uint32_t s_mask[];  // [tiles_x][kW2]
-  const L2Block b = l2_block(blockIdx.x, row_of, row_start, block_base, g.tiles_x);
-  uint2 sp[kSpansPerThread];
-  build_col_masks(s_mask, spans, b, g.tiles_x, sp);
-  for (uint32_t x = threadIdx.x; x < g.tiles_x; x += kB2Threads) {
-    uint32_t c = 0;
-#pragma unroll
-    for (int w = 0; w < kW2; w++) c += __popc(s_mask[x * kW2 + w]);
-    c2[b.cbase + (size_t)x * b.nbr] = c;
+  if (threadIdx.x < kK) {
+    const uint32_t b = threadIdx.x;
+    counts[(size_t)kK * cs.cb + (size_t)b * cs.nch + cs.k] = s_cnt[0][b] + s_cnt[1][b] + s_cnt[2][b] + s_cnt[3][b];
   }
 }
 
-__global__ __launch_bounds__(kB2Threads) void k_cols_scatter(const uint2 *__restrict__ spans,
-                                                             const uint32_t *__restrict__ row_of,
-                                                             const uint32_t *__restrict__ row_start,
-                                                             const uint32_t *__restrict__ block_base, BinGrid g,
-                                                             const uint32_t *__restrict__ base2,
-                                                             uint32_t *__restrict__ pvals) {
-  extern __shared__ uint32_t s_dyn[];
-  uint32_t *s_mask = s_dyn;                                                  // [tiles_x][kW2]
-  uint32_t *s_base = s_dyn + g.tiles_x * kW2;                                // [tiles_x]
-  uint16_t *s_pre = reinterpret_cast<uint16_t *>(s_base + g.tiles_x);        // [tiles_x][kW2]
-  const L2Block b = l2_block(blockIdx.x, row_of, row_start, block_base, g.tiles_x);
-  for (uint32_t x = threadIdx.x; x < g.tiles_x; x += kB2Threads) s_base[x] = base2[b.cbase + (size_t)x * b.nbr];
-  uint2 sp[kSpansPerThread];
-  build_col_masks(s_mask, spans, b, g.tiles_x, sp);
-  for (uint32_t x = threadIdx.x; x < g.tiles_x; x += kB2Threads) {
-    uint32_t run = 0;
+template <int LEVEL>
+__global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ items, MbsGeom G,
+                                                   const uint32_t *__restrict__ base, uint2 *__restrict__ out_items,
+                                                   uint32_t *__restrict__ out_ids) {
+  __shared__ uint32_t s_cnt[4][kK];
+  const ChunkSpan cs = chunk_span(G, blockIdx.x);
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  uint2 it[4];
+  uint32_t lo[4], hi[4];
+  uint32_t cnt[kK];
 #pragma unroll
-    for (int w = 0; w < kW2; w++) {
-      s_pre[x * kW2 + w] = (uint16_t)run;
-      run += __popc(s_mask[x * kW2 + w]);
+  for (int b = 0; b < kK; b++) cnt[b] = 0;
+#pragma unroll
+  for (int step = 0; step < 4; step++) {
+    const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
+    lo[step] = 1;
+    hi[step] = 0;
+    it[step] = make_uint2(0u, 0u);
+    if (i < cs.iend) {
+      it[step] = items[i];
+      item_buckets<LEVEL>(it[step], cs.seg, G.groups, lo[step], hi[step]);
     }
+#pragma unroll
+    for (int b = 0; b < kK; b++)
+      cnt[b] += (uint32_t)__popcll(__ballot(lo[step] <= (uint32_t)b && (uint32_t)b <= hi[step]));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int b = 0; b < kK; b++) s_cnt[w][b] = cnt[b];
   }
   __syncthreads();
+  uint32_t run[kK];
 #pragma unroll
-  for (int q = 0; q < kSpansPerThread; q++) {
-    const uint32_t s = threadIdx.x + q * kB2Threads;
-    if (sp[q].x == 0xFFFFFFFFu) continue;
-    const uint32_t xa = sp[q].y & 0xFFFFu, xb = sp[q].y >> 16;
-    const uint32_t word = s >> 5, below = (1u << (s & 31)) - 1u;
-    for (uint32_t x = xa; x <= xb; x++) {
-      const uint32_t rank = s_pre[x * kW2 + word] + __popc(s_mask[x * kW2 + word] & below);
-      pvals[(size_t)s_base[x] + rank] = sp[q].x;
+  for (int b = 0; b < kK; b++) {
+    uint32_t r = base[(size_t)kK * cs.cb + (size_t)b * cs.nch + cs.k];
+    for (int v = 0; v < w; v++) r += s_cnt[v][b];
+    run[b] = r;
+  }
+#pragma unroll
+  for (int step = 0; step < 4; step++) {
+#pragma unroll
+    for (int b = 0; b < kK; b++) {
+      const bool in = lo[step] <= (uint32_t)b && (uint32_t)b <= hi[step];
+      const uint64_t m = __ballot(in);
+      if (in) {
+        const uint32_t pos = run[b] + (uint32_t)__popcll(m & lt);
+        if (LEVEL == 0)
+          out_items[pos] = it[step];
+        else
+          out_ids[pos] = it[step].x;
+      }
+      run[b] += (uint32_t)__popcll(m);
     }
   }
 }
 
-// per-tile pair totals from the (tile, block) counts; rows without spans -> 0
-__global__ __launch_bounds__(256) void k_tile_totals(const uint32_t *__restrict__ c2,
-                                                     const uint32_t *__restrict__ block_base, BinGrid g,
-                                                     uint32_t *__restrict__ totals) {
-  const uint32_t tile = blockIdx.x * 256 + threadIdx.x;
-  if (tile >= g.tiles_x * g.tiles_y) return;
-  const uint32_t row = tile / g.tiles_x, x = tile % g.tiles_x;
-  const uint32_t bb = block_base[row], nbr = block_base[row + 1] - bb;
-  const size_t base = (size_t)g.tiles_x * bb + (size_t)x * nbr;
-  uint32_t s = 0;
-  for (uint32_t k = 0; k < nbr; k++) s += c2[base + k];
-  totals[tile] = s;
+// Start of every (segment, bucket) list from the scanned MBS counts: the
+// (segment, bucket, chunk 0) entry; a segment without chunks takes the entry
+// where it would start, or the total at the end. Level 2a: o = row*groups +
+// group. Level 2b: o = tile = row*tiles_x + x -> (row*groups + x/16, x%16).
+__global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restrict__ base,
+                                                        const uint32_t *__restrict__ chunk_base, uint32_t n_out,
+                                                        uint32_t groups, uint32_t tiles_x, int tile_mode,
+                                                        size_t ncounts, const uint32_t *__restrict__ d_total,
+                                                        uint32_t *__restrict__ out) {
+  const uint32_t o = blockIdx.x * 256 + threadIdx.x;
+  if (o > n_out) return;
+  if (o == n_out) {
+    out[o] = *d_total;
+    return;
+  }
+  uint32_t seg, b;
+  if (tile_mode) {
+    const uint32_t r = o / tiles_x, x = o % tiles_x;
+    seg = r * groups + x / 16u;
+    b = x % 16u;
+  } else {
+    seg = o / groups;
+    b = o % groups;
+  }
+  const uint32_t cb = chunk_base[seg], nch = chunk_base[seg + 1] - cb;
+  const size_t pos = (size_t)kK * cb + (size_t)b * nch;
+  out[o] = pos < ncounts ? base[pos] : *d_total;
+}
+
+// Chunk table of one MBS level: nchunks per segment, its scan (chunk_base,
+// with the total in chunk_base[nseg]) and chunk -> segment. Returns the
+// number of chunks (host sync).
+uint32_t build_chunk_table(hipStream_t s, const uint32_t *seg_start, uint32_t nseg, DeviceBuffer &tmp,
+                           DeviceBuffer &chunk_base, DeviceBuffer &seg_of, DeviceBuffer &scan_ws,
+                           uint32_t max_chunks) {
+  tmp.reserve((size_t)nseg * 4 + 4);
+  chunk_base.reserve(((size_t)nseg + 1) * 4);
+  seg_of.reserve((size_t)std::max<uint32_t>(max_chunks, 1) * 4);
+  scan_ws.reserve(scan_workspace_bytes(nseg));
+  k_seg_chunks<<<div_up(nseg, 256), 256, 0, s>>>(seg_start, nseg, tmp.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+  exclusive_scan_u32(s, tmp.as<uint32_t>(), chunk_base.as<uint32_t>(), nseg, chunk_base.as<uint32_t>() + nseg,
+                     scan_ws.get());
+  uint32_t total = 0;
+  DL_HIP(hipMemcpyAsync(&total, chunk_base.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s));
+  DL_HIP(hipStreamSynchronize(s));
+  if (total > max_chunks) fail(DL_ERR_INTERNAL, "span binning: chunk table overflow");
+  k_seg_of<<<div_up(nseg, 256), 256, 0, s>>>(chunk_base.as<uint32_t>(), nseg, seg_of.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+  return total;
 }
 
 }  // namespace
@@ -290,8 +339,9 @@ bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y) {
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small) {
   const uint32_t num_tiles = g.tiles_x * g.tiles_y;
+  const uint32_t groups = div_up(g.tiles_x, 16);
   uint32_t *d_num_spans = d_small;
-  uint32_t *d_nb2 = d_small + 1;
+  uint32_t *d_num_gspans = d_small + 1;
   uint32_t *d_npairs = d_small + 2;
   // level 1: splats -> row spans (at most one span per pair)
   const uint32_t nb1 = div_up(cn, kB1);
@@ -306,42 +356,60 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
   const size_t lds1 = (size_t)g.tiles_y * kW1 * 4 + (size_t)g.tiles_y * 4 + (size_t)g.tiles_y * kW1 * 2;
   k_rows_scatter<<<nb1, kB1, lds1, s>>>(rec, ids, cn, g, nb1, ws.base1.as<uint32_t>(), ws.spans.as<uint2>());
   DL_HIP(hipGetLastError());
-  const uint32_t max_blocks = div_up(npairs, kB2) + g.tiles_y;
   ws.row_start.reserve(((size_t)g.tiles_y + 1) * 4);
-  ws.block_base.reserve(((size_t)g.tiles_y + 1) * 4);
-  ws.row_of.reserve((size_t)max_blocks * 4);
-  k_row_blocks<<<1, 1024, 0, s>>>(ws.base1.as<uint32_t>(), nb1, g.tiles_y, d_num_spans, ws.row_start.as<uint32_t>(),
-                                  ws.block_base.as<uint32_t>(), ws.row_of.as<uint32_t>(), max_blocks, d_nb2);
+  k_row_start<<<div_up((size_t)g.tiles_y + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), nb1, g.tiles_y, d_num_spans,
+                                                                ws.row_start.as<uint32_t>());
   DL_HIP(hipGetLastError());
-  uint32_t nb2 = 0;
-  DL_HIP(hipMemcpyAsync(&nb2, d_nb2, 4, hipMemcpyDeviceToHost, s));
-  DL_HIP(hipStreamSynchronize(s));
-  if (nb2 > max_blocks) fail(DL_ERR_INTERNAL, "span binning: block table overflow");
-  // level 2: row spans -> per-tile lists
-  const size_t ncounts = (size_t)g.tiles_x * nb2;
-  ws.c2.reserve(std::max<size_t>(ncounts, 1) * 4);
-  ws.base2.reserve(std::max<size_t>(ncounts, 1) * 4);
-  ws.totals.reserve(((size_t)num_tiles + 1) * 4);
-  ws.scan_ws.reserve(scan_workspace_bytes(std::max<size_t>(ncounts, (size_t)num_tiles + 1)));
-  if (nb2 > 0) {
-    k_cols_count<<<nb2, kB2Threads, (size_t)g.tiles_x * kW2 * 4, s>>>(
-        ws.spans.as<uint2>(), ws.row_of.as<uint32_t>(), ws.row_start.as<uint32_t>(), ws.block_base.as<uint32_t>(), g,
-        ws.c2.as<uint32_t>());
-    DL_HIP(hipGetLastError());
-    exclusive_scan_u32(s, ws.c2.as<uint32_t>(), ws.base2.as<uint32_t>(), ncounts, nullptr, ws.scan_ws.get());
-  }
-  k_tile_totals<<<div_up(num_tiles, 256), 256, 0, s>>>(ws.c2.as<uint32_t>(), ws.block_base.as<uint32_t>(), g,
-                                                       ws.totals.as<uint32_t>());
-  DL_HIP(hipGetLastError());
-  exclusive_scan_u32(s, ws.totals.as<uint32_t>(), d_off, num_tiles, d_npairs, ws.scan_ws.get());
-  DL_HIP(hipMemcpyAsync(d_off + num_tiles, d_npairs, 4, hipMemcpyDeviceToDevice, s));
-  if (nb2 > 0) {
-    const size_t lds2 = (size_t)g.tiles_x * kW2 * 4 + (size_t)g.tiles_x * 4 + (size_t)g.tiles_x * kW2 * 2;
-    k_cols_scatter<<<nb2, kB2Threads, lds2, s>>>(ws.spans.as<uint2>(), ws.row_of.as<uint32_t>(),
-                                                 ws.row_start.as<uint32_t>(), ws.block_base.as<uint32_t>(), g,
-                                                 ws.base2.as<uint32_t>(), pvals);
+
+  // level 2a: row spans -> 16-column groups (segments = rows)
+  const uint32_t max_ch_a = div_up(npairs, kChunk) + g.tiles_y;
+  const uint32_t nch_a = build_chunk_table(s, ws.row_start.as<uint32_t>(), g.tiles_y, ws.tmp, ws.chunk_base_a,
+                                           ws.seg_of_a, ws.scan_ws, max_ch_a);
+  MbsGeom ga{ws.seg_of_a.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), ws.row_start.as<uint32_t>(), groups};
+  const size_t nc_a = (size_t)kK * nch_a;
+  ws.c2.reserve(std::max<size_t>(nc_a, 1) * 4);
+  ws.base2.reserve(std::max<size_t>(nc_a, 1) * 4);
+  ws.gspans.reserve((size_t)std::max<uint32_t>(npairs, 1) * 8);
+  ws.scan_ws.reserve(scan_workspace_bytes(std::max<size_t>(nc_a, 1)));
+  if (nch_a) {
+    k_mbs_count<0><<<nch_a, 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.c2.as<uint32_t>());
     DL_HIP(hipGetLastError());
   }
+  exclusive_scan_u32(s, ws.c2.as<uint32_t>(), ws.base2.as<uint32_t>(), nc_a, d_num_gspans, ws.scan_ws.get());
+  if (nch_a) {
+    k_mbs_write<0><<<nch_a, 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.base2.as<uint32_t>(), ws.gspans.as<uint2>(),
+                                         nullptr);
+    DL_HIP(hipGetLastError());
+  }
+  const uint32_t nseg_b = g.tiles_y * groups;
+  ws.seg_start_b.reserve(((size_t)nseg_b + 1) * 4);
+  k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(),
+                                                                   nseg_b, groups, g.tiles_x, 0, nc_a, d_num_gspans,
+                                                                   ws.seg_start_b.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+
+  // level 2b: group spans -> columns (segments = (row, group))
+  const uint32_t max_ch_b = div_up(npairs, kChunk) + nseg_b;
+  const uint32_t nch_b = build_chunk_table(s, ws.seg_start_b.as<uint32_t>(), nseg_b, ws.tmp, ws.chunk_base_b,
+                                           ws.seg_of_b, ws.scan_ws, max_ch_b);
+  MbsGeom gb{ws.seg_of_b.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), ws.seg_start_b.as<uint32_t>(), groups};
+  const size_t nc_b = (size_t)kK * nch_b;
+  ws.c3.reserve(std::max<size_t>(nc_b, 1) * 4);
+  ws.base3.reserve(std::max<size_t>(nc_b, 1) * 4);
+  ws.scan_ws.reserve(scan_workspace_bytes(std::max<size_t>(nc_b, 1)));
+  if (nch_b) {
+    k_mbs_count<1><<<nch_b, 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.c3.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+  }
+  exclusive_scan_u32(s, ws.c3.as<uint32_t>(), ws.base3.as<uint32_t>(), nc_b, d_npairs, ws.scan_ws.get());
+  if (nch_b) {
+    k_mbs_write<1><<<nch_b, 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.base3.as<uint32_t>(), nullptr, pvals);
+    DL_HIP(hipGetLastError());
+  }
+  k_bucket_offsets<<<div_up((size_t)num_tiles + 1, 256), 256, 0, s>>>(ws.base3.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(),
+                                                                      num_tiles, groups, g.tiles_x, 1, nc_b, d_npairs,
+                                                                      d_off);
+  DL_HIP(hipGetLastError());
 }
 
 }  // namespace dl

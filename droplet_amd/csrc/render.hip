@@ -217,9 +217,20 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
 #pragma unroll
   for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
   const float alpha = (dl_sigmoidf(logit) * fade) * aa;
+  // Rasterizer skip threshold in the record's padding lane (0 in the
+  // reference, project.ts:272): for power < tau the reference's pixel alpha,
+  // min(cap, alpha*max(0, exp(power) - floor)), is below 1/255. tau is
+  // ln(floor + (1/255)/alpha) lowered by margins that cover exp's error and
+  // the f32 roundings of that expression.
+  float tau = __builtin_inff();
+  if (alpha > 0.0f) {
+    const double target = (double)kGaussFloor + ((double)kMinAlpha / (double)alpha) * (1.0 - 1e-6);
+    const double t = log(target) - 1e-5;
+    tau = (float)(t - fabs(t) * 1e-6 - 1e-30);
+  }
   o0 = make_float4(sx, sy, radius, depth);
   o1 = make_float4(conic_a, conic_b, conic_c, alpha);
-  o2 = make_float4(col[0], col[1], col[2], 0.0f);
+  o2 = make_float4(col[0], col[1], col[2], tau);
   const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
   const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
   const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
@@ -403,24 +414,62 @@ __device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, cons
   return R | (G << 8) | (B << 16) | (A << 24);
 }
 
+// Can a pixel centre of the rectangle [x0, x1] x [y0, y1] pass the
+// reference's r-box test with power >= tau? Evaluated in f64 on the exact
+// quadratic q = A dx^2 + 2B dx dy + C dy^2 (power = -q/2): the minimum of a
+// positive-definite q over a rectangle is 0 when the rectangle holds the
+// centre and otherwise lies on an edge. The dx/dy intervals are widened for
+// the f32 rounding of px - sx and the minimum is lowered by 1e-5 of the
+// terms' magnitudes, far above the f32 evaluation error of q, so "false" is a
+// proof that every pixel of the rectangle skips this splat.
+__device__ inline bool rect_may_hit(double sx, double sy, double r, double A, double B, double C, double tau,
+                                    double x0, double x1, double y0, double y1) {
+  if (!(tau < INFINITY)) return false;
+  if (!(A > 0.0 && C > 0.0)) return true;
+  const double ex = (fabs(sx) + fabs(x1) + 1.0) * 2.4e-7;
+  const double ey = (fabs(sy) + fabs(y1) + 1.0) * 2.4e-7;
+  const double rx = r + ex, ry = r + ey;
+  const double dxl = fmax(x0 - sx - ex, -rx), dxh = fmin(x1 - sx + ex, rx);
+  const double dyl = fmax(y0 - sy - ey, -ry), dyh = fmin(y1 - sy + ey, ry);
+  if (dxl > dxh || dyl > dyh) return false;
+  double qmin = 0.0;
+  if (!(dxl <= 0.0 && 0.0 <= dxh && dyl <= 0.0 && 0.0 <= dyh)) {
+    qmin = INFINITY;
+    const double X[2] = {dxl, dxh}, Y[2] = {dyl, dyh};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const double yv = fmin(fmax(-B * X[e] / C, dyl), dyh);
+      qmin = fmin(qmin, A * X[e] * X[e] + 2.0 * B * X[e] * yv + C * yv * yv);
+      const double xv = fmin(fmax(-B * Y[e] / A, dxl), dxh);
+      qmin = fmin(qmin, A * xv * xv + 2.0 * B * xv * Y[e] + C * Y[e] * Y[e]);
+    }
+  }
+  const double mx = fmax(fabs(dxl), fabs(dxh)), my = fmax(fabs(dyl), fabs(dyh));
+  const double mag = A * mx * mx + 2.0 * fabs(B) * mx * my + C * my * my;
+  return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
                                                   const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
                                                   uint32_t *__restrict__ out_rgba, RastParams p,
                                                   unsigned long long *__restrict__ d_iters) {
   __shared__ float4 s_rec[64 * 3];
+  __shared__ uint32_t s_mask[64];
   uint32_t iters = 0;
   const uint32_t tile = blockIdx.x;
   const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
   const int lane = threadIdx.x;
+  // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
+  // this lane's row in strip k is 4k + lane/16
   const uint32_t px = tx * kTile + (lane & 15);
-  const uint32_t py0 = ty * kTile + (lane >> 4) * 4;
+  const uint32_t py0 = ty * kTile + (lane >> 4);
   const float pxf = (float)px + 0.5f;
   float cr[4], cg[4], cb[4], T[4], pyf[4];
   bool live[4], inb[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const uint32_t y = py0 + k;
+    const uint32_t y = py0 + 4 * k;
     inb[k] = px < p.width && y < p.height;
     pyf[k] = (float)y + 0.5f;
     cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = 1.0f;
@@ -432,32 +481,48 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
   }
   if (p.has_pairs) {
     const uint32_t start = off[tile], end = off[tile + 1];
+    const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
     for (uint32_t base = start; base < end; base += 64) {
-      const bool any_live = live[0] || live[1] || live[2] || live[3];
-      if (__ballot(any_live) == 0) break;
+      uint32_t strips_live = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) strips_live |= (__ballot(live[k]) != 0 ? 1u : 0u) << k;
+      if (strips_live == 0) break;
       const uint32_t i = base + lane;
       if (i < end) {
         const uint32_t j = pvals[i];
-        s_rec[3 * lane + 0] = rec[3 * (size_t)j + 0];
-        s_rec[3 * lane + 1] = rec[3 * (size_t)j + 1];
-        s_rec[3 * lane + 2] = rec[3 * (size_t)j + 2];
+        const float4 a = rec[3 * (size_t)j + 0];
+        const float4 b = rec[3 * (size_t)j + 1];
+        const float4 col = rec[3 * (size_t)j + 2];
+        s_rec[3 * lane + 0] = a;
+        s_rec[3 * lane + 1] = b;
+        s_rec[3 * lane + 2] = col;
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
+          if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+        }
+        s_mask[lane] = m;
       }
       __syncthreads();
       const uint32_t cnt = min(64u, end - base);
       for (uint32_t c = 0; c < cnt; c++) {
+        if (COUNT) iters += (uint32_t)live[0] + (uint32_t)live[1] + (uint32_t)live[2] + (uint32_t)live[3];
+        const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[c]) & strips_live;
+        if (m == 0) continue;
         const float4 a = s_rec[3 * c + 0];
         const float4 b = s_rec[3 * c + 1];
         const float4 col = s_rec[3 * c + 2];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
+          if (!((m >> k) & 1u)) continue;
           if (!live[k]) continue;
-          if (COUNT) iters++;
           const float dx = pxf - a.x;
           const float dy = pyf[k] - a.y;
           const float r = a.z;
           if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
           const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
-          if (power > 0.0f) continue;
+          if (power > 0.0f || power < col.w) continue;
           const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf(power) - kGaussFloor));
           if (alpha < kMinAlpha) continue;
           const float w = T[k] * alpha;
@@ -474,7 +539,7 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     if (!inb[k]) continue;
-    const size_t pix = (size_t)(py0 + k) * p.width + px;
+    const size_t pix = (size_t)(py0 + 4 * k) * p.width + px;
     if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
     if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }

@@ -43,9 +43,11 @@ def assert_render_parity(renderer, data, cam, bg=None, chunk_cap=None, max_cover
     proj = t["projected"].reshape(-1, 12)
     rp = ref["projected"]
     culled = rp[:, 2] < 0
-    # culled splats: only slot 0 is defined (project.ts:217-219)
+    # culled splats: only slot 0 is defined (project.ts:217-219); slot 2.w is
+    # the reference's padding, which the device uses for the rasterizer's
+    # skip threshold
     np.testing.assert_array_equal(proj[culled, :4].view(np.uint32), rp[culled, :4].view(np.uint32))
-    np.testing.assert_array_equal(proj[~culled].view(np.uint32), rp[~culled].view(np.uint32))
+    np.testing.assert_array_equal(proj[~culled, :11].view(np.uint32), rp[~culled, :11].view(np.uint32))
     np.testing.assert_array_equal(t["tile_counts"], ref["tile_counts"])
     np.testing.assert_array_equal(t["pair_tiles"], ref["pair_tiles"])
     np.testing.assert_array_equal(t["pair_splats"], ref["pair_splats"])
@@ -191,3 +193,50 @@ def test_wide_grid_falls_back_to_sort(renderer):
     # 4400 px = 275 tile columns: above the span path's 256-column LDS limit
     data = S.gaussian_cloud(20_000, seed=14)
     assert_render_parity(renderer, data, S.default_camera(4400, 64))
+
+
+def test_rasterize_work_count_matches_oracle(device):
+    """The instrumented kernel counts the reference's per-pixel loop
+    iterations (executed before the T < 1e-4 break); strip culling and the
+    exp skip inside the kernel must not change that count."""
+    data, cam = S.plain_render_workload(seed=0)
+    r = Renderer(device)
+    try:
+        r.set_counting(True)
+        img = r.render(data, cam)
+        iters = r.stats()["pixel_iters"]
+    finally:
+        r.destroy()
+    ref = oracle.render(data, cam, taps=False)
+    assert iters == ref["pixel_iters"]
+    np.testing.assert_array_equal(img, ref["rgba"])
+
+
+def test_exp_skip_threshold_is_conservative(renderer):
+    """The device stores a per-splat power threshold tau in record slot 2.w and
+    skips the pixel body for power < tau. Check on every projected splat that
+    at the largest f32 power below tau the reference's alpha,
+    min(0.999, a*max(0, exp(power) - floor)), is already below 1/255 (alpha is
+    monotone in power, so no skipped pixel could have composited)."""
+    data, cam = S.plain_render_workload(seed=15)
+    _, t, _ = _gpu_taps(renderer, data, cam)
+    rec = t["projected"].reshape(-1, 12)
+    rec = rec[rec[:, 2] > 0]
+    a = rec[:, 7].astype(np.float32)
+    tau = rec[:, 11].astype(np.float32)
+    finite = np.isfinite(tau)
+    assert finite.mean() > 0.9
+    below = np.nextafter(tau[finite], np.float32(-np.inf)).astype(np.float32)
+    below = np.minimum(below, np.float32(0.0))
+    e = oracle.expf(below)
+    f32 = np.float32
+    alpha = np.minimum(f32(0.999), a[finite] * np.maximum(f32(0.0), e - f32(1.1108996538e-2)))
+    assert (alpha < f32(0.00392156862745098)).all()
+    # and the threshold is tight: just above tau, most splats do composite
+    above = np.nextafter(tau[finite], np.float32(np.inf)).astype(np.float32) + np.float32(1e-3)
+    above = np.minimum(above, np.float32(0.0))
+    e2 = oracle.expf(above)
+    alpha2 = np.minimum(f32(0.999), a[finite] * np.maximum(f32(0.0), e2 - f32(1.1108996538e-2)))
+    assert (alpha2 >= f32(0.00392156862745098)).mean() > 0.95
+    # tau is +inf only where alpha == 0
+    assert (a[~finite] == 0).all()
