@@ -83,6 +83,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--splats", type=int, default=6_000_000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (a kernel of known traffic) so that a "
+                         "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE run can calibrate the counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the renderer runs on a single "
@@ -94,6 +97,17 @@ def main() -> int:
 
     def note(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.copy_probe:
+        import torch  # only the calibration copy uses torch
+
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        note("copy probe: 1 GiB device-to-device copy done")
+        del src, dst
 
     note(f"generating {args.splats} splats")
     data, cam = S.large_render_workload(n=args.splats)
