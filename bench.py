@@ -119,9 +119,11 @@ def main() -> int:
     # one untimed instrumented frame: rasterize work units (pixel loop iterations)
     r.set_counting(True)
     r.render_scene(scene, cam, readback=False)
-    iters = r.stats()["pixel_iters"]
+    cst = r.stats()
+    iters = cst["pixel_iters"]
+    consumed = cst["pairs_consumed"]
     r.set_counting(False)
-    note(f"instrumented frame: {iters} pixel iterations")
+    note(f"instrumented frame: {iters} pixel iterations, {consumed} tile-list entries read")
 
     for _ in range(args.warmup):
         r.render_scene(scene, cam, readback=False)
@@ -165,7 +167,7 @@ def main() -> int:
             "workload": "large forward render: 6M Gaussians, SH bands 3, 3840x2160 pinhole, scene resident in HBM",
             "splats": data.count, "sh_bands": data.sh_bands, "width": cam.width, "height": cam.height,
             "projection": cam.projection, "visible": st["num_visible"], "pairs": st["num_pairs"],
-            "tiles": st["num_tiles"],
+            "tiles": st["num_tiles"], "pairs_read_by_rasterize": consumed,
         },
         "roofline": {
             "kernel": "k_rasterize", "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,

@@ -58,7 +58,7 @@ class DlRenderStats(C.Structure):
     _fields_ = [("num_visible", C.c_uint32), ("num_chunks", C.c_uint32), ("num_tiles", C.c_uint32),
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("pair_key_bits", C.c_uint32),
                 ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
-                ("pixel_iters", C.c_uint64)]
+                ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64)]
 
 
 class DlVoxelOpts(C.Structure):

@@ -89,9 +89,32 @@ void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, si
 // Stable LSD radix sort of (key, value) pairs on bits [0, key_bits) of the
 // key. Results end in either (keys, vals) or (tmp_keys, tmp_vals); the
 // return value is true when they are in the tmp buffers.
+// Digits are at most max_digit_bits (1..8) wide; passes = ceil(key_bits /
+// max_digit_bits).
 size_t radix_workspace_bytes(size_t n, int key_bits);
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
-                      uint32_t *tmp_vals, size_t n, int key_bits, void *ws);
+                      uint32_t *tmp_vals, size_t n, int key_bits, void *ws, int max_digit_bits = 8);
+int radix_passes(int key_bits, int max_digit_bits);
+
+// Segment table of the segmented sort (device pointers): segments are the
+// contiguous element ranges [seg_start[s], seg_start[s+1]) with seg_start[0]
+// = 0; blk0 / blk_seg are the sort's own tile tables.
+struct SegGeom {
+  const uint32_t *seg_start = nullptr;
+  const uint32_t *blk0 = nullptr;
+  const uint32_t *blk_seg = nullptr;
+  uint32_t nseg = 0;
+};
+
+// Stable LSD radix sort of every segment on its own, with the segment bounds
+// read on the device (d_seg_start, nseg + 1 entries, at most max_elems
+// elements in all): no host synchronisation, so the launches can be captured
+// in a graph. Result placement as radix_sort_pairs (it depends only on the
+// pass count).
+size_t seg_radix_workspace_bytes(size_t max_elems, uint32_t nseg);
+bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
+                          const uint32_t *d_seg_start, uint32_t nseg, size_t max_elems, int key_bits,
+                          int max_digit_bits, void *ws);
 
 // off[b] = lower_bound(sorted keys, b) for b < num_bins and off[num_bins] = n
 // (all zeros when n == 0): the result of find-boundaries.ts:17-99.

@@ -1,0 +1,435 @@
+// Device code shared by the single-view renderer (render.hip) and the
+// multi-view batch (batch.hip): projection, cull, the rasterizer's tile loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dl_common.h"
+#include "render.h"
+
+namespace dl {
+namespace {
+
+// ---------------------------------------------------------------------------
+// project (project.ts:73-383)
+// ---------------------------------------------------------------------------
+__device__ inline void quat_rot(const float q[4], float v0, float v1, float v2, float o[3]) {
+  const float c0 = q[2] * v2 - q[3] * v1;
+  const float c1 = q[3] * v0 - q[1] * v2;
+  const float c2 = q[1] * v1 - q[2] * v0;
+  const float t0 = 2.0f * c0, t1 = 2.0f * c1, t2 = 2.0f * c2;
+  const float d0 = q[2] * t2 - q[3] * t1;
+  const float d1 = q[3] * t0 - q[1] * t2;
+  const float d2 = q[1] * t1 - q[2] * t0;
+  o[0] = (v0 + q[0] * t0) + d0;
+  o[1] = (v1 + q[0] * t1) + d1;
+  o[2] = (v2 + q[0] * t2) + d2;
+}
+
+template <int BANDS>
+__device__ inline void eval_sh(const float *__restrict__ sh, float x, float y, float z, float out[3]) {
+  constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+  float c[3 * M];
+#pragma unroll
+  for (int k = 0; k < 3 * M; k++) c[k] = sh[k];
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  const float SH1 = (float)0.4886025119029199;
+  const float e0 = (-SH1) * y, e1 = SH1 * z, e2 = (-SH1) * x;
+  r = r + e0 * c[0]; r = r + e1 * c[1]; r = r + e2 * c[2];
+  g = g + e0 * c[M + 0]; g = g + e1 * c[M + 1]; g = g + e2 * c[M + 2];
+  b = b + e0 * c[2 * M + 0]; b = b + e1 * c[2 * M + 1]; b = b + e2 * c[2 * M + 2];
+  if constexpr (BANDS >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    float b2[5];
+    b2[0] = (float)1.0925484305920792 * xy;
+    b2[1] = (float)-1.0925484305920792 * yz;
+    b2[2] = (float)0.31539156525252005 * ((2.0f * zz - xx) - yy);
+    b2[3] = (float)-1.0925484305920792 * xz;
+    b2[4] = (float)0.5462742152960396 * (xx - yy);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      r = r + b2[k] * c[3 + k];
+      g = g + b2[k] * c[M + 3 + k];
+      b = b + b2[k] * c[2 * M + 3 + k];
+    }
+    if constexpr (BANDS >= 3) {
+      float b3[7];
+      b3[0] = ((float)-0.5900435899266435 * y) * (3.0f * xx - yy);
+      b3[1] = ((float)2.890611442640554 * xy) * z;
+      b3[2] = ((float)-0.4570457994644658 * y) * ((4.0f * zz - xx) - yy);
+      b3[3] = ((float)0.3731763325901154 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+      b3[4] = ((float)-0.4570457994644658 * x) * ((4.0f * zz - xx) - yy);
+      b3[5] = ((float)1.445305721320277 * z) * (xx - yy);
+      b3[6] = ((float)-0.5900435899266435 * x) * (xx - 3.0f * yy);
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        r = r + b3[k] * c[8 + k];
+        g = g + b3[k] * c[M + 8 + k];
+        b = b + b3[k] * c[2 * M + 8 + k];
+      }
+    }
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = b;
+}
+
+// One splat through project.ts:111-285. Returns the tile count; a culled
+// splat gets slot 0 = (0, 0, -1, 0) and count 0 (project.ts:216-219, 247-250).
+template <int BANDS>
+__device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const ProjUniforms &u, float4 &o0,
+                                         float4 &o1, float4 &o2) {
+  const float *m = u.view;
+  o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+  o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float px = sc.pos[3 * (size_t)id], py = sc.pos[3 * (size_t)id + 1], pz = sc.pos[3 * (size_t)id + 2];
+  const float4 qv = reinterpret_cast<const float4 *>(sc.rot)[id];
+  const float q[4] = {qv.x, qv.y, qv.z, qv.w};
+  const float s0 = dl_expf(sc.ls[3 * (size_t)id]);
+  const float s1 = dl_expf(sc.ls[3 * (size_t)id + 1]);
+  const float s2 = dl_expf(sc.ls[3 * (size_t)id + 2]);
+  const float logit = sc.op[id];
+
+  const float pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
+  const float pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
+  const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
+  const float depth = -pcz;
+  float sx, sy;
+  if (u.proj == DL_PINHOLE) {
+    if (depth <= u.near_ || depth >= u.far_) return 0;
+    const float proj_x = (pcx * u.focal) / (-pcz);
+    const float proj_y = ((-pcy) * u.focal) / (-pcz);
+    sx = proj_x + u.cx;
+    sy = proj_y + u.cy;
+  } else {
+    const float rr = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+    if (rr <= u.near_) return 0;
+    const float lon = (float)atan2((double)pcx, (double)(-pcz));
+    const float lat = (float)asin((double)(pcy / rr));
+    sx = (lon / kTwoPi + 0.5f) * u.width;
+    sy = (0.5f - lat / kPi) * u.height;
+  }
+  float rx[3], ry[3], rz[3];
+  quat_rot(q, 1.0f, 0.0f, 0.0f, rx);
+  quat_rot(q, 0.0f, 1.0f, 0.0f, ry);
+  quat_rot(q, 0.0f, 0.0f, 1.0f, rz);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    rx[k] = rx[k] * s0;
+    ry[k] = ry[k] * s1;
+    rz[k] = rz[k] * s2;
+  }
+  const float c00 = (rx[0] * rx[0] + ry[0] * ry[0]) + rz[0] * rz[0];
+  const float c01 = (rx[0] * rx[1] + ry[0] * ry[1]) + rz[0] * rz[1];
+  const float c02 = (rx[0] * rx[2] + ry[0] * ry[2]) + rz[0] * rz[2];
+  const float c11 = (rx[1] * rx[1] + ry[1] * ry[1]) + rz[1] * rz[1];
+  const float c12 = (rx[1] * rx[2] + ry[1] * ry[2]) + rz[1] * rz[2];
+  const float c22 = (rx[2] * rx[2] + ry[2] * ry[2]) + rz[2] * rz[2];
+  const float v00 = m[0], v01 = m[4], v02 = m[8];
+  const float v10 = m[1], v11 = m[5], v12 = m[9];
+  const float v20 = m[2], v21 = m[6], v22 = m[10];
+  const float a00 = (v00 * c00 + v01 * c01) + v02 * c02;
+  const float a01 = (v00 * c01 + v01 * c11) + v02 * c12;
+  const float a02 = (v00 * c02 + v01 * c12) + v02 * c22;
+  const float a10 = (v10 * c00 + v11 * c01) + v12 * c02;
+  const float a11 = (v10 * c01 + v11 * c11) + v12 * c12;
+  const float a12 = (v10 * c02 + v11 * c12) + v12 * c22;
+  const float a20 = (v20 * c00 + v21 * c01) + v22 * c02;
+  const float a21 = (v20 * c01 + v21 * c11) + v22 * c12;
+  const float a22 = (v20 * c02 + v21 * c12) + v22 * c22;
+  const float cc00 = (a00 * v00 + a01 * v01) + a02 * v02;
+  const float cc01 = (a00 * v10 + a01 * v11) + a02 * v12;
+  const float cc02 = (a00 * v20 + a01 * v21) + a02 * v22;
+  const float cc11 = (a10 * v10 + a11 * v11) + a12 * v12;
+  const float cc12 = (a10 * v20 + a11 * v21) + a12 * v22;
+  const float cc22 = (a20 * v20 + a21 * v21) + a22 * v22;
+  const float inv_z = 1.0f / pcz;
+  const float j00 = u.focal * inv_z;
+  const float j02 = (((-u.focal) * pcx) * inv_z) * inv_z;
+  const float j11 = u.focal * inv_z;
+  const float j12 = (((-u.focal) * pcy) * inv_z) * inv_z;
+  const float b00 = j00 * cc00 + j02 * cc02;
+  const float b01 = j00 * cc01 + j02 * cc12;
+  const float b02 = j00 * cc02 + j02 * cc22;
+  const float b11 = j11 * cc11 + j12 * cc12;
+  const float b12 = j11 * cc12 + j12 * cc22;
+  float cov00 = b00 * j00 + b02 * j02;
+  const float cov01 = b01 * j11 + b02 * j12;
+  float cov11 = b11 * j11 + b12 * j12;
+  const float det_orig = cov00 * cov11 - cov01 * cov01;
+  cov00 = cov00 + kAA;
+  cov11 = cov11 + kAA;
+  const float det = cov00 * cov11 - cov01 * cov01;
+  const float aa = __builtin_sqrtf(wmax(det_orig / det, 0.0f));
+  if (det <= kDetMin) return 0;
+  const float inv_det = 1.0f / det;
+  const float conic_a = cov11 * inv_det;
+  const float conic_b = (-cov01) * inv_det;
+  const float conic_c = cov00 * inv_det;
+  const float mid = 0.5f * (cov00 + cov11);
+  const float r_eig = __builtin_sqrtf(wmax(kEigMin, mid * mid - det));
+  const float lambda1 = mid + r_eig;
+  const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
+  const float fade_s = kFadeStart * u.height;
+  const float fade_e = kFadeEnd * u.height;
+  const float fade = wclamp((fade_e - radius_raw) / (fade_e - fade_s), 0.0f, 1.0f);
+  if (fade <= 0.0f) return 0;
+  const float radius = __builtin_ceilf(wmin(radius_raw, fade_e));
+  float col[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
+  if constexpr (BANDS > 0) {
+    constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+    const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+    float sh[3];
+    eval_sh<BANDS>(sc.sh + (size_t)id * 3 * M, pcx / len, pcy / len, pcz / len, sh);
+    col[0] = col[0] + sh[0];
+    col[1] = col[1] + sh[1];
+    col[2] = col[2] + sh[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
+  const float alpha = (dl_sigmoidf(logit) * fade) * aa;
+  // Rasterizer skip threshold in the record's padding lane (0 in the
+  // reference, project.ts:272): for power < tau the reference's pixel alpha,
+  // min(cap, alpha*max(0, exp(power) - floor)), is below 1/255. tau is
+  // ln(floor + (1/255)/alpha) lowered by margins that cover exp's error and
+  // the f32 roundings of that expression.
+  float tau = __builtin_inff();
+  if (alpha > 0.0f) {
+    const double target = (double)kGaussFloor + ((double)kMinAlpha / (double)alpha) * (1.0 - 1e-6);
+    const double t = log(target) - 1e-5;
+    tau = (float)(t - fabs(t) * 1e-6 - 1e-30);
+  }
+  o0 = make_float4(sx, sy, radius, depth);
+  o1 = make_float4(conic_a, conic_b, conic_c, alpha);
+  o2 = make_float4(col[0], col[1], col[2], tau);
+  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
+  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
+  const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
+  const uint32_t max_ty = u32sat(wmin((float)u.tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
+  uint32_t count = 0;
+  if (max_tx >= min_tx && max_ty >= min_ty) {
+    count = (max_tx - min_tx + 1u) * (max_ty - min_ty + 1u);
+    if (count > u.max_cov) count = u.max_cov;
+  }
+  return count;
+}
+
+
+// f64 cull (camera.ts:56-99) and sortable depth key of splat i, then
+// project() when it is visible; the outputs go to slot o of keys / vals /
+// rec / counts, with vals[o] = val.
+template <int BANDS>
+__device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms &u, double near_plane,
+                                        double far_plane, uint32_t i, size_t o, uint32_t val,
+                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                        float4 *__restrict__ rec, uint32_t *__restrict__ counts, bool &vis,
+                                        uint32_t &count) {
+  const float *m = u.view;
+  const double x = sc.pos[3 * (size_t)i], y = sc.pos[3 * (size_t)i + 1], z = sc.pos[3 * (size_t)i + 2];
+  float depth_f;
+  if (u.proj == DL_PINHOLE) {
+    double cz = (double)m[2] * x + (double)m[6] * y;
+    cz = cz + (double)m[10] * z;
+    cz = cz + (double)m[14];
+    const double depth = -cz;
+    vis = depth > near_plane && depth < far_plane;
+    depth_f = (float)depth;
+  } else {
+    double cx = (double)m[0] * x + (double)m[4] * y;
+    cx = cx + (double)m[8] * z;
+    cx = cx + (double)m[12];
+    double cy = (double)m[1] * x + (double)m[5] * y;
+    cy = cy + (double)m[9] * z;
+    cy = cy + (double)m[13];
+    double cz = (double)m[2] * x + (double)m[6] * y;
+    cz = cz + (double)m[10] * z;
+    cz = cz + (double)m[14];
+    double r2 = cx * cx + cy * cy;
+    r2 = r2 + cz * cz;
+    vis = r2 > near_plane * near_plane;
+    depth_f = (float)__builtin_sqrt(r2);
+  }
+  keys[o] = vis ? sortable_key(depth_f) : kInvisibleKey;
+  vals[o] = val;
+  float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f), o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o2 = o1;
+  count = 0;
+  if (vis) count = project_splat<BANDS>(sc, i, u, o0, o1, o2);
+  rec[3 * o + 0] = o0;
+  rec[3 * o + 1] = o1;
+  rec[3 * o + 2] = o2;
+  counts[o] = count;
+}
+
+// ---------------------------------------------------------------------------
+// rasterize (rasterize.ts:36-82) + finalize (finalize.ts:29-42)
+//
+// One wave64 per 16x16 tile. Lane l owns pixel column l%16 and the four rows
+// 4*(l/16) .. 4*(l/16)+3, so the wave covers the whole tile and needs no
+// workgroup barrier. The tile's splats are staged 64 at a time in LDS (one
+// record per lane) and read back as broadcasts. Every pixel runs exactly the
+// reference's loop body in the reference's order; a pixel stops when T < 1e-4
+// and the wave stops when all its pixels have.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, const RastParams &p) {
+  const float r = wclamp(cr + T * p.bg_r, 0.0f, 1.0f);
+  const float g = wclamp(cg + T * p.bg_g, 0.0f, 1.0f);
+  const float b = wclamp(cb + T * p.bg_b, 0.0f, 1.0f);
+  const float a = wclamp((1.0f - T) + T * p.bg_a, 0.0f, 1.0f);
+  const uint32_t R = u32sat(r * 255.0f + 0.5f);
+  const uint32_t G = u32sat(g * 255.0f + 0.5f);
+  const uint32_t B = u32sat(b * 255.0f + 0.5f);
+  const uint32_t A = u32sat(a * 255.0f + 0.5f);
+  return R | (G << 8) | (B << 16) | (A << 24);
+}
+
+// Can a pixel centre of the rectangle [x0, x1] x [y0, y1] pass the
+// reference's r-box test with power >= tau? Evaluated in f64 on the exact
+// quadratic q = A dx^2 + 2B dx dy + C dy^2 (power = -q/2): the minimum of a
+// positive-definite q over a rectangle is 0 when the rectangle holds the
+// centre and otherwise lies on an edge. The dx/dy intervals are widened for
+// the f32 rounding of px - sx and the minimum is lowered by 1e-5 of the
+// terms' magnitudes, far above the f32 evaluation error of q, so "false" is a
+// proof that every pixel of the rectangle skips this splat.
+__device__ inline bool rect_may_hit(double sx, double sy, double r, double A, double B, double C, double tau,
+                                    double x0, double x1, double y0, double y1) {
+  if (!(tau < INFINITY)) return false;
+  if (!(A > 0.0 && C > 0.0)) return true;
+  const double ex = (fabs(sx) + fabs(x1) + 1.0) * 2.4e-7;
+  const double ey = (fabs(sy) + fabs(y1) + 1.0) * 2.4e-7;
+  const double rx = r + ex, ry = r + ey;
+  const double dxl = fmax(x0 - sx - ex, -rx), dxh = fmin(x1 - sx + ex, rx);
+  const double dyl = fmax(y0 - sy - ey, -ry), dyh = fmin(y1 - sy + ey, ry);
+  if (dxl > dxh || dyl > dyh) return false;
+  double qmin = 0.0;
+  if (!(dxl <= 0.0 && 0.0 <= dxh && dyl <= 0.0 && 0.0 <= dyh)) {
+    qmin = INFINITY;
+    const double X[2] = {dxl, dxh}, Y[2] = {dyl, dyh};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const double yv = fmin(fmax(-B * X[e] / C, dyl), dyh);
+      qmin = fmin(qmin, A * X[e] * X[e] + 2.0 * B * X[e] * yv + C * yv * yv);
+      const double xv = fmin(fmax(-B * Y[e] / A, dxl), dxh);
+      qmin = fmin(qmin, A * xv * xv + 2.0 * B * xv * Y[e] + C * Y[e] * Y[e]);
+    }
+  }
+  const double mx = fmax(fabs(dxl), fabs(dxh)), my = fmax(fabs(dyl), fabs(dyh));
+  const double mag = A * mx * mx + 2.0 * fabs(B) * mx * my + C * my * my;
+  return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
+}
+
+// The work of one wave64 on one tile. off / accum / out_rgba are the
+// view's own (tile-indexed offsets, row-major pixels); pvals index rec.
+template <bool COUNT>
+__device__ inline void rasterize_tile(uint32_t tile, const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
+                                      const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
+                                      uint32_t *__restrict__ out_rgba, const RastParams &p,
+                                      unsigned long long *__restrict__ d_iters, float4 *s_rec, uint32_t *s_mask) {
+  uint32_t iters = 0;
+  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+  const int lane = threadIdx.x;
+  // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
+  // this lane's row in strip k is 4k + lane/16
+  const uint32_t px = tx * kTile + (lane & 15);
+  const uint32_t py0 = ty * kTile + (lane >> 4);
+  const float pxf = (float)px + 0.5f;
+  float cr[4], cg[4], cb[4], T[4], pyf[4];
+  bool live[4], inb[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t y = py0 + 4 * k;
+    inb[k] = px < p.width && y < p.height;
+    pyf[k] = (float)y + 0.5f;
+    cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = 1.0f;
+    if (inb[k] && p.resume) {
+      const float4 a = accum[(size_t)y * p.width + px];
+      cr[k] = a.x; cg[k] = a.y; cb[k] = a.z; T[k] = a.w;
+    }
+    live[k] = inb[k] && !(T[k] < kMinT);
+  }
+  uint32_t consumed = 0;
+  if (p.has_pairs) {
+    const uint32_t start = off[tile], end = off[tile + 1];
+    const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
+    uint32_t stop = end;
+    for (uint32_t base = start; base < end; base += 64) {
+      uint32_t strips_live = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) strips_live |= (__ballot(live[k]) != 0 ? 1u : 0u) << k;
+      if (strips_live == 0) {
+        stop = base;
+        break;
+      }
+      const uint32_t i = base + lane;
+      if (i < end) {
+        const uint32_t j = pvals[i];
+        const float4 a = rec[3 * (size_t)j + 0];
+        const float4 b = rec[3 * (size_t)j + 1];
+        const float4 col = rec[3 * (size_t)j + 2];
+        s_rec[3 * lane + 0] = a;
+        s_rec[3 * lane + 1] = b;
+        s_rec[3 * lane + 2] = col;
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
+          if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+        }
+        s_mask[lane] = m;
+      }
+      __syncthreads();
+      const uint32_t cnt = min(64u, end - base);
+      for (uint32_t c = 0; c < cnt; c++) {
+        if (COUNT) iters += (uint32_t)live[0] + (uint32_t)live[1] + (uint32_t)live[2] + (uint32_t)live[3];
+        const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[c]) & strips_live;
+        if (m == 0) continue;
+        const float4 a = s_rec[3 * c + 0];
+        const float4 b = s_rec[3 * c + 1];
+        const float4 col = s_rec[3 * c + 2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          if (!((m >> k) & 1u)) continue;
+          if (!live[k]) continue;
+          const float dx = pxf - a.x;
+          const float dy = pyf[k] - a.y;
+          const float r = a.z;
+          if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
+          const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
+          if (power > 0.0f || power < col.w) continue;
+          const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf(power) - kGaussFloor));
+          if (alpha < kMinAlpha) continue;
+          const float w = T[k] * alpha;
+          cr[k] = cr[k] + w * col.x;
+          cg[k] = cg[k] + w * col.y;
+          cb[k] = cb[k] + w * col.z;
+          T[k] = T[k] * (1.0f - alpha);
+          live[k] = !(T[k] < kMinT);
+        }
+      }
+      __syncthreads();
+    }
+    consumed = stop - start;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!inb[k]) continue;
+    const size_t pix = (size_t)(py0 + 4 * k) * p.width + px;
+    if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
+    if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
+  }
+  if (COUNT) {
+    uint32_t s = iters;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) {
+      atomicAdd(d_iters, (unsigned long long)s);
+      atomicAdd(d_iters + 1, (unsigned long long)consumed);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dl

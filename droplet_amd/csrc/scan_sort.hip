@@ -121,24 +121,63 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict
 }
 
 // ------------------------------- radix ------------------------------------
+// A sort block owns one 4096-element tile. Unsegmented: tile b is elements
+// [4096 b, 4096 b + 4096) and the count of digit d of block b sits at
+// d * nblocks + b. Segmented (SegGeom::seg_start set): segments are
+// contiguous element ranges from 0, tiles never straddle a segment, and the
+// counts of segment s are laid out [digit][tile of s] from R * blk0[s]
+// (R = digits of the pass), so one flat exclusive scan orders (segment,
+// digit, tile) and every tile's digit base is its absolute output position.
+struct TileSpan {
+  size_t i0;        // first element
+  uint32_t n;       // elements (0: block unused)
+  size_t cbase;     // count index of digit 0
+  uint32_t cstride; // count index step per digit
+};
+
+__device__ inline TileSpan tile_span(const SegGeom &g, size_t n, uint32_t nblocks, uint32_t b, uint32_t ndig) {
+  TileSpan t;
+  if (!g.seg_start) {
+    t.i0 = (size_t)b * kTileN;
+    t.n = t.i0 < n ? (uint32_t)min((size_t)kTileN, n - t.i0) : 0u;
+    t.cbase = b;
+    t.cstride = nblocks;
+    return t;
+  }
+  t.n = 0;
+  t.i0 = 0;
+  t.cbase = 0;
+  t.cstride = 0;
+  if (b >= g.blk0[g.nseg]) return t;
+  const uint32_t sg = g.blk_seg[b];
+  const uint32_t b0 = g.blk0[sg], nb = g.blk0[sg + 1] - b0, k = b - b0;
+  const uint32_t s0 = g.seg_start[sg] + k * (uint32_t)kTileN;
+  t.i0 = s0;
+  t.n = min((uint32_t)kTileN, g.seg_start[sg + 1] - s0);
+  t.cbase = (size_t)ndig * b0 + k;
+  t.cstride = nb;
+  return t;
+}
+
 // Tile layout: wave w owns elements [w*1024, (w+1)*1024) of the tile; item k
 // of lane l is element w*1024 + k*64 + l. Loads are coalesced and the
 // (k, lane) processing order is the input order, which makes ranks stable.
 __global__ __launch_bounds__(kBlock) void k_radix_upsweep(const uint32_t *__restrict__ keys, size_t n, int shift,
                                                           int bits, uint32_t *__restrict__ counts,
-                                                          uint32_t nblocks) {
+                                                          uint32_t nblocks, SegGeom g) {
   __shared__ uint32_t s_hist[256];
   const uint32_t mask = (1u << bits) - 1u;
+  const TileSpan ts = tile_span(g, n, nblocks, blockIdx.x, mask + 1u);
+  if (ts.n == 0 && g.seg_start) return;  // unused capacity block
   s_hist[threadIdx.x] = 0;
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * kTileN;
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
-    size_t i = base + (size_t)k * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&s_hist[(keys[i] >> shift) & mask], 1u);
+    const uint32_t li = (uint32_t)k * kBlock + threadIdx.x;
+    if (li < ts.n) atomicAdd(&s_hist[(keys[ts.i0 + li] >> shift) & mask], 1u);
   }
   __syncthreads();
-  if (threadIdx.x <= mask) counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = s_hist[threadIdx.x];
+  if (threadIdx.x <= mask) counts[ts.cbase + (size_t)threadIdx.x * ts.cstride] = s_hist[threadIdx.x];
 }
 
 template <int BITS>
@@ -147,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
                                                             uint32_t *__restrict__ keys_out,
                                                             uint32_t *__restrict__ vals_out, size_t n, int shift,
                                                             const uint32_t *__restrict__ offsets,
-                                                            uint32_t nblocks) {
+                                                            uint32_t nblocks, SegGeom g) {
   constexpr uint32_t kDigits = 1u << BITS;
   constexpr uint32_t kMask = kDigits - 1u;
   __shared__ uint32_t s_whist[kWaves][256];
@@ -159,8 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
 
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
-  const size_t tile0 = (size_t)blockIdx.x * kTileN;
-  const uint32_t tile_n = (uint32_t)std::min<size_t>(kTileN, n - tile0);
+  const TileSpan ts = tile_span(g, n, nblocks, blockIdx.x, kDigits);
+  if (ts.n == 0) return;
+  const size_t tile0 = ts.i0;
+  const uint32_t tile_n = ts.n;
 
   for (int d = threadIdx.x; d < kWaves * 256; d += kBlock) (&s_whist[0][0])[d] = 0;
   __syncthreads();
@@ -206,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
     uint32_t total;
     uint32_t start = block_exclusive_scan(cnt, s_wave, &total);
     s_start[d] = start;
-    if (d < kDigits) s_gbase[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+    if (d < kDigits) s_gbase[d] = offsets[ts.cbase + (size_t)d * ts.cstride];
   }
   __syncthreads();
 #pragma unroll
@@ -255,8 +296,23 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Segment tables: units[s] = ceil(len(s) / unit), then (after a scan into
+// first[]) owner[u] = s for every unit u of segment s.
+__global__ __launch_bounds__(256) void k_seg_units(const uint32_t *__restrict__ seg_start, uint32_t nseg, uint32_t unit,
+                                                   uint32_t *__restrict__ units) {
+  const uint32_t sg = blockIdx.x * 256 + threadIdx.x;
+  if (sg < nseg) units[sg] = (seg_start[sg + 1] - seg_start[sg] + unit - 1) / unit;
+}
+
+__global__ __launch_bounds__(256) void k_unit_owner(const uint32_t *__restrict__ first, uint32_t nseg,
+                                                    uint32_t *__restrict__ owner) {
+  const uint32_t sg = blockIdx.x * 256 + threadIdx.x;
+  if (sg >= nseg) return;
+  for (uint32_t u = first[sg]; u < first[sg + 1]; u++) owner[u] = sg;
+}
+
 using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
-                        const uint32_t *, uint32_t);
+                        const uint32_t *, uint32_t, SegGeom);
 
 DownFn downsweep_for(int bits) {
   switch (bits) {
@@ -282,7 +338,9 @@ size_t scan_levels_bytes(size_t n) {
 
 void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, uint8_t *ws) {
   if (n == 0) {
-    if (d_total) DL_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
+    // a fill kernel rather than a memset node, so that the scan can be captured in a graph
+    if (d_total) k_fill_u32<<<1, 256, 0, s>>>(d_total, 1, 0u);
+    DL_HIP(hipGetLastError());
     return;
   }
   if (n <= (size_t)kTileN) {
@@ -300,8 +358,9 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
   DL_HIP(hipGetLastError());
 }
 
-void pass_layout(int key_bits, int *passes, int bits[8]) {
-  int p = std::max(1, (key_bits + 7) / 8);
+void pass_layout(int key_bits, int max_bits, int *passes, int bits[32]) {
+  max_bits = std::min(8, std::max(1, max_bits));
+  int p = std::max(1, (key_bits + max_bits - 1) / max_bits);
   int per = (key_bits + p - 1) / p;
   int left = key_bits;
   for (int i = 0; i < p; i++) {
@@ -337,7 +396,7 @@ size_t radix_workspace_bytes(size_t n, int key_bits) {
 }
 
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
-                      size_t n, int key_bits, void *ws) {
+                      size_t n, int key_bits, void *ws, int max_digit_bits) {
   if (n <= 1 || key_bits <= 0) return false;
   if (key_bits > 32) key_bits = 32;
   const uint32_t nb = div_up(n, kTileN);
@@ -346,16 +405,75 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
   uint32_t *counts = reinterpret_cast<uint32_t *>(w);
   uint32_t *offsets = reinterpret_cast<uint32_t *>(w + counts_bytes);
   uint8_t *scan_ws = w + 2 * counts_bytes;
-  int passes, bits[8];
-  pass_layout(key_bits, &passes, bits);
+  int passes, bits[32];
+  pass_layout(key_bits, max_digit_bits, &passes, bits);
+  uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
+  int shift = 0;
+  const SegGeom none{};
+  for (int p = 0; p < passes; p++) {
+    const size_t ndig = (size_t)1 << bits[p];
+    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none);
+    DL_HIP(hipGetLastError());
+    scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
+    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb, none);
+    DL_HIP(hipGetLastError());
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+    shift += bits[p];
+  }
+  return ki == tmp_keys;
+}
+
+int radix_passes(int key_bits, int max_digit_bits) {
+  if (key_bits <= 0) return 0;
+  int passes, bits[32];
+  pass_layout(std::min(32, key_bits), max_digit_bits, &passes, bits);
+  return passes;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t seg_radix_workspace_bytes(size_t max_elems, uint32_t nseg) {
+  const size_t mb = (size_t)div_up(max_elems, kTileN) + nseg + 1;
+  return align256((size_t)nseg * 4 + 4) + align256(((size_t)nseg + 1) * 4) + align256(mb * 4) +
+         2 * align256(256 * mb * 4) + std::max(scan_levels_bytes(nseg), scan_levels_bytes(256 * mb));
+}
+
+bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
+                          const uint32_t *d_seg_start, uint32_t nseg, size_t max_elems, int key_bits,
+                          int max_digit_bits, void *ws) {
+  if (nseg == 0 || key_bits <= 0) return false;
+  if (key_bits > 32) key_bits = 32;
+  const uint32_t mb = div_up(max_elems, kTileN) + nseg + 1;
+  uint8_t *w = static_cast<uint8_t *>(ws);
+  uint32_t *units = reinterpret_cast<uint32_t *>(w);
+  w += align256((size_t)nseg * 4 + 4);
+  uint32_t *blk0 = reinterpret_cast<uint32_t *>(w);
+  w += align256(((size_t)nseg + 1) * 4);
+  uint32_t *blk_seg = reinterpret_cast<uint32_t *>(w);
+  w += align256((size_t)mb * 4);
+  uint32_t *counts = reinterpret_cast<uint32_t *>(w);
+  w += align256(256 * (size_t)mb * 4);
+  uint32_t *offsets = reinterpret_cast<uint32_t *>(w);
+  w += align256(256 * (size_t)mb * 4);
+  uint8_t *scan_ws = w;
+  // tile table: blk0[s] = first tile of segment s, blk0[nseg] = tiles in use
+  k_seg_units<<<div_up(nseg, 256), 256, 0, s>>>(d_seg_start, nseg, (uint32_t)kTileN, units);
+  DL_HIP(hipGetLastError());
+  scan_rec(s, units, blk0, nseg, blk0 + nseg, scan_ws);
+  k_unit_owner<<<div_up(nseg, 256), 256, 0, s>>>(blk0, nseg, blk_seg);
+  DL_HIP(hipGetLastError());
+  const SegGeom g{d_seg_start, blk0, blk_seg, nseg};
+  int passes, bits[32];
+  pass_layout(key_bits, max_digit_bits, &passes, bits);
   uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
   int shift = 0;
   for (int p = 0; p < passes; p++) {
     const size_t ndig = (size_t)1 << bits[p];
-    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb);
+    k_radix_upsweep<<<mb, kBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g);
     DL_HIP(hipGetLastError());
-    scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb);
+    scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
+    downsweep_for(bits[p])<<<mb, kBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g);
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);

@@ -166,6 +166,7 @@ class Renderer:
             "num_pairs": int(st.num_pairs),
             "stage_ms": {name: float(st.stage_ms[i]) for i, name in enumerate(L.STAGES)},
             "total_ms": float(st.total_ms), "pixel_iters": int(st.pixel_iters),
+            "pairs_consumed": int(st.pairs_consumed),
         }
 
     def tap(self, name: str) -> np.ndarray:

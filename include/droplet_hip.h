@@ -133,6 +133,7 @@ typedef struct {
     float stage_ms[DL_NUM_STAGES]; /* hipEvent times on the renderer's stream (timing enabled) */
     float total_ms;
     uint64_t pixel_iters;        /* rasterize loop iterations over all pixels (counting enabled) */
+    uint64_t pairs_consumed;     /* tile-list entries read before each tile saturated (counting enabled) */
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
 /* Counts rasterize loop iterations (the work unit of the rasterize roofline)

@@ -204,11 +204,13 @@ def test_rasterize_work_count_matches_oracle(device):
     try:
         r.set_counting(True)
         img = r.render(data, cam)
-        iters = r.stats()["pixel_iters"]
+        st = r.stats()
+        iters = st["pixel_iters"]
     finally:
         r.destroy()
     ref = oracle.render(data, cam, taps=False)
     assert iters == ref["pixel_iters"]
+    assert 0 < st["pairs_consumed"] <= st["num_pairs"]
     np.testing.assert_array_equal(img, ref["rgba"])
 
 
