@@ -68,49 +68,117 @@ __device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t ti
   return R;
 }
 
+// Depth-ordered splat records of one chunk: (id, x0 | x1 << 16, y0 |
+// full_rows << 16, rem). One gather of the id-ordered projection records, so
+// that both level-1 passes read coalesced 16-byte records.
+__device__ inline uint4 pack_rect(uint32_t id, const Rect &R) {
+  return make_uint4(id, R.x0 | (R.x1 << 16), R.y0 | (R.full_rows << 16), R.rem);
+}
+
+__device__ inline Rect unpack_rect(const uint4 p) {
+  Rect R;
+  R.x0 = p.y & 0xFFFFu;
+  R.x1 = p.y >> 16;
+  R.y0 = p.z & 0xFFFFu;
+  R.full_rows = p.z >> 16;
+  R.rem = p.w;
+  R.nrows = R.full_rows + (R.rem > 0 ? 1u : 0u);
+  return R;
+}
+
+__global__ __launch_bounds__(256) void k_rects(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
+                                               uint32_t cn, BinGrid g, uint4 *__restrict__ rects) {
+  const uint32_t jj = blockIdx.x * 256 + threadIdx.x;
+  if (jj >= cn) return;
+  const uint32_t id = ids[jj];
+  rects[jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], g.tiles_x, g.tiles_y, g.max_cov));
+}
+
+// Does row y (virtual) hold a live tile in columns [xa, xb]? Without a map
+// every tile is live.
+__device__ inline bool live_range(const LiveMap &L, uint32_t y, uint32_t xa, uint32_t xb) {
+  if (!L.pre) return true;
+  const uint16_t *r = L.pre + (size_t)y * L.stride;
+  return r[xb + 1] != r[xa];
+}
+
+// The level-1 view that owns level-1 block b: the single view `one`, or a
+// binary search of the batch table by first block.
+__device__ inline L1View l1_view(const L1View &one, const L1View *__restrict__ tab, uint32_t ntab, uint32_t b) {
+  if (!tab) return one;
+  uint32_t lo = 0, hi = ntab - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (tab[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return tab[lo];
+}
+
 // ---------------------------- level 1 -------------------------------------
-__global__ __launch_bounds__(kB1) void k_rows_count(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
-                                                    uint32_t cn, BinGrid g, uint32_t nb1, uint32_t *__restrict__ c1) {
-  extern __shared__ uint32_t s_mask[];  // [tiles_y][kW1]
+// Level-1 block b of view v covers items [item0 + 256 k, ...) with k = b -
+// blk0, and rows [row0, row0 + rows); its count of row r sits at c1_base +
+// (r - row0) * nb1 + k, so the flat scan orders (view, row, block).
+__global__ __launch_bounds__(kB1) void k_rows_count(const uint4 *__restrict__ rects, L1View one,
+                                                    const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
+                                                    uint32_t *__restrict__ c1) {
+  extern __shared__ uint32_t s_mask[];  // [rows][kW1]
+  const L1View v = l1_view(one, tab, ntab, blockIdx.x);
   const uint32_t t = threadIdx.x;
-  const uint32_t jj = blockIdx.x * kB1 + t;
-  for (uint32_t i = t; i < g.tiles_y * kW1; i += kB1) s_mask[i] = 0;
+  const uint32_t k_blk = blockIdx.x - v.blk0;
+  const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  const uint32_t jj = k_blk * kB1 + t;
+  for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
   __syncthreads();
-  if (jj < cn) {
-    const Rect R = splat_rect(rec[3 * (size_t)ids[jj]], g.tiles_x, g.tiles_y, g.max_cov);
-    for (uint32_t k = 0; k < R.nrows; k++) atomicOr(&s_mask[(R.y0 + k) * kW1 + (t >> 5)], 1u << (t & 31));
+  if (jj < v.n) {
+    const Rect R = unpack_rect(rects[v.item0 + jj]);
+    for (uint32_t k = 0; k < R.nrows; k++) {
+      const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
+      if (live_range(live, v.row0 + R.y0 + k, R.x0, xb))
+        atomicOr(&s_mask[(R.y0 + k) * kW1 + (t >> 5)], 1u << (t & 31));
+    }
   }
   __syncthreads();
-  for (uint32_t y = t; y < g.tiles_y; y += kB1) {
+  for (uint32_t y = t; y < v.rows; y += kB1) {
     uint32_t c = 0;
 #pragma unroll
     for (int w = 0; w < kW1; w++) c += __popc(s_mask[y * kW1 + w]);
-    c1[(size_t)y * nb1 + blockIdx.x] = c;
+    c1[(size_t)v.c1_base + (size_t)y * nb1 + k_blk] = c;
   }
 }
 
-__global__ __launch_bounds__(kB1) void k_rows_scatter(const float4 *__restrict__ rec,
-                                                      const uint32_t *__restrict__ ids, uint32_t cn, BinGrid g,
-                                                      uint32_t nb1, const uint32_t *__restrict__ base1,
+__global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ rects, L1View one,
+                                                      const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
+                                                      const uint32_t *__restrict__ base1, uint32_t cap,
                                                       uint2 *__restrict__ spans) {
   extern __shared__ uint32_t s_dyn[];
-  uint32_t *s_mask = s_dyn;                                                  // [tiles_y][kW1]
-  uint32_t *s_base = s_dyn + g.tiles_y * kW1;                                // [tiles_y]
-  uint16_t *s_pre = reinterpret_cast<uint16_t *>(s_base + g.tiles_y);        // [tiles_y][kW1]
+  const L1View v = l1_view(one, tab, ntab, blockIdx.x);
+  uint32_t *s_mask = s_dyn;                                                  // [rows][kW1]
+  uint32_t *s_base = s_dyn + v.rows * kW1;                                   // [rows]
+  uint16_t *s_pre = reinterpret_cast<uint16_t *>(s_base + v.rows);           // [rows][kW1]
   const uint32_t t = threadIdx.x;
-  const uint32_t jj = blockIdx.x * kB1 + t;
-  for (uint32_t i = t; i < g.tiles_y * kW1; i += kB1) s_mask[i] = 0;
-  for (uint32_t y = t; y < g.tiles_y; y += kB1) s_base[y] = base1[(size_t)y * nb1 + blockIdx.x];
+  const uint32_t k_blk = blockIdx.x - v.blk0;
+  const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  const uint32_t jj = k_blk * kB1 + t;
+  for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
+  for (uint32_t y = t; y < v.rows; y += kB1) s_base[y] = base1[(size_t)v.c1_base + (size_t)y * nb1 + k_blk];
   __syncthreads();
   Rect R{0, 0, 0, 0, 0, 0};
   uint32_t id = 0;
-  if (jj < cn) {
-    id = ids[jj];
-    R = splat_rect(rec[3 * (size_t)id], g.tiles_x, g.tiles_y, g.max_cov);
-    for (uint32_t k = 0; k < R.nrows; k++) atomicOr(&s_mask[(R.y0 + k) * kW1 + (t >> 5)], 1u << (t & 31));
+  uint32_t rows_live = 0;  // bit k: row R.y0 + k has a live tile in the span (nrows <= 32 rows tracked)
+  if (jj < v.n) {
+    const uint4 pr = rects[v.item0 + jj];
+    id = pr.x;
+    R = unpack_rect(pr);
+    for (uint32_t k = 0; k < R.nrows; k++) {
+      const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
+      if (live_range(live, v.row0 + R.y0 + k, R.x0, xb)) {
+        atomicOr(&s_mask[(R.y0 + k) * kW1 + (t >> 5)], 1u << (t & 31));
+        if (k < 32) rows_live |= 1u << k;
+      }
+    }
   }
   __syncthreads();
-  for (uint32_t y = t; y < g.tiles_y; y += kB1) {
+  for (uint32_t y = t; y < v.rows; y += kB1) {
     uint32_t run = 0;
 #pragma unroll
     for (int w = 0; w < kW1; w++) {
@@ -119,25 +187,42 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const float4 *__restrict__
     }
   }
   __syncthreads();
-  if (jj < cn) {
+  if (jj < v.n) {
     const uint32_t word = t >> 5, below = (1u << (t & 31)) - 1u;
     for (uint32_t k = 0; k < R.nrows; k++) {
+      const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
+      if (k < 32 ? !((rows_live >> k) & 1u) : !live_range(live, v.row0 + R.y0 + k, R.x0, xb)) continue;
       const uint32_t y = R.y0 + k;
       const uint32_t rank = s_pre[y * kW1 + word] + __popc(s_mask[y * kW1 + word] & below);
-      const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
-      spans[(size_t)s_base[y] + rank] = make_uint2(id, R.x0 | (xb << 16));
+      const uint32_t pos = s_base[y] + rank;
+      if (pos < cap) spans[pos] = make_uint2(id, R.x0 | (xb << 16));
     }
   }
 }
 
-// row_start[r] = spans before row r (the (row r, block 0) entry of the
-// level-1 scan), row_start[tiles_y] = all spans
-__global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, uint32_t nb1,
-                                                   uint32_t tiles_y, const uint32_t *__restrict__ d_num_spans,
+// row_start[r] = spans before virtual row r (the (row r, block 0) entry of
+// the level-1 scan, or the running total for a view without blocks),
+// row_start[rows] = all spans; clamped to the span capacity cap
+__global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, size_t n1, L1View one,
+                                                   const L1View *__restrict__ tab, uint32_t ntab, uint32_t rows,
+                                                   const uint32_t *__restrict__ d_num_spans, uint32_t cap,
                                                    uint32_t *__restrict__ row_start) {
   const uint32_t r = blockIdx.x * 256 + threadIdx.x;
-  if (r < tiles_y) row_start[r] = base1[(size_t)r * nb1];
-  if (r == tiles_y) row_start[r] = *d_num_spans;
+  const uint32_t total = min(*d_num_spans, cap);
+  if (r == rows) row_start[r] = total;
+  if (r >= rows) return;
+  L1View v = one;
+  if (tab) {
+    uint32_t lo = 0, hi = ntab - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (tab[mid].row0 <= r) lo = mid; else hi = mid - 1;
+    }
+    v = tab[lo];
+  }
+  const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  const size_t idx = (size_t)v.c1_base + (size_t)(r - v.row0) * nb1;
+  row_start[r] = idx < n1 ? min(base1[idx], cap) : total;
 }
 
 // ----------------------- MBS chunk tables ---------------------------------
@@ -147,33 +232,58 @@ __global__ __launch_bounds__(256) void k_seg_chunks(const uint32_t *__restrict__
   if (s < nseg) nchunks[s] = (seg_start[s + 1] - seg_start[s] + kChunk - 1) / kChunk;
 }
 
-__global__ __launch_bounds__(256) void k_seg_of(const uint32_t *__restrict__ chunk_base, uint32_t nseg,
+__global__ __launch_bounds__(256) void k_seg_of(const uint32_t *__restrict__ chunk_base, uint32_t nseg, uint32_t cap,
                                                 uint32_t *__restrict__ seg_of) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= nseg) return;
-  for (uint32_t c = chunk_base[s]; c < chunk_base[s + 1]; c++) seg_of[c] = s;
+  const uint32_t e = min(chunk_base[s + 1], cap);
+  for (uint32_t c = chunk_base[s]; c < e; c++) seg_of[c] = s;
 }
 
 // ---------------------------- MBS -----------------------------------------
 struct MbsGeom {
   const uint32_t *seg_of;      // chunk -> segment
-  const uint32_t *chunk_base;  // segment -> first chunk (nseg + 1 entries)
+  const uint32_t *chunk_base;  // segment -> first chunk (nseg + 1 entries; [nseg] = chunks)
   const uint32_t *seg_start;   // segment -> first item (nseg + 1 entries)
   uint32_t groups;             // 16-column groups per row (level 2b: segment = row * groups + group)
+  uint32_t nseg;
+  uint32_t chunk_cap;          // capacity of the chunk table
+  uint32_t out_cap;            // capacity of the output array
+  LiveMap live;                // tiles still open (no map: all)
 };
 
-// buckets [lo, hi] of an item; lo > hi: none
+// chunks of this level (read on the device), clamped to the table capacity
+__device__ inline uint32_t mbs_chunks(const MbsGeom &G) { return min(G.chunk_base[G.nseg], G.chunk_cap); }
+
+// Buckets of an item as a 16-bit mask: the groups (level 0, segment = row)
+// or the columns of the segment's group (level 1, segment = row * groups +
+// group) that the item's column range [xa, xb] covers and that hold a live
+// tile.
 template <int LEVEL>
-__device__ inline void item_buckets(uint2 it, uint32_t seg, uint32_t groups, uint32_t &lo, uint32_t &hi) {
+__device__ inline uint32_t item_buckets(uint2 it, uint32_t seg, const MbsGeom &G) {
   const uint32_t xa = it.y & 0xFFFFu, xb = it.y >> 16;
+  uint32_t lo, hi, row, g0;
   if (LEVEL == 0) {
     lo = xa >> 4;
     hi = xb >> 4;
+    row = seg;
+    g0 = 0;
   } else {
-    const uint32_t g0 = (seg % groups) * 16u;
+    g0 = (seg % G.groups) * 16u;
     lo = max(xa, g0) - g0;
     hi = min(xb, g0 + 15u) - g0;
+    row = seg / G.groups;
   }
+  if (lo > hi) return 0;
+  const uint32_t span = ((hi - lo + 1u) >= 32u ? 0xFFFFFFFFu : ((1u << (hi - lo + 1u)) - 1u)) << lo;
+  if (!G.live.pre) return span & 0xFFFFu;
+  uint32_t m = 0;
+  for (uint32_t b = lo; b <= hi; b++) {
+    const uint32_t c0 = LEVEL == 0 ? max(xa, 16u * b) : g0 + b;
+    const uint32_t c1 = LEVEL == 0 ? min(xb, 16u * b + 15u) : g0 + b;
+    if (live_range(G.live, row, c0, c1)) m |= 1u << b;
+  }
+  return m;
 }
 
 struct ChunkSpan {
@@ -193,11 +303,15 @@ __device__ inline ChunkSpan chunk_span(const MbsGeom &G, uint32_t c) {
 
 // counts[(16 * cb + b) * ... ] laid out per segment as [bucket][chunk]:
 // index 16*cb + b*nch + k, so a flat exclusive scan orders (segment, bucket, chunk)
+// Both MBS kernels run a fixed grid that strides over the chunks, whose
+// number is only known on the device.
 template <int LEVEL>
 __global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ items, MbsGeom G,
                                                    uint32_t *__restrict__ counts) {
   __shared__ uint32_t s_cnt[4][kK];
-  const ChunkSpan cs = chunk_span(G, blockIdx.x);
+  const uint32_t nch = mbs_chunks(G);
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+  const ChunkSpan cs = chunk_span(G, c);
   const int lane = __lane_id(), w = threadIdx.x >> 6;
   uint32_t cnt[kK];
 #pragma unroll
@@ -205,10 +319,9 @@ __global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ ite
 #pragma unroll
   for (int step = 0; step < 4; step++) {
     const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
-    uint32_t lo = 1, hi = 0;
-    if (i < cs.iend) item_buckets<LEVEL>(items[i], cs.seg, G.groups, lo, hi);
+    const uint32_t bm = i < cs.iend ? item_buckets<LEVEL>(items[i], cs.seg, G) : 0u;
 #pragma unroll
-    for (int b = 0; b < kK; b++) cnt[b] += (uint32_t)__popcll(__ballot(lo <= (uint32_t)b && (uint32_t)b <= hi));
+    for (int b = 0; b < kK; b++) cnt[b] += (uint32_t)__popcll(__ballot((bm >> b) & 1u));
   }
   if (lane == 0) {
 #pragma unroll
@@ -219,6 +332,8 @@ __global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ ite
     const uint32_t b = threadIdx.x;
     counts[(size_t)kK * cs.cb + (size_t)b * cs.nch + cs.k] = s_cnt[0][b] + s_cnt[1][b] + s_cnt[2][b] + s_cnt[3][b];
   }
+  __syncthreads();
+  }
 }
 
 template <int LEVEL>
@@ -226,27 +341,27 @@ __global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ ite
                                                    const uint32_t *__restrict__ base, uint2 *__restrict__ out_items,
                                                    uint32_t *__restrict__ out_ids) {
   __shared__ uint32_t s_cnt[4][kK];
-  const ChunkSpan cs = chunk_span(G, blockIdx.x);
+  const uint32_t nch = mbs_chunks(G);
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+  const ChunkSpan cs = chunk_span(G, c);
   const int lane = __lane_id(), w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
   uint2 it[4];
-  uint32_t lo[4], hi[4];
+  uint32_t bm[4];
   uint32_t cnt[kK];
 #pragma unroll
   for (int b = 0; b < kK; b++) cnt[b] = 0;
 #pragma unroll
   for (int step = 0; step < 4; step++) {
     const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
-    lo[step] = 1;
-    hi[step] = 0;
+    bm[step] = 0;
     it[step] = make_uint2(0u, 0u);
     if (i < cs.iend) {
       it[step] = items[i];
-      item_buckets<LEVEL>(it[step], cs.seg, G.groups, lo[step], hi[step]);
+      bm[step] = item_buckets<LEVEL>(it[step], cs.seg, G);
     }
 #pragma unroll
-    for (int b = 0; b < kK; b++)
-      cnt[b] += (uint32_t)__popcll(__ballot(lo[step] <= (uint32_t)b && (uint32_t)b <= hi[step]));
+    for (int b = 0; b < kK; b++) cnt[b] += (uint32_t)__popcll(__ballot((bm[step] >> b) & 1u));
   }
   if (lane == 0) {
 #pragma unroll
@@ -264,17 +379,21 @@ __global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ ite
   for (int step = 0; step < 4; step++) {
 #pragma unroll
     for (int b = 0; b < kK; b++) {
-      const bool in = lo[step] <= (uint32_t)b && (uint32_t)b <= hi[step];
+      const bool in = (bm[step] >> b) & 1u;
       const uint64_t m = __ballot(in);
       if (in) {
         const uint32_t pos = run[b] + (uint32_t)__popcll(m & lt);
-        if (LEVEL == 0)
-          out_items[pos] = it[step];
-        else
-          out_ids[pos] = it[step].x;
+        if (pos < G.out_cap) {
+          if (LEVEL == 0)
+            out_items[pos] = it[step];
+          else
+            out_ids[pos] = it[step].x;
+        }
       }
       run[b] += (uint32_t)__popcll(m);
     }
+  }
+  __syncthreads();
   }
 }
 
@@ -283,16 +402,19 @@ __global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ ite
 // where it would start, or the total at the end. Level 2a: o = row*groups +
 // group. Level 2b: o = tile = row*tiles_x + x -> (row*groups + x/16, x%16).
 __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restrict__ base,
-                                                        const uint32_t *__restrict__ chunk_base, uint32_t n_out,
-                                                        uint32_t groups, uint32_t tiles_x, int tile_mode,
-                                                        size_t ncounts, const uint32_t *__restrict__ d_total,
+                                                        const uint32_t *__restrict__ chunk_base, uint32_t nseg,
+                                                        uint32_t chunk_cap, uint32_t n_out, uint32_t groups,
+                                                        uint32_t tiles_x, int tile_mode,
+                                                        const uint32_t *__restrict__ d_total, uint32_t out_cap,
                                                         uint32_t *__restrict__ out) {
   const uint32_t o = blockIdx.x * 256 + threadIdx.x;
   if (o > n_out) return;
+  const uint32_t total = min(*d_total, out_cap);
   if (o == n_out) {
-    out[o] = *d_total;
+    out[o] = total;
     return;
   }
+  const size_t ncounts = (size_t)kK * min(chunk_base[nseg], chunk_cap);
   uint32_t seg, b;
   if (tile_mode) {
     const uint32_t r = o / tiles_x, x = o % tiles_x;
@@ -304,15 +426,15 @@ __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restri
   }
   const uint32_t cb = chunk_base[seg], nch = chunk_base[seg + 1] - cb;
   const size_t pos = (size_t)kK * cb + (size_t)b * nch;
-  out[o] = pos < ncounts ? base[pos] : *d_total;
+  out[o] = pos < ncounts ? min(base[pos], total) : total;
 }
 
-// Chunk table of one MBS level: nchunks per segment, its scan (chunk_base,
-// with the total in chunk_base[nseg]) and chunk -> segment. Returns the
-// number of chunks (host sync).
-uint32_t build_chunk_table(hipStream_t s, const uint32_t *seg_start, uint32_t nseg, DeviceBuffer &tmp,
-                           DeviceBuffer &chunk_base, DeviceBuffer &seg_of, DeviceBuffer &scan_ws,
-                           uint32_t max_chunks) {
+// Chunk table of one MBS level, built on the device: chunks per segment,
+// their scan (chunk_base, with the total in chunk_base[nseg]) and chunk ->
+// segment. max_chunks bounds the total: every segment of len items has
+// ceil(len / kChunk) <= len / kChunk + 1 chunks.
+void build_chunk_table(hipStream_t s, const uint32_t *seg_start, uint32_t nseg, DeviceBuffer &tmp,
+                       DeviceBuffer &chunk_base, DeviceBuffer &seg_of, DeviceBuffer &scan_ws, uint32_t max_chunks) {
   tmp.reserve((size_t)nseg * 4 + 4);
   chunk_base.reserve(((size_t)nseg + 1) * 4);
   seg_of.reserve((size_t)std::max<uint32_t>(max_chunks, 1) * 4);
@@ -321,14 +443,12 @@ uint32_t build_chunk_table(hipStream_t s, const uint32_t *seg_start, uint32_t ns
   DL_HIP(hipGetLastError());
   exclusive_scan_u32(s, tmp.as<uint32_t>(), chunk_base.as<uint32_t>(), nseg, chunk_base.as<uint32_t>() + nseg,
                      scan_ws.get());
-  uint32_t total = 0;
-  DL_HIP(hipMemcpyAsync(&total, chunk_base.as<uint32_t>() + nseg, 4, hipMemcpyDeviceToHost, s));
-  DL_HIP(hipStreamSynchronize(s));
-  if (total > max_chunks) fail(DL_ERR_INTERNAL, "span binning: chunk table overflow");
-  k_seg_of<<<div_up(nseg, 256), 256, 0, s>>>(chunk_base.as<uint32_t>(), nseg, seg_of.as<uint32_t>());
+  k_seg_of<<<div_up(nseg, 256), 256, 0, s>>>(chunk_base.as<uint32_t>(), nseg, max_chunks, seg_of.as<uint32_t>());
   DL_HIP(hipGetLastError());
-  return total;
 }
+
+// fixed grid of the chunk-striding MBS kernels
+uint32_t mbs_grid(uint32_t max_chunks) { return std::max<uint32_t>(1, std::min<uint32_t>(max_chunks, 4096)); }
 
 }  // namespace
 
@@ -336,80 +456,107 @@ bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y) {
   return tiles_x >= 1 && tiles_y >= 1 && tiles_x <= kMaxBinTilesX && tiles_y <= kMaxBinTilesY;
 }
 
-void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
-              BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small) {
-  const uint32_t num_tiles = g.tiles_x * g.tiles_y;
-  const uint32_t groups = div_up(g.tiles_x, 16);
+void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals,
+                    uint32_t *d_small) {
+  const uint32_t groups = div_up(a.cols, 16);
+  const uint32_t num_vtiles = a.rows * a.cols;
+  const uint32_t cap = std::max<uint32_t>(a.pair_cap, 1);
   uint32_t *d_num_spans = d_small;
   uint32_t *d_num_gspans = d_small + 1;
   uint32_t *d_npairs = d_small + 2;
   // level 1: splats -> row spans (at most one span per pair)
-  const uint32_t nb1 = div_up(cn, kB1);
-  const size_t n1 = (size_t)g.tiles_y * nb1;
+  const size_t n1 = std::max<size_t>(a.n1, 1);
   ws.c1.reserve(n1 * 4);
   ws.base1.reserve(n1 * 4);
-  ws.spans.reserve((size_t)std::max<uint32_t>(npairs, 1) * 8);
+  ws.spans.reserve((size_t)cap * 8);
   ws.scan_ws.reserve(scan_workspace_bytes(n1));
-  k_rows_count<<<nb1, kB1, (size_t)g.tiles_y * kW1 * 4, s>>>(rec, ids, cn, g, nb1, ws.c1.as<uint32_t>());
-  DL_HIP(hipGetLastError());
-  exclusive_scan_u32(s, ws.c1.as<uint32_t>(), ws.base1.as<uint32_t>(), n1, d_num_spans, ws.scan_ws.get());
-  const size_t lds1 = (size_t)g.tiles_y * kW1 * 4 + (size_t)g.tiles_y * 4 + (size_t)g.tiles_y * kW1 * 2;
-  k_rows_scatter<<<nb1, kB1, lds1, s>>>(rec, ids, cn, g, nb1, ws.base1.as<uint32_t>(), ws.spans.as<uint2>());
-  DL_HIP(hipGetLastError());
-  ws.row_start.reserve(((size_t)g.tiles_y + 1) * 4);
-  k_row_start<<<div_up((size_t)g.tiles_y + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), nb1, g.tiles_y, d_num_spans,
-                                                                ws.row_start.as<uint32_t>());
+  if (a.l1_blocks) {
+    k_rows_count<<<a.l1_blocks, kB1, (size_t)a.max_view_rows * kW1 * 4, s>>>(a.rects, a.one, a.tab, a.ntab, a.live,
+                                                                            ws.c1.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+  }
+  exclusive_scan_u32(s, ws.c1.as<uint32_t>(), ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, d_num_spans,
+                     ws.scan_ws.get());
+  if (a.l1_blocks) {
+    const size_t lds1 = (size_t)a.max_view_rows * (kW1 * 4 + 4 + kW1 * 2);
+    k_rows_scatter<<<a.l1_blocks, kB1, lds1, s>>>(a.rects, a.one, a.tab, a.ntab, a.live, ws.base1.as<uint32_t>(), cap,
+                                                  ws.spans.as<uint2>());
+    DL_HIP(hipGetLastError());
+  }
+  ws.row_start.reserve(((size_t)a.rows + 1) * 4);
+  k_row_start<<<div_up((size_t)a.rows + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, a.one,
+                                                              a.tab, a.ntab, a.rows, d_num_spans, cap,
+                                                              ws.row_start.as<uint32_t>());
   DL_HIP(hipGetLastError());
 
   // level 2a: row spans -> 16-column groups (segments = rows)
-  const uint32_t max_ch_a = div_up(npairs, kChunk) + g.tiles_y;
-  const uint32_t nch_a = build_chunk_table(s, ws.row_start.as<uint32_t>(), g.tiles_y, ws.tmp, ws.chunk_base_a,
-                                           ws.seg_of_a, ws.scan_ws, max_ch_a);
-  MbsGeom ga{ws.seg_of_a.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), ws.row_start.as<uint32_t>(), groups};
-  const size_t nc_a = (size_t)kK * nch_a;
-  ws.c2.reserve(std::max<size_t>(nc_a, 1) * 4);
-  ws.base2.reserve(std::max<size_t>(nc_a, 1) * 4);
-  ws.gspans.reserve((size_t)std::max<uint32_t>(npairs, 1) * 8);
-  ws.scan_ws.reserve(scan_workspace_bytes(std::max<size_t>(nc_a, 1)));
-  if (nch_a) {
-    k_mbs_count<0><<<nch_a, 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.c2.as<uint32_t>());
-    DL_HIP(hipGetLastError());
-  }
-  exclusive_scan_u32(s, ws.c2.as<uint32_t>(), ws.base2.as<uint32_t>(), nc_a, d_num_gspans, ws.scan_ws.get());
-  if (nch_a) {
-    k_mbs_write<0><<<nch_a, 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.base2.as<uint32_t>(), ws.gspans.as<uint2>(),
-                                         nullptr);
-    DL_HIP(hipGetLastError());
-  }
-  const uint32_t nseg_b = g.tiles_y * groups;
+  const uint32_t max_ch_a = div_up(cap, kChunk) + a.rows;
+  build_chunk_table(s, ws.row_start.as<uint32_t>(), a.rows, ws.tmp, ws.chunk_base_a, ws.seg_of_a, ws.scan_ws,
+                    max_ch_a);
+  MbsGeom ga{ws.seg_of_a.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), ws.row_start.as<uint32_t>(), groups,
+             a.rows, max_ch_a, cap, a.live};
+  const size_t nc_a = (size_t)kK * max_ch_a;
+  ws.c2.reserve(nc_a * 4);
+  ws.base2.reserve(nc_a * 4);
+  ws.gspans.reserve((size_t)cap * 8);
+  ws.scan_ws.reserve(scan_workspace_bytes(nc_a));
+  k_mbs_count<0><<<mbs_grid(max_ch_a), 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.c2.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+  exclusive_scan_u32_dev(s, ws.c2.as<uint32_t>(), ws.base2.as<uint32_t>(), nc_a, ws.chunk_base_a.as<uint32_t>() + a.rows,
+                         kK, d_num_gspans, ws.scan_ws.get());
+  k_mbs_write<0><<<mbs_grid(max_ch_a), 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.base2.as<uint32_t>(),
+                                                    ws.gspans.as<uint2>(), nullptr);
+  DL_HIP(hipGetLastError());
+  const uint32_t nseg_b = a.rows * groups;
   ws.seg_start_b.reserve(((size_t)nseg_b + 1) * 4);
-  k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(),
-                                                                   nseg_b, groups, g.tiles_x, 0, nc_a, d_num_gspans,
-                                                                   ws.seg_start_b.as<uint32_t>());
+  k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(
+      ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), a.rows, max_ch_a, nseg_b, groups, a.cols, 0,
+      d_num_gspans, cap, ws.seg_start_b.as<uint32_t>());
   DL_HIP(hipGetLastError());
 
   // level 2b: group spans -> columns (segments = (row, group))
-  const uint32_t max_ch_b = div_up(npairs, kChunk) + nseg_b;
-  const uint32_t nch_b = build_chunk_table(s, ws.seg_start_b.as<uint32_t>(), nseg_b, ws.tmp, ws.chunk_base_b,
-                                           ws.seg_of_b, ws.scan_ws, max_ch_b);
-  MbsGeom gb{ws.seg_of_b.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), ws.seg_start_b.as<uint32_t>(), groups};
-  const size_t nc_b = (size_t)kK * nch_b;
-  ws.c3.reserve(std::max<size_t>(nc_b, 1) * 4);
-  ws.base3.reserve(std::max<size_t>(nc_b, 1) * 4);
-  ws.scan_ws.reserve(scan_workspace_bytes(std::max<size_t>(nc_b, 1)));
-  if (nch_b) {
-    k_mbs_count<1><<<nch_b, 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.c3.as<uint32_t>());
-    DL_HIP(hipGetLastError());
-  }
-  exclusive_scan_u32(s, ws.c3.as<uint32_t>(), ws.base3.as<uint32_t>(), nc_b, d_npairs, ws.scan_ws.get());
-  if (nch_b) {
-    k_mbs_write<1><<<nch_b, 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.base3.as<uint32_t>(), nullptr, pvals);
-    DL_HIP(hipGetLastError());
-  }
-  k_bucket_offsets<<<div_up((size_t)num_tiles + 1, 256), 256, 0, s>>>(ws.base3.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(),
-                                                                      num_tiles, groups, g.tiles_x, 1, nc_b, d_npairs,
-                                                                      d_off);
+  const uint32_t max_ch_b = div_up(cap, kChunk) + nseg_b;
+  build_chunk_table(s, ws.seg_start_b.as<uint32_t>(), nseg_b, ws.tmp, ws.chunk_base_b, ws.seg_of_b, ws.scan_ws,
+                    max_ch_b);
+  MbsGeom gb{ws.seg_of_b.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), ws.seg_start_b.as<uint32_t>(), groups,
+             nseg_b, max_ch_b, cap, a.live};
+  const size_t nc_b = (size_t)kK * max_ch_b;
+  ws.c3.reserve(nc_b * 4);
+  ws.base3.reserve(nc_b * 4);
+  ws.scan_ws.reserve(scan_workspace_bytes(nc_b));
+  k_mbs_count<1><<<mbs_grid(max_ch_b), 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.c3.as<uint32_t>());
   DL_HIP(hipGetLastError());
+  exclusive_scan_u32_dev(s, ws.c3.as<uint32_t>(), ws.base3.as<uint32_t>(), nc_b, ws.chunk_base_b.as<uint32_t>() + nseg_b,
+                         kK, d_npairs, ws.scan_ws.get());
+  k_mbs_write<1><<<mbs_grid(max_ch_b), 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.base3.as<uint32_t>(), nullptr,
+                                                    pvals);
+  DL_HIP(hipGetLastError());
+  k_bucket_offsets<<<div_up((size_t)num_vtiles + 1, 256), 256, 0, s>>>(
+      ws.base3.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), nseg_b, max_ch_b, num_vtiles, groups, a.cols, 1,
+      d_npairs, cap, d_off);
+  DL_HIP(hipGetLastError());
+}
+
+void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
+              const LiveMap &live, BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small) {
+  ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
+  if (cn) {
+    k_rects<<<div_up(cn, 256), 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());
+    DL_HIP(hipGetLastError());
+  }
+  SpanBinArgs a;
+  a.rects = ws.rects.as<uint4>();
+  a.one = L1View{0, cn, 0, 0, g.tiles_y, 0};
+  a.tab = nullptr;
+  a.ntab = 0;
+  a.l1_blocks = div_up(cn, kB1);
+  a.n1 = (size_t)g.tiles_y * a.l1_blocks;
+  a.rows = g.tiles_y;
+  a.max_view_rows = g.tiles_y;
+  a.cols = g.tiles_x;
+  a.pair_cap = npairs;
+  a.live = live;
+  span_bin_rects(s, a, ws, d_off, pvals, d_small);
 }
 
 }  // namespace dl

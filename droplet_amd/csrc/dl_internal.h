@@ -85,6 +85,10 @@ struct Context {
 size_t scan_workspace_bytes(size_t n);
 void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n,
                         uint32_t *d_total, void *ws);
+// The same with the element count on the device: min(n_cap, d_n[0] * mult)
+// elements; launches are sized by n_cap and ws holds scan_workspace_bytes(n_cap).
+void exclusive_scan_u32_dev(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n_cap,
+                            const uint32_t *d_n, uint32_t mult, uint32_t *d_total, void *ws);
 
 // Stable LSD radix sort of (key, value) pairs on bits [0, key_bits) of the
 // key. Results end in either (keys, vals) or (tmp_keys, tmp_vals); the

@@ -392,7 +392,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       if (span) {
         // stable two-level bucketing straight into per-tile lists (binning.hip)
         mark(DL_STAGE_EMIT);
-        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, bin_ws,
+        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, LiveMap{}, bin_ws,
                  ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin);
       } else {
         pkeys.reserve((size_t)npairs * 4);

@@ -59,10 +59,21 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave, ui
 }
 
 // -------------------------------- scan ------------------------------------
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in, size_t n,
-                                                        uint32_t *__restrict__ partials) {
+// Element count of a scan level: the static n, or with a device count
+// min(n, d_n[0] * mult) (a level sized on the device, capacity n).
+__device__ inline size_t level_n(size_t n, const uint32_t *d_n, uint32_t mult) {
+  return d_n ? min(n, (size_t)d_n[0] * mult) : n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in, size_t n_cap,
+                                                        uint32_t *__restrict__ partials,
+                                                        const uint32_t *__restrict__ d_n, uint32_t mult,
+                                                        uint32_t *__restrict__ d_next_n) {
   __shared__ uint32_t s_wave[kWaves];
+  const size_t n = level_n(n_cap, d_n, mult);
+  if (d_next_n && blockIdx.x == 0 && threadIdx.x == 0) *d_next_n = (uint32_t)((n + kTileN - 1) / kTileN);
   const size_t base = (size_t)blockIdx.x * kTileN;
+  if (base >= n) return;
   uint32_t acc = 0;
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
@@ -82,11 +93,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restri
 // Exclusive scan of one 4096-element tile plus the tile's prefix (from the
 // scanned partials). The last tile also writes the grand total.
 __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
-                                                      size_t n, const uint32_t *__restrict__ prefixes,
-                                                      uint32_t *__restrict__ d_total) {
+                                                      size_t n_cap, const uint32_t *__restrict__ prefixes,
+                                                      uint32_t *__restrict__ d_total,
+                                                      const uint32_t *__restrict__ d_n, uint32_t mult) {
   __shared__ uint32_t s_data[kTileN];
   __shared__ uint32_t s_wave[kWaves];
+  const size_t n = level_n(n_cap, d_n, mult);
   const size_t base = (size_t)blockIdx.x * kTileN;
+  if (n == 0) {
+    if (d_total && blockIdx.x == 0 && threadIdx.x == 0) *d_total = 0;
+    return;
+  }
+  if (base >= n) return;
+  const bool last = base + kTileN >= n;
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
     int li = k * kBlock + threadIdx.x;
@@ -117,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict
     size_t i = base + li;
     if (i < n) out[i] = s_data[li];
   }
-  if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_total = prefix + total;
+  if (d_total && last && threadIdx.x == 0) *d_total = prefix + total;
 }
 
 // ------------------------------- radix ------------------------------------
@@ -327,16 +346,22 @@ DownFn downsweep_for(int bits) {
   }
 }
 
+// per level: the partials (256-byte aligned) and a 256-byte slot for the
+// level's device count
 size_t scan_levels_bytes(size_t n) {
   size_t bytes = 0;
   while (n > (size_t)kTileN) {
     n = div_up(n, kTileN);
-    bytes += ((n * sizeof(uint32_t) + 255) / 256) * 256;
+    bytes += ((n * sizeof(uint32_t) + 255) / 256) * 256 + 256;
   }
   return bytes + 256;
 }
 
-void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, uint8_t *ws) {
+// Reduce-then-scan, recursive over the per-tile partials. With d_n the
+// element count is read on the device (capacity n, see level_n) and every
+// launch is sized by the capacity, so no host synchronisation is needed.
+void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, uint8_t *ws,
+              const uint32_t *d_n = nullptr, uint32_t mult = 1) {
   if (n == 0) {
     // a fill kernel rather than a memset node, so that the scan can be captured in a graph
     if (d_total) k_fill_u32<<<1, 256, 0, s>>>(d_total, 1, 0u);
@@ -344,17 +369,19 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
     return;
   }
   if (n <= (size_t)kTileN) {
-    k_scan_down<<<1, kBlock, 0, s>>>(in, out, n, nullptr, d_total);
+    k_scan_down<<<1, kBlock, 0, s>>>(in, out, n, nullptr, d_total, d_n, mult);
     DL_HIP(hipGetLastError());
     return;
   }
   const uint32_t nb = div_up(n, kTileN);
   uint32_t *partials = reinterpret_cast<uint32_t *>(ws);
-  uint8_t *next = ws + ((nb * sizeof(uint32_t) + 255) / 256) * 256;
-  k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, partials);
+  uint8_t *p = ws + ((nb * sizeof(uint32_t) + 255) / 256) * 256;
+  uint32_t *d_nb = d_n ? reinterpret_cast<uint32_t *>(p) : nullptr;
+  uint8_t *next = p + 256;
+  k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, partials, d_n, mult, d_nb);
   DL_HIP(hipGetLastError());
-  scan_rec(s, partials, partials, nb, nullptr, next);
-  k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total);
+  scan_rec(s, partials, partials, nb, nullptr, next, d_nb, 1);
+  k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total, d_n, mult);
   DL_HIP(hipGetLastError());
 }
 
@@ -377,6 +404,11 @@ size_t scan_workspace_bytes(size_t n) { return scan_levels_bytes(n); }
 void exclusive_scan_u32(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n, uint32_t *d_total,
                         void *ws) {
   scan_rec(s, d_in, d_out, n, d_total, static_cast<uint8_t *>(ws));
+}
+
+void exclusive_scan_u32_dev(hipStream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n_cap, const uint32_t *d_n,
+                            uint32_t mult, uint32_t *d_total, void *ws) {
+  scan_rec(s, d_in, d_out, n_cap, d_total, static_cast<uint8_t *>(ws), d_n, mult);
 }
 
 void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_bins, uint32_t *d_off) {
