@@ -11,7 +11,9 @@ Prints one JSON line (metric/value in ms per frame, lower is better) with:
                 duration from hipEvents on the renderer's stream
   cpu_baseline  the CPU oracle on the same scene: every stage on all 6M
                 splats, rasterize on every ROW_STEP-th tile row scaled up
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  batched       512 views (8 scenes x 64 orbit cameras) as one batched call
+                against the same views rendered one by one
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 from __future__ import annotations
 
@@ -76,6 +78,61 @@ def cpu_baseline(data, cam):
     }
 
 
+def batched_views(dev, steps: int, note):
+    """8 resident scenes (50k .. 4M Gaussians, render distributions) x 64
+    orbit cameras at 640x360 / 1280x720 / 1920x1080: 512 views. Plain
+    one-by-one renders of every view against one batched call (graph
+    replay); a sample of views is checked bit-exact between the two."""
+    from droplet_amd import synthetic as S
+    from droplet_amd.batch import BatchRenderer, View
+    from droplet_amd.renderer import Renderer, Scene
+
+    datas = [S.gaussian_cloud(n, seed=100 + i) for i, n in enumerate(S.BATCH_SCENE_SIZES)]
+    scenes = [Scene(dev, d) for d in datas]
+    del datas
+    cams = S.orbit_cameras(64)
+    views = [View(si, cam) for si in range(len(scenes)) for cam in cams]
+    r = Renderer(dev)
+    for v in views[::37]:
+        r.render_scene(scenes[v.scene], v.camera, readback=False)
+    t0 = time.perf_counter()
+    for v in views:
+        r.render_scene(scenes[v.scene], v.camera, readback=False)
+    plain_s = time.perf_counter() - t0
+    note(f"batched workload: one-by-one {plain_s * 1e3:.1f} ms for {len(views)} views")
+    b = BatchRenderer(dev)
+    b.render(scenes, views)   # plans, renders directly, captures the graph
+    b.render(scenes, views)   # first replay
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.render(scenes, views)
+    batch_s = (time.perf_counter() - t0) / steps
+    st = b.stats()
+    checked = 0
+    for k in range(0, len(views), 29):
+        v = views[k]
+        ref = r.render_scene(scenes[v.scene], v.camera)
+        if not np.array_equal(b.image(k), ref):
+            raise RuntimeError(f"batched view {k} differs from the single-view render")
+        checked += 1
+    note(f"batched workload: {batch_s * 1e3:.1f} ms per call ({st['num_groups']} groups, "
+         f"{st['graph_replays']} graph replays), {checked} views verified bit-exact")
+    out = {
+        "workload": "8 scenes (50k-4M Gaussians) x 64 orbit cameras, 640x360/1280x720/1920x1080",
+        "views": len(views), "scene_splats": [s.count for s in scenes],
+        "frames_per_s": round(len(views) / batch_s, 1),
+        "one_by_one_frames_per_s": round(len(views) / plain_s, 1),
+        "speedup": round(plain_s / batch_s, 3),
+        "ms_per_call": round(batch_s * 1e3, 3), "device_ms_per_call": round(st["device_ms"], 3),
+        "groups": st["num_groups"], "pairs": st["num_pairs"], "views_verified_bitexact": checked,
+    }
+    b.destroy()
+    r.destroy()
+    for s in scenes:
+        s.destroy()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +140,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--splats", type=int, default=6_000_000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic) so that a "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE run can calibrate the counters")
@@ -180,12 +239,18 @@ def main() -> int:
         "readback_ms": round(readback_ms, 3),
         "image_nonblack_px": int(((img.reshape(-1, 4)[:, :3] > 5).any(1)).sum()),
     }
+    if not args.no_batch:
+        r.destroy()
+        scene.destroy()
+        line["batched"] = batched_views(dev, args.batch_steps, note)
+        r = scene = None
     if not args.no_cpu:
         note("cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(data, cam)
     print(json.dumps(line))
-    r.destroy()
-    scene.destroy()
+    if r is not None:
+        r.destroy()
+        scene.destroy()
     dev.destroy()
     return 0
 

@@ -51,14 +51,24 @@ class DlDeviceOpts(C.Structure):
 
 
 class DlRendererOpts(C.Structure):
-    _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32), ("binning", C.c_uint32)]
+    _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32), ("binning", C.c_uint32),
+                ("first_pass", C.c_uint32), ("pass_growth", C.c_uint32), ("full_lists", C.c_uint32)]
 
 
 class DlRenderStats(C.Structure):
     _fields_ = [("num_visible", C.c_uint32), ("num_chunks", C.c_uint32), ("num_tiles", C.c_uint32),
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("pair_key_bits", C.c_uint32),
                 ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
-                ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64)]
+                ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64)]
+
+
+class DlView(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("camera", DlCamera), ("background", DlBackground)]
+
+
+class DlBatchStats(C.Structure):
+    _fields_ = [("num_views", C.c_uint32), ("num_groups", C.c_uint32), ("graph_replays", C.c_uint32),
+                ("captures", C.c_uint32), ("num_pairs", C.c_uint64), ("device_ms", C.c_float)]
 
 
 class DlVoxelOpts(C.Structure):
@@ -84,6 +94,12 @@ SIGNATURES = [
     ("dl_renderer_stats", C.c_int, [C.c_void_p, C.POINTER(DlRenderStats)]),
     ("dl_renderer_set_taps", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_tap", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("dl_batch_create", C.c_int, [C.c_void_p, C.POINTER(DlRendererOpts), C.POINTER(C.c_void_p)]),
+    ("dl_batch_destroy", None, [C.c_void_p]),
+    ("dl_batch_set_graph", C.c_int, [C.c_void_p, C.c_int]),
+    ("dl_batch_render", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DlView), C.c_uint32]),
+    ("dl_batch_read_image", C.c_int, [C.c_void_p, C.c_uint32, u8p]),
+    ("dl_batch_get_stats", C.c_int, [C.c_void_p, C.POINTER(DlBatchStats)]),
     ("dl_sort_pairs_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint32, C.c_uint32]),
     ("dl_voxelize", C.c_int, [C.c_void_p, C.POINTER(DlSplats), C.POINTER(DlVoxelOpts), C.POINTER(C.c_void_p)]),
     ("dl_voxel_result_count", C.c_uint64, [C.c_void_p]),

@@ -30,7 +30,14 @@ struct LiveMap {
 // per-row counts start at c1_base.
 struct L1View {
   uint32_t item0, n, blk0, row0, rows, c1_base;
+  uint32_t tiles_x, max_cov;  // the view's tile grid is tiles_x x rows
 };
+
+// Packed rects of every view of a batch: rects[item0 + j] for the j-th
+// depth-ordered item (ids[item0 + j], an index of rec) of each view; blocks
+// of 256 items as the level-1 blocks.
+void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
+                 uint32_t blocks, uint4 *rects);
 
 struct BinWorkspace {
   // level 1

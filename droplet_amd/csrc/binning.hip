@@ -114,6 +114,16 @@ __device__ inline L1View l1_view(const L1View &one, const L1View *__restrict__ t
   return tab[lo];
 }
 
+__global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
+                                                     const L1View *__restrict__ tab, uint32_t ntab,
+                                                     uint4 *__restrict__ rects) {
+  const L1View v = l1_view(L1View{}, tab, ntab, blockIdx.x);
+  const uint32_t jj = (blockIdx.x - v.blk0) * kB1 + threadIdx.x;
+  if (jj >= v.n) return;
+  const uint32_t id = ids[v.item0 + jj];
+  rects[v.item0 + jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], v.tiles_x, v.rows, v.max_cov));
+}
+
 // ---------------------------- level 1 -------------------------------------
 // Level-1 block b of view v covers items [item0 + 256 k, ...) with k = b -
 // blk0, and rows [row0, row0 + rows); its count of row r sits at c1_base +
@@ -537,6 +547,13 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   DL_HIP(hipGetLastError());
 }
 
+void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
+                 uint32_t blocks, uint4 *rects) {
+  if (!blocks) return;
+  k_batch_rects<<<blocks, kB1, 0, s>>>(rec, ids, tab, ntab, rects);
+  DL_HIP(hipGetLastError());
+}
+
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               const LiveMap &live, BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small) {
   ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
@@ -546,7 +563,7 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
   }
   SpanBinArgs a;
   a.rects = ws.rects.as<uint4>();
-  a.one = L1View{0, cn, 0, 0, g.tiles_y, 0};
+  a.one = L1View{0, cn, 0, 0, g.tiles_y, 0, g.tiles_x, g.max_cov};
   a.tab = nullptr;
   a.ntab = 0;
   a.l1_blocks = div_up(cn, kB1);

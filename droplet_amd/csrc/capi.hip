@@ -9,6 +9,7 @@
 
 #include "dl_common.h"
 #include "dl_internal.h"
+#include "batch.h"
 #include "render.h"
 
 struct dl_renderer {
@@ -16,6 +17,9 @@ struct dl_renderer {
 };
 struct dl_scene {
   dl::Scene s;
+};
+struct dl_batch {
+  dl::Batch b;
 };
 
 namespace dl {
@@ -270,6 +274,60 @@ int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *
     if (!dst) return;
     if (bytes < t.size()) fail(DL_ERR_CAPACITY, "tap buffer too small");
     if (!t.empty()) std::memcpy(dst, t.data(), t.size());
+  });
+}
+
+int dl_batch_create(dl_context *ctx, const dl_renderer_opts *opts, dl_batch **out) {
+  return guard([&] {
+    if (!ctx || !out) fail(DL_ERR_INVALID, "null argument");
+    std::unique_ptr<dl_batch> b(new dl_batch());
+    b->b.ctx = &ctx->c;
+    if (opts) b->b.opts = *opts;
+    b->b.max_cov = (opts && opts->max_coverage_per_splat) ? opts->max_coverage_per_splat : kMaxCoverageDefault;
+    if (opts && opts->chunk_cap) b->b.item_cap = opts->chunk_cap;
+    *out = b.release();
+  });
+}
+
+void dl_batch_destroy(dl_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->b.ctx->device);
+  (void)hipStreamSynchronize(b->b.ctx->stream);
+  delete b;
+}
+
+int dl_batch_set_graph(dl_batch *b, int enabled) {
+  return guard([&] {
+    if (!b) fail(DL_ERR_INVALID, "null batch");
+    b->b.use_graph = enabled != 0;
+  });
+}
+
+int dl_batch_render(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, const dl_view *views,
+                    uint32_t num_views) {
+  return guard([&] {
+    if (!b || !scenes || !views) fail(DL_ERR_INVALID, "null argument");
+    std::vector<SceneView> sv(num_scenes);
+    for (uint32_t k = 0; k < num_scenes; k++) {
+      if (!scenes[k]) fail(DL_ERR_INVALID, "null scene");
+      if (scenes[k]->s.ctx != b->b.ctx) fail(DL_ERR_INVALID, "scene and batch belong to different contexts");
+      sv[k] = scenes[k]->s.view();
+    }
+    b->b.render(sv, views, num_views);
+  });
+}
+
+int dl_batch_read_image(dl_batch *b, uint32_t view, uint8_t *out_rgba) {
+  return guard([&] {
+    if (!b || !out_rgba) fail(DL_ERR_INVALID, "null argument");
+    b->b.read_image(view, out_rgba);
+  });
+}
+
+int dl_batch_get_stats(dl_batch *b, dl_batch_stats *out) {
+  return guard([&] {
+    if (!b || !out) fail(DL_ERR_INVALID, "null argument");
+    *out = b->b.stats;
   });
 }
 

@@ -29,10 +29,13 @@ struct ProjUniforms {
   uint32_t tiles_x, tiles_y, max_cov, proj;
 };
 
+ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov);
+
 struct RastParams {
   uint32_t tiles_x, width, height;
   uint32_t has_pairs, resume, finalize, write_accum;
   float bg_r, bg_g, bg_b, bg_a;
+  uint8_t *closed;  // per (virtual) tile: 1 once every pixel has T < 1e-4 (null: not tracked)
 };
 
 struct Scene {
@@ -56,8 +59,8 @@ struct Renderer {
   DeviceBuffer pkeys, pvals, ptkeys, ptvals, psort_ws;
   // span binning
   BinWorkspace bin_ws;
-  // image
-  DeviceBuffer accum, rgba, small;
+  // image; closed-tile flags and open-tile map of front-to-back passes
+  DeviceBuffer accum, rgba, small, closed, open_pre;
   uint32_t width = 0, height = 0;
   dl_render_stats stats{};
   bool timing = false;

@@ -155,8 +155,24 @@ __global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec
                                                   unsigned long long *__restrict__ d_iters) {
   __shared__ float4 s_rec[64 * 3];
   __shared__ uint32_t s_mask[64];
-  rasterize_tile<COUNT>(blockIdx.x, rec, off, pvals, accum, out_rgba, p, d_iters, s_rec, s_mask);
+  const uint32_t tile = blockIdx.x;
+  const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
+  rasterize_tile<COUNT>(tile % p.tiles_x, tile / p.tiles_x, tile, start, end, rec, pvals, accum, out_rgba, p, d_iters,
+                        s_rec, s_mask);
 }
+
+__global__ void k_zero_u64(unsigned long long *__restrict__ p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ void k_add_entries(const uint32_t *__restrict__ n, unsigned long long *__restrict__ acc) {
+  if (threadIdx.x == 0) *acc += *n;
+}
+
+__global__ void k_add_const(uint32_t n, unsigned long long *__restrict__ acc) {
+  if (threadIdx.x == 0) *acc += n;
+}
+
 
 void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float4 *rec, const uint32_t *off,
                       const uint32_t *pvals, float4 *accum, uint32_t *out_rgba, const RastParams &p,
@@ -215,6 +231,25 @@ static std::vector<T> to_host(const T *dev, size_t count, hipStream_t s) {
   return h;
 }
 
+// pipeline.ts:475-506: the project uniforms as the f32 values the
+// reference writes (focal from the f64 fovY)
+ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov) {
+  ProjUniforms u{};
+  std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
+  u.width = (float)cam.width;
+  u.height = (float)cam.height;
+  u.focal = (float)((double)cam.height / (2.0 * std::tan(cam.fov_y / 2.0)));
+  u.cx = (float)((double)cam.width / 2.0);
+  u.cy = (float)((double)cam.height / 2.0);
+  u.near_ = (float)cam.near_plane;
+  u.far_ = (float)cam.far_plane;
+  u.tiles_x = div_up(cam.width, kTile);
+  u.tiles_y = div_up(cam.height, kTile);
+  u.max_cov = max_cov;
+  u.proj = cam.projection;
+  return u;
+}
+
 void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_background &bg) {
   if (cam.width == 0 || cam.height == 0) fail(DL_ERR_INVALID, "camera width and height must be > 0");
   if (sc.bands > 3) fail(DL_ERR_INVALID, "shBands must be 0..3");
@@ -240,14 +275,17 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rgba.reserve(npx * 4);
   small.reserve(256);
   // device scalars: [0] visible, [1] chunk pairs, [4..6] span-binning
-  // scalars, [8..9] u64 total pairs, [10..13] u64 loop iterations and
-  // consumed list entries (counting frames)
+  // scalars, [8..9] u64 total pairs, [10..15] u64 loop iterations and
+  // consumed list entries (counting frames), list entries built
   uint32_t *d_nvis = small.as<uint32_t>();
   uint32_t *d_total = d_nvis + 1;
   uint32_t *d_bin = d_nvis + 4;
   unsigned long long *d_pairs_all = reinterpret_cast<unsigned long long *>(d_nvis + 8);
   unsigned long long *d_iters = reinterpret_cast<unsigned long long *>(d_nvis + 10);
+  unsigned long long *d_entries = d_iters + 2;
   if (counting) DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
+  k_zero_u64<<<1, 64, 0, s>>>(d_entries);
+  DL_HIP(hipGetLastError());
 
   RastParams rp{};
   rp.tiles_x = tiles_x;
@@ -258,19 +296,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rp.bg_b = bg.b;
   rp.bg_a = bg.a;
 
-  ProjUniforms u{};
-  std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
-  u.width = (float)W;
-  u.height = (float)H;
-  u.focal = (float)((double)H / (2.0 * std::tan(cam.fov_y / 2.0)));
-  u.cx = (float)((double)W / 2.0);
-  u.cy = (float)((double)H / 2.0);
-  u.near_ = (float)cam.near_plane;
-  u.far_ = (float)cam.far_plane;
-  u.tiles_x = tiles_x;
-  u.tiles_y = tiles_y;
-  u.max_cov = max_cov;
-  u.proj = cam.projection;
+  const ProjUniforms u = make_uniforms(cam, max_cov);
 
   // ---- cull + project (splat order), depth sort ----
   uint32_t nvis = 0;
@@ -340,22 +366,45 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     launch_rasterize(counting, num_tiles, s, nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp,
                      d_iters);
   } else {
-    const uint32_t chunk_cap = (opts.chunk_cap && opts.chunk_cap < nvis) ? opts.chunk_cap : nvis;
-    const uint32_t chunks = div_up(nvis, chunk_cap);
     const bool span = opts.binning == 0 && span_binning_supported(tiles_x, tiles_y);
+    // Front-to-back passes that stop filling closed tiles (span binning, no taps)
+    const bool early_stop = span && !keep_taps && !opts.full_lists;
+    std::vector<uint32_t> cuts;  // pass starts, then nvis
+    if (opts.chunk_cap && opts.chunk_cap < nvis) {
+      for (uint64_t c0 = 0; c0 < nvis; c0 += opts.chunk_cap) cuts.push_back((uint32_t)c0);
+    } else if (early_stop) {
+      uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(65536, nvis / 16);
+      const uint64_t growth = opts.pass_growth ? opts.pass_growth : 4;
+      for (uint64_t c0 = 0; c0 < nvis; c0 += len, len *= growth) cuts.push_back((uint32_t)c0);
+    } else {
+      cuts.push_back(0);
+    }
+    cuts.push_back(nvis);
+    const uint32_t chunks = (uint32_t)cuts.size() - 1;
+    // exact per-chunk pair totals (host sync) only where a pass needs them
+    const bool exact_counts = !span || (chunks > 1 && !early_stop);
+    const bool track_closed = early_stop && chunks > 1;
     ranges.reserve(((size_t)num_tiles + 1) * 4);
     if (chunks > 1 || need_accum) accum.reserve(npx * 16);
+    if (track_closed) {
+      closed.reserve(num_tiles);
+      open_pre.reserve((size_t)tiles_y * (tiles_x + 1) * 2);
+      k_fill_u8<<<div_up(num_tiles, 256), 256, 0, s>>>(closed.as<uint8_t>(), num_tiles, 0);
+      DL_HIP(hipGetLastError());
+    }
     const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(num_tiles - 1));
     stats.pair_key_bits = key_bits;
-    for (uint32_t c0 = 0; c0 < nvis; c0 += chunk_cap) {
-      const uint32_t cn = std::min(chunk_cap, nvis - c0);
-      const bool last = c0 + cn == nvis;
+    if (!exact_counts) stats.num_pairs = pairs_all;
+    for (uint32_t c = 0; c < chunks; c++) {
+      const uint32_t c0 = cuts[c];
+      const uint32_t cn = cuts[c + 1] - c0;
+      const bool last = c + 1 == chunks;
       const uint32_t *cids = sorted_ids + c0;
       stats.num_chunks++;
       // ---- pair count of this chunk (and scanned offsets for emit) ----
       mark(DL_STAGE_SCAN);
-      uint64_t npairs64 = pairs_all;
-      if (chunks > 1 || !span) {
+      uint64_t npairs64 = pairs_all;  // bound of every chunk
+      if (exact_counts) {
         chunk_counts.reserve((size_t)cn * 4);
         offsets.reserve((size_t)cn * 4);
         scan_ws.reserve(scan_workspace_bytes(cn));
@@ -366,13 +415,14 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         DL_HIP(hipMemcpyAsync(&t32, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         DL_HIP(hipStreamSynchronize(s));
         npairs64 = t32;
+        stats.num_pairs += t32;
       }
       if (npairs64 > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (tile, splat) pairs in one chunk");
       const uint32_t npairs = (uint32_t)npairs64;
-      stats.num_pairs += npairs;
       rp.resume = accum_valid;
       rp.finalize = last;
       rp.write_accum = !last || need_accum;
+      rp.closed = track_closed ? closed.as<uint8_t>() : nullptr;
       if (npairs == 0) {
         // pipeline.ts:306: nothing to rasterize in this chunk
         if (keep_taps) {
@@ -392,8 +442,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       if (span) {
         // stable two-level bucketing straight into per-tile lists (binning.hip)
         mark(DL_STAGE_EMIT);
-        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, LiveMap{}, bin_ws,
+        const LiveMap live = (track_closed && c > 0) ? LiveMap{open_pre.as<uint16_t>(), tiles_x + 1} : LiveMap{};
+        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, live, bin_ws,
                  ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin);
+        k_add_entries<<<1, 64, 0, s>>>(d_bin + 2, d_entries);
+        DL_HIP(hipGetLastError());
       } else {
         pkeys.reserve((size_t)npairs * 4);
         ptkeys.reserve((size_t)npairs * 4);
@@ -410,6 +463,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         svals = in_tmp ? ptvals.as<uint32_t>() : pvals.as<uint32_t>();
         mark(DL_STAGE_RANGES);
         bin_ranges(s, skeys, npairs, num_tiles, ranges.as<uint32_t>());
+        k_add_const<<<1, 64, 0, s>>>(npairs, d_entries);
+        DL_HIP(hipGetLastError());
       }
       if (keep_taps) {
         // reference view: sorted tile ids and chunk-local depth ranks
@@ -428,15 +483,22 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
                        rgba.as<uint32_t>(), rp, d_iters);
       accum_valid = true;
+      if (track_closed && !last) {
+        k_open_prefix<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tiles_x, open_pre.as<uint16_t>());
+        DL_HIP(hipGetLastError());
+      }
     }
   }
   mark(-1);
-  if (counting) {
-    unsigned long long cnt[2] = {0, 0};
+  {
+    unsigned long long cnt[3] = {0, 0, 0};
     DL_HIP(hipMemcpyAsync(cnt, d_iters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
-    stats.pixel_iters = cnt[0];
-    stats.pairs_consumed = cnt[1];
+    if (counting) {
+      stats.pixel_iters = cnt[0];
+      stats.pairs_consumed = cnt[1];
+    }
+    stats.list_entries = cnt[2];
   }
   DL_HIP(hipStreamSynchronize(s));
   if (keep_taps) {

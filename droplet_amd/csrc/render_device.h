@@ -221,7 +221,7 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
 
 // f64 cull (camera.ts:56-99) and sortable depth key of splat i, then
 // project() when it is visible; the outputs go to slot o of keys / vals /
-// rec / counts, with vals[o] = val.
+// rec / counts (counts may be null), with vals[o] = val.
 template <int BANDS>
 __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms &u, double near_plane,
                                         double far_plane, uint32_t i, size_t o, uint32_t val,
@@ -261,7 +261,7 @@ __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms 
   rec[3 * o + 0] = o0;
   rec[3 * o + 1] = o1;
   rec[3 * o + 2] = o2;
-  counts[o] = count;
+  if (counts) counts[o] = count;
 }
 
 // ---------------------------------------------------------------------------
@@ -321,15 +321,21 @@ __device__ inline bool rect_may_hit(double sx, double sy, double r, double A, do
   return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
 }
 
-// The work of one wave64 on one tile. off / accum / out_rgba are the
-// view's own (tile-indexed offsets, row-major pixels); pvals index rec.
+// The work of one wave64 on one tile: tile (tx, ty) of the view, whose
+// accum / out_rgba are the view's own row-major pixels, with list entries
+// [start, end) of pvals (which index rec); vtile
+// indexes the open/closed flags (p.closed). A tile is closed once all its
+// pixels have T < 1e-4: later passes give it no entries, and until the
+// final pass it is skipped, like a tile without entries in this pass, as
+// long as the accumulator already holds its state.
 template <bool COUNT>
-__device__ inline void rasterize_tile(uint32_t tile, const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
-                                      const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
-                                      uint32_t *__restrict__ out_rgba, const RastParams &p,
-                                      unsigned long long *__restrict__ d_iters, float4 *s_rec, uint32_t *s_mask) {
+__device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
+                                      const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
+                                      float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
+                                      const RastParams &p, unsigned long long *__restrict__ d_iters, float4 *s_rec,
+                                      uint32_t *s_mask) {
+  if (p.closed && p.resume && !p.finalize && (start == end || p.closed[vtile])) return;
   uint32_t iters = 0;
-  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
   const int lane = threadIdx.x;
   // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
   // this lane's row in strip k is 4k + lane/16
@@ -351,8 +357,7 @@ __device__ inline void rasterize_tile(uint32_t tile, const float4 *__restrict__ 
     live[k] = inb[k] && !(T[k] < kMinT);
   }
   uint32_t consumed = 0;
-  if (p.has_pairs) {
-    const uint32_t start = off[tile], end = off[tile + 1];
+  if (start < end) {
     const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
     uint32_t stop = end;
     for (uint32_t base = start; base < end; base += 64) {
@@ -420,6 +425,10 @@ __device__ inline void rasterize_tile(uint32_t tile, const float4 *__restrict__ 
     if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
     if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }
+  if (p.closed && !p.finalize) {
+    const bool open = __ballot(live[0] || live[1] || live[2] || live[3]) != 0;
+    if (lane == 0) p.closed[vtile] = open ? 0 : 1;
+  }
   if (COUNT) {
     uint32_t s = iters;
 #pragma unroll
@@ -429,6 +438,35 @@ __device__ inline void rasterize_tile(uint32_t tile, const float4 *__restrict__ 
       atomicAdd(d_iters + 1, (unsigned long long)consumed);
     }
   }
+}
+
+// Open-tile map for the next pass: per row, prefix counts of open tiles.
+// One 256-thread block per row (cols <= 256).
+__global__ __launch_bounds__(256) void k_open_prefix(const uint8_t *__restrict__ closed, uint32_t cols,
+                                                     uint16_t *__restrict__ pre) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t y = blockIdx.x, x = threadIdx.x;
+  const uint32_t open = (x < cols && !closed[(size_t)y * cols + x]) ? 1u : 0u;
+  uint32_t inc = open;
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_w[x >> 6] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < (x >> 6); w++) off += s_w[w];
+  inc += off;
+  uint16_t *r = pre + (size_t)y * (cols + 1);
+  if (x == 0) r[0] = 0;
+  if (x < cols) r[x + 1] = (uint16_t)inc;
+}
+
+__global__ void k_fill_u8(uint8_t *__restrict__ p, size_t n, uint8_t v) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 }  // namespace

@@ -33,6 +33,11 @@ class RendererOptions:
     chunk_cap: Optional[int] = None
     max_coverage_per_splat: Optional[int] = None
     binning: str = "span"   # "span" (two-level bucketing) or "sort" (emit + radix sort)
+    # span binning: front-to-back passes that stop filling tiles whose pixels
+    # have all saturated (T < 1e-4); the image is unchanged
+    first_pass: Optional[int] = None
+    pass_growth: Optional[int] = None
+    full_lists: bool = False
 
 
 class Device:
@@ -108,6 +113,9 @@ class Renderer:
         if opts.binning not in ("span", "sort"):
             raise ValueError("binning must be 'span' or 'sort'")
         o.binning = 0 if opts.binning == "span" else 1
+        o.first_pass = int(opts.first_pass or 0)
+        o.pass_growth = int(opts.pass_growth or 0)
+        o.full_lists = 1 if opts.full_lists else 0
         h = C.c_void_p()
         L.check(L.load().dl_renderer_create(device.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -166,7 +174,7 @@ class Renderer:
             "num_pairs": int(st.num_pairs),
             "stage_ms": {name: float(st.stage_ms[i]) for i, name in enumerate(L.STAGES)},
             "total_ms": float(st.total_ms), "pixel_iters": int(st.pixel_iters),
-            "pairs_consumed": int(st.pairs_consumed),
+            "pairs_consumed": int(st.pairs_consumed), "list_entries": int(st.list_entries),
         }
 
     def tap(self, name: str) -> np.ndarray:

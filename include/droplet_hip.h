@@ -45,6 +45,7 @@ typedef struct dl_context dl_context;
 typedef struct dl_renderer dl_renderer;
 typedef struct dl_scene dl_scene;
 typedef struct dl_voxel_result dl_voxel_result;
+typedef struct dl_batch dl_batch;
 
 /* SplatData (src/types.ts:23-32): struct of float32 arrays, N = count.
  * positions 3N, rotations 4N [w,x,y,z], log_scales 3N, opacity_logits N,
@@ -86,6 +87,17 @@ typedef struct {
      * 256x1024 tiles; wider grids fall back to 1); 1: emit + stable radix
      * sort of (tile, splat) pairs. Both give identical lists and images. */
     uint32_t binning;
+    /* Span binning runs in front-to-back passes over the depth-sorted
+     * splats. After each pass, tiles whose pixels all have T < 1e-4 are
+     * closed. Later passes bin nothing into closed tiles: the reference's
+     * per-pixel loop breaks before it reads those entries, so the image is
+     * unchanged. first_pass is the number of splats in the first pass
+     * (0: automatic), and each later pass is pass_growth times larger
+     * (0: automatic). full_lists = 1 bins every pair in one pass, as the
+     * reference does. Debug taps always use full lists. */
+    uint32_t first_pass;
+    uint32_t pass_growth;
+    uint32_t full_lists;
 } dl_renderer_opts;
 
 /* ---- device (replaces createDevice, src/gpu/device.ts:27-41) ---- */
@@ -134,6 +146,7 @@ typedef struct {
     float total_ms;
     uint64_t pixel_iters;        /* rasterize loop iterations over all pixels (counting enabled) */
     uint64_t pairs_consumed;     /* tile-list entries read before each tile saturated (counting enabled) */
+    uint64_t list_entries;       /* tile-list entries built (<= num_pairs; fewer when closed tiles are skipped) */
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
 /* Counts rasterize loop iterations (the work unit of the rasterize roofline)
@@ -154,6 +167,41 @@ int dl_renderer_stats(dl_renderer *r, dl_render_stats *out);
 int dl_renderer_set_taps(dl_renderer *r, int enabled);
 /* bytes: capacity of dst; returns DL_ERR_CAPACITY if too small */
 int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *written);
+
+/* ---- batched views (no reference counterpart: it stands beside
+ * Renderer.render, src/render/pipeline.ts:181-185, whose per-view image it
+ * reproduces bit for bit). Many views over several resident scenes go
+ * through one pass of every stage. Views are grouped up to 2^28 scene
+ * splats per group. The first call with a new layout (scene sizes and bands,
+ * view scenes and sizes) runs directly and captures the sequence of launches
+ * in a hipGraph; later calls with the same layout replace the cameras and
+ * backgrounds and replay the graph. The list capacity comes from the
+ * measured pair counts, and a later call that needs more re-plans by
+ * itself. Views are limited to 256 x 1024 tiles (4096 x 16384 pixels). */
+typedef struct {
+    uint32_t scene;              /* index into the scenes array */
+    dl_camera camera;
+    dl_background background;
+} dl_view;
+typedef struct {
+    uint32_t num_views;
+    uint32_t num_groups;
+    uint32_t graph_replays;      /* calls served by replaying the captured graph */
+    uint32_t captures;           /* graphs captured so far */
+    uint64_t num_pairs;          /* (tile, splat) pairs of the last call, all views */
+    float device_ms;             /* device time of the last call (hipEvents around its work) */
+} dl_batch_stats;
+/* opts: max_coverage_per_splat as for the renderer; chunk_cap, when set,
+ * is the number of scene splats per group (default 2^28). */
+int dl_batch_create(dl_context *ctx, const dl_renderer_opts *opts, dl_batch **out);
+void dl_batch_destroy(dl_batch *b);
+/* enabled = 0: run every call directly (same kernels, no graph) */
+int dl_batch_set_graph(dl_batch *b, int enabled);
+int dl_batch_render(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, const dl_view *views,
+                    uint32_t num_views);
+/* RGBA8 of view `view` of the last call (width*height*4 bytes) */
+int dl_batch_read_image(dl_batch *b, uint32_t view, uint8_t *out_rgba);
+int dl_batch_get_stats(dl_batch *b, dl_batch_stats *out);
 
 /* ---- stable u32 key/value sort (replaces GpuRadixSorter.sort,
  * src/gpu/radix-sort.ts:142-215, which is unstable; this one is stable).

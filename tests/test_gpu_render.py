@@ -242,3 +242,51 @@ def test_exp_skip_threshold_is_conservative(renderer):
     assert (alpha2 >= f32(0.00392156862745098)).mean() > 0.95
     # tau is +inf only where alpha == 0
     assert (a[~finite] == 0).all()
+
+
+@pytest.mark.parametrize("first_pass,growth", [(None, None), (1000, 2), (5000, 1), (50_000, 8)])
+def test_front_to_back_passes_match_oracle(device, first_pass, growth):
+    """Span binning in front-to-back passes closes tiles whose pixels have all
+    saturated and bins nothing into them afterwards. The image and the
+    reference's per-pixel loop count are unchanged."""
+    r = Renderer(device, RendererOptions(first_pass=first_pass, pass_growth=growth))
+    try:
+        for data, cam in [S.plain_render_workload(seed=21), (S.dense_cluster(30_000), S.default_camera(256, 256)),
+                          (S.gaussian_cloud(60_000, seed=22, sh_bands=2, pos_std=0.75, pos_clip=3.0),
+                           S.default_camera(640, 360))]:
+            r.set_counting(True)
+            img = r.render(data, cam)
+            st = r.stats()
+            r.set_counting(False)
+            ref = oracle.render(data, cam, taps=False)
+            print(f"case: n={data.count} {cam.width}x{cam.height} passes={st['num_chunks']} "
+                  f"entries={st['list_entries']} of {st['num_pairs']}", flush=True)
+            np.testing.assert_array_equal(img, ref["rgba"])
+            assert st["pixel_iters"] == ref["pixel_iters"]
+            assert st["num_pairs"] == ref["num_pairs"]
+            assert st["list_entries"] <= st["num_pairs"]
+            if first_pass is not None and st["num_visible"] > first_pass:
+                assert st["num_chunks"] > 1
+    finally:
+        r.destroy()
+
+
+def test_front_to_back_passes_skip_closed_tiles(device):
+    """On a dense scene many (tile, splat) pairs lie behind saturated pixels:
+    the passes build fewer list entries than the full lists hold (tiles at
+    the cluster's rim never saturate and keep their whole lists)."""
+    data = S.dense_cluster(200_000)
+    cam = S.default_camera(512, 512)
+    r = Renderer(device, RendererOptions(first_pass=20_000, pass_growth=4))
+    full = Renderer(device, RendererOptions(full_lists=True))
+    try:
+        a = r.render(data, cam)
+        st = r.stats()
+        b = full.render(data, cam)
+        fst = full.stats()
+        np.testing.assert_array_equal(a, b)
+        assert fst["list_entries"] == fst["num_pairs"] == st["num_pairs"]
+        assert st["list_entries"] < 0.9 * st["num_pairs"]
+    finally:
+        r.destroy()
+        full.destroy()
