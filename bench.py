@@ -13,6 +13,8 @@ Prints one JSON line (metric/value in ms per frame, lower is better) with:
                 splats, rasterize on every ROW_STEP-th tile row scaled up
   batched       512 views (8 scenes x 64 orbit cameras) as one batched call
                 against the same views rendered one by one
+  voxelize      the 1M-Gaussian voxelizer workload (device time and wall
+                time with the host transfers)
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 from __future__ import annotations
@@ -78,7 +80,7 @@ def cpu_baseline(data, cam):
     }
 
 
-def batched_views(dev, steps: int, note):
+def batched_views(dev, steps: int, note, tune: bool = True):
     """8 resident scenes (50k .. 4M Gaussians, render distributions) x 64
     orbit cameras at 640x360 / 1280x720 / 1920x1080: 512 views. Plain
     one-by-one renders of every view against one batched call (graph
@@ -101,6 +103,11 @@ def batched_views(dev, steps: int, note):
     plain_s = time.perf_counter() - t0
     note(f"batched workload: one-by-one {plain_s * 1e3:.1f} ms for {len(views)} views")
     b = BatchRenderer(dev)
+    tuned = None
+    if tune:
+        tuned = b.autotune(scenes, views)
+        note(f"batched autotune: {tuned['chosen']} {tuned['chosen_ms']:.1f} ms per call "
+             f"(default {tuned['default_ms']:.1f} ms)")
     b.render(scenes, views)   # plans, renders directly, captures the graph
     b.render(scenes, views)   # first replay
     t0 = time.perf_counter()
@@ -125,12 +132,36 @@ def batched_views(dev, steps: int, note):
         "speedup": round(plain_s / batch_s, 3),
         "ms_per_call": round(batch_s * 1e3, 3), "device_ms_per_call": round(st["device_ms"], 3),
         "groups": st["num_groups"], "pairs": st["num_pairs"], "views_verified_bitexact": checked,
+        "autotune": tuned,
     }
     b.destroy()
     r.destroy()
     for s in scenes:
         s.destroy()
     return out
+
+
+def voxelize_leg(dev, note):
+    """The voxelizer workload: 1M Gaussians uniform in a 10 x 3 x 10 m box,
+    voxelResolution 0.02, opacityCutoff 0.1 (host arrays in, block list out,
+    as voxelize() takes and returns them)."""
+    from droplet_amd import synthetic as S
+    from droplet_amd.voxel import VoxelizeOptions, voxelize
+
+    data, o = S.voxel_workload()
+    opts = VoxelizeOptions(voxel_resolution=o["voxel_resolution"], opacity_cutoff=o["opacity_cutoff"])
+    voxelize(dev, data, opts)  # warm
+    walls, devs = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = voxelize(dev, data, opts)
+        walls.append((time.perf_counter() - t0) * 1e3)
+        devs.append(res.device_ms)
+    note(f"voxelize: {min(devs):.3f} ms device, {min(walls):.3f} ms wall, {res.buffer.count} blocks")
+    return {"workload": "1M Gaussians in a 10x3x10 m box, voxelResolution 0.02, opacityCutoff 0.1",
+            "device_ms": round(min(devs), 3), "wall_ms_incl_pcie": round(min(walls), 3),
+            "blocks": int(res.buffer.count), "solid_voxels": int(res.buffer.popcount()),
+            "gaussians_per_s": round(data.count / (min(devs) * 1e-3), 1)}
 
 
 def main() -> int:
@@ -141,6 +172,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--splats", type=int, default=6_000_000)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--no-tune", action="store_true", help="time the default options (no autotune)")
+    ap.add_argument("--no-voxel", action="store_true", help="skip the voxelizer leg")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic) so that a "
@@ -184,6 +217,11 @@ def main() -> int:
     r.set_counting(False)
     note(f"instrumented frame: {iters} pixel iterations, {consumed} tile-list entries read")
 
+    tune = None
+    if not args.no_tune:
+        tune = r.autotune(scene, cam)
+        note(f"autotune: {tune['chosen']} {tune['chosen_ms']:.3f} ms (default {tune['default_ms']:.3f} ms, "
+             f"{tune['candidates']} candidates)")
     for _ in range(args.warmup):
         r.render_scene(scene, cam, readback=False)
 
@@ -235,14 +273,17 @@ def main() -> int:
             "work": f"{iters} pixel loop iterations x {RAST_OPS_PER_ITER} ops",
             "launch_ms": rast_ms,
         },
+        "autotune": tune,
         "stage_ms": stage_ms,
         "readback_ms": round(readback_ms, 3),
         "image_nonblack_px": int(((img.reshape(-1, 4)[:, :3] > 5).any(1)).sum()),
     }
+    if not args.no_voxel:
+        line["voxelize"] = voxelize_leg(dev, note)
     if not args.no_batch:
         r.destroy()
         scene.destroy()
-        line["batched"] = batched_views(dev, args.batch_steps, note)
+        line["batched"] = batched_views(dev, args.batch_steps, note, tune=not args.no_tune)
         r = scene = None
     if not args.no_cpu:
         note("cpu baseline (oracle) ...")

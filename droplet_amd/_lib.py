@@ -52,7 +52,8 @@ class DlDeviceOpts(C.Structure):
 
 class DlRendererOpts(C.Structure):
     _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32), ("binning", C.c_uint32),
-                ("first_pass", C.c_uint32), ("pass_growth", C.c_uint32), ("full_lists", C.c_uint32)]
+                ("first_pass", C.c_uint32), ("pass_growth", C.c_uint32), ("full_lists", C.c_uint32),
+                ("sort_digit_bits", C.c_uint32)]
 
 
 class DlRenderStats(C.Structure):
@@ -60,6 +61,11 @@ class DlRenderStats(C.Structure):
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("pair_key_bits", C.c_uint32),
                 ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
                 ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64)]
+
+
+class DlTuneResult(C.Structure):
+    _fields_ = [("chosen", DlRendererOpts), ("chosen_ms", C.c_float), ("default_ms", C.c_float),
+                ("candidates", C.c_uint32)]
 
 
 class DlView(C.Structure):
@@ -94,6 +100,9 @@ SIGNATURES = [
     ("dl_renderer_stats", C.c_int, [C.c_void_p, C.POINTER(DlRenderStats)]),
     ("dl_renderer_set_taps", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_tap", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("dl_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlTuneResult)]),
+    ("dl_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DlView), C.c_uint32,
+                                    C.POINTER(DlTuneResult)]),
     ("dl_batch_create", C.c_int, [C.c_void_p, C.POINTER(DlRendererOpts), C.POINTER(C.c_void_p)]),
     ("dl_batch_destroy", None, [C.c_void_p]),
     ("dl_batch_set_graph", C.c_int, [C.c_void_p, C.c_int]),

@@ -15,7 +15,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _lib as L
-from .renderer import Device, RendererOptions, Scene
+from .renderer import Device, RendererOptions, Scene, tune_result_dict
 from .types import BackgroundRGBA, RenderCamera
 
 
@@ -33,6 +33,10 @@ class BatchRenderer:
         o = L.DlRendererOpts()
         o.max_coverage_per_splat = int(opts.max_coverage_per_splat or 0)
         o.chunk_cap = int(opts.chunk_cap or 0)   # scene splats per group
+        o.first_pass = int(opts.first_pass or 0)
+        o.pass_growth = int(opts.pass_growth or 0)
+        o.full_lists = 1 if opts.full_lists else 0
+        o.sort_digit_bits = int(opts.sort_digit_bits or 0)
         h = C.c_void_p()
         L.check(L.load().dl_batch_create(device.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -51,19 +55,34 @@ class BatchRenderer:
     def render(self, scenes: Sequence[Scene], views: Sequence[View]) -> None:
         """Renders every view; images stay on the device (image(k))."""
         sc = (C.c_void_p * len(scenes))(*[s._h for s in scenes])
-        vs = (L.DlView * len(views))()
-        for k, v in enumerate(views):
-            vs[k].scene = v.scene
-            vs[k].camera = L.camera_struct(v.camera)
-            vs[k].background = L.background_struct(v.background or BackgroundRGBA())
+        vs = self._views_struct(views)
         L.check(L.load().dl_batch_render(self.handle, sc, len(scenes), vs, len(views)))
         self._sizes = [(v.camera.width, v.camera.height) for v in views]
+
+    def autotune(self, scenes: Sequence[Scene], views: Sequence[View]) -> dict:
+        """Times candidate options on this batch and keeps the fastest
+        (dl_batch_autotune); the images are those of the last call."""
+        sc = (C.c_void_p * len(scenes))(*[s._h for s in scenes])
+        vs = self._views_struct(views)
+        res = L.DlTuneResult()
+        L.check(L.load().dl_batch_autotune(self.handle, sc, len(scenes), vs, len(views), C.byref(res)))
+        self._sizes = [(v.camera.width, v.camera.height) for v in views]
+        return tune_result_dict(res)
 
     def image(self, k: int) -> np.ndarray:
         w, h = self._sizes[k]
         out = np.empty(w * h * 4, np.uint8)
         L.check(L.load().dl_batch_read_image(self.handle, k, L.ptr(out, L.u8p)))
         return out
+
+    @staticmethod
+    def _views_struct(views: Sequence[View]):
+        vs = (L.DlView * len(views))()
+        for k, v in enumerate(views):
+            vs[k].scene = v.scene
+            vs[k].camera = L.camera_struct(v.camera)
+            vs[k].background = L.background_struct(v.background or BackgroundRGBA())
+        return vs
 
     def stats(self) -> dict:
         st = L.DlBatchStats()

@@ -178,13 +178,14 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
   uint32_t blocks_max = 0;
   size_t rows_max = 0;
   vtiles_max = 0;
-  // front-to-back passes of a view of n splats: max(65536, n / 16) first,
-  // then growth-times larger (one pass with full_lists)
-  const uint64_t growth = opts.pass_growth ? opts.pass_growth : 4;
+  // front-to-back passes of a view of n splats, on the single-view
+  // renderer's schedule (one pass with full_lists); the batch always uses
+  // them because its launches serve every view of the group at once
+  const uint64_t growth = opts.pass_growth ? opts.pass_growth : kPassGrowth;
   auto cuts_of = [&](uint32_t n) {
     std::vector<uint32_t> c{0};
     if (!opts.full_lists) {
-      uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(65536, n / 16);
+      uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(kPassMinFirst, n / kPassFirstDiv);
       for (uint64_t c0 = len; c0 < n; len *= growth, c0 += len) c.push_back((uint32_t)c0);
     }
     c.push_back(n);
@@ -324,7 +325,8 @@ void Batch::enqueue(bool eager) {
     }
     const bool in_tmp = seg_radix_sort_pairs(s, keys.as<uint32_t>(), vals.as<uint32_t>(), tkeys.as<uint32_t>(),
                                              tvals.as<uint32_t>(), d_seg.as<uint32_t>() + g.seg_off, g.nv, g.items,
-                                             32, 8, sort_ws.get());
+                                             32, opts.sort_digit_bits ? (int)std::min<uint32_t>(opts.sort_digit_bits, 8) : 8,
+                                             sort_ws.get());
     const uint32_t *sorted = in_tmp ? tvals.as<uint32_t>() : vals.as<uint32_t>();
     const uint32_t passes = (uint32_t)g.blocks.size();
     const size_t vtiles = (size_t)g.rows * g.cols;

@@ -2,10 +2,13 @@
 // scene handles, the standalone stable sort and the numerics self-check.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "dl_common.h"
 #include "dl_internal.h"
@@ -120,6 +123,42 @@ __global__ void k_debug_math(int op, const float *a, const float *b, float *out,
 }  // namespace dl
 
 using namespace dl;
+
+// ---- autotuner --------------------------------------------------------------
+namespace {
+
+// Candidate options around `base`: radix digit widths 8 / 6 / 4 and pass
+// schedules (full lists; first pass n/16 growth 4, n/8 growth 8, n/4 growth
+// 8; the default rule), n = the largest scene.
+std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint64_t n) {
+  std::vector<dl_renderer_opts> out;
+  const uint32_t digits[] = {8, 6, 4};
+  struct Sched {
+    uint32_t div, growth, full;
+  };
+  const Sched scheds[] = {{0, 0, 0}, {0, 0, 1}, {16, 4, 0}, {8, 8, 0}, {4, 8, 0}};
+  for (uint32_t d : digits)
+    for (const Sched &sc : scheds) {
+      dl_renderer_opts o = base;
+      o.sort_digit_bits = d;
+      o.full_lists = sc.full;
+      o.first_pass = sc.div ? (uint32_t)std::max<uint64_t>(65536, n / sc.div) : 0;
+      o.pass_growth = sc.growth;
+      out.push_back(o);
+    }
+  return out;
+}
+
+template <class F>
+float wall_ms_per_call(F &&f, int reps) {
+  f();  // warm: plans, allocations, graph capture
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < reps; k++) f();
+  const auto t1 = std::chrono::steady_clock::now();
+  return (float)(std::chrono::duration<double, std::milli>(t1 - t0).count() / reps);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -274,6 +313,74 @@ int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *
     if (!dst) return;
     if (bytes < t.size()) fail(DL_ERR_CAPACITY, "tap buffer too small");
     if (!t.empty()) std::memcpy(dst, t.data(), t.size());
+  });
+}
+
+
+int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, dl_tune_result *out) {
+  return guard([&] {
+    if (!r || !scene || !cam || !out) fail(DL_ERR_INVALID, "null argument");
+    if (scene->s.ctx != r->r.ctx) fail(DL_ERR_INVALID, "scene and renderer belong to different contexts");
+    Renderer &R = r->r;
+    const dl_renderer_opts before = R.opts;
+    const bool keep_taps = R.keep_taps, timing = R.timing, counting = R.counting;
+    R.keep_taps = R.timing = R.counting = false;
+    const dl_background bg{0.0f, 0.0f, 0.0f, 1.0f};
+    auto frame = [&] { R.render(scene->s.view(), *cam, bg); };
+    dl_tune_result res{};
+    res.default_ms = wall_ms_per_call(frame, 3);
+    res.chosen = before;
+    res.chosen_ms = res.default_ms;
+    for (const auto &o : tune_candidates(before, scene->s.count)) {
+      R.opts = o;
+      const float ms = wall_ms_per_call(frame, 3);
+      res.candidates++;
+      if (ms < res.chosen_ms) {
+        res.chosen_ms = ms;
+        res.chosen = o;
+      }
+    }
+    R.opts = res.chosen;
+    R.keep_taps = keep_taps;
+    R.timing = timing;
+    R.counting = counting;
+    *out = res;
+  });
+}
+
+int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, const dl_view *views,
+                      uint32_t num_views, dl_tune_result *out) {
+  return guard([&] {
+    if (!b || !scenes || !views || !out) fail(DL_ERR_INVALID, "null argument");
+    std::vector<SceneView> sv(num_scenes);
+    uint64_t nmax = 0;
+    for (uint32_t k = 0; k < num_scenes; k++) {
+      if (!scenes[k]) fail(DL_ERR_INVALID, "null scene");
+      if (scenes[k]->s.ctx != b->b.ctx) fail(DL_ERR_INVALID, "scene and batch belong to different contexts");
+      sv[k] = scenes[k]->s.view();
+      nmax = std::max<uint64_t>(nmax, sv[k].count);
+    }
+    Batch &B = b->b;
+    const dl_renderer_opts before = B.opts;
+    auto call = [&] { B.render(sv, views, num_views); };
+    dl_tune_result res{};
+    res.default_ms = wall_ms_per_call(call, 2);
+    res.chosen = before;
+    res.chosen_ms = res.default_ms;
+    for (const auto &o : tune_candidates(before, nmax)) {
+      if (o.sort_digit_bits == 4) continue;  // 8 passes over every view's keys: never faster here
+      B.opts = o;
+      B.layout.clear();  // a new schedule is a new plan (and graph)
+      const float ms = wall_ms_per_call(call, 2);
+      res.candidates++;
+      if (ms < res.chosen_ms) {
+        res.chosen_ms = ms;
+        res.chosen = o;
+      }
+    }
+    B.opts = res.chosen;
+    B.layout.clear();
+    *out = res;
   });
 }
 

@@ -1,6 +1,7 @@
 // Renderer state and kernel parameter blocks (host + device visible).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <vector>
 
@@ -30,6 +31,16 @@ struct ProjUniforms {
 };
 
 ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov);
+
+// Default schedule of the front-to-back passes: a first pass of
+// max(kPassMinFirst, visible / kPassFirstDiv) splats, each later pass
+// kPassGrowth times larger; the single-view renderer uses passes by default
+// from kPassMinPairs (tile, splat) pairs per frame (measured on MI355X:
+// below that the ~30 launches of a pass cost more than the binning saved).
+constexpr uint64_t kPassMinPairs = 64ull << 20;
+constexpr uint64_t kPassMinFirst = 65536;
+constexpr uint64_t kPassFirstDiv = 8;
+constexpr uint64_t kPassGrowth = 8;
 
 struct RastParams {
   uint32_t tiles_x, width, height;
@@ -74,6 +85,7 @@ struct Renderer {
 
   ~Renderer();
   void render(const SceneView &sc, const dl_camera &cam, const dl_background &bg);
+  int digit_bits() const { return opts.sort_digit_bits ? (int)std::min<uint32_t>(opts.sort_digit_bits, 8) : 8; }
   void read_image(uint8_t *out);
 
  private:

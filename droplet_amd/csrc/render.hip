@@ -324,7 +324,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     DL_HIP(hipGetLastError());
     mark(DL_STAGE_DEPTH_SORT);
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
-                                         tids.as<uint32_t>(), n, 32, sort_ws.get());
+                                         tids.as<uint32_t>(), n, 32, sort_ws.get(), digit_bits());
     sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
     const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
     DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -367,14 +367,17 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
                      d_iters);
   } else {
     const bool span = opts.binning == 0 && span_binning_supported(tiles_x, tiles_y);
-    // Front-to-back passes that stop filling closed tiles (span binning, no taps)
-    const bool early_stop = span && !keep_taps && !opts.full_lists;
+    // Front-to-back passes that stop filling closed tiles (span binning, no
+    // taps). A pass costs about 30 launches, so by default they are used only
+    // when the frame has enough pairs for the skipped binning to pay for them.
+    const bool early_stop = span && !keep_taps && !opts.full_lists &&
+                            (opts.first_pass || opts.pass_growth || pairs_all >= kPassMinPairs);
     std::vector<uint32_t> cuts;  // pass starts, then nvis
     if (opts.chunk_cap && opts.chunk_cap < nvis) {
       for (uint64_t c0 = 0; c0 < nvis; c0 += opts.chunk_cap) cuts.push_back((uint32_t)c0);
     } else if (early_stop) {
-      uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(65536, nvis / 16);
-      const uint64_t growth = opts.pass_growth ? opts.pass_growth : 4;
+      uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(kPassMinFirst, nvis / kPassFirstDiv);
+      const uint64_t growth = opts.pass_growth ? opts.pass_growth : kPassGrowth;
       for (uint64_t c0 = 0; c0 < nvis; c0 += len, len *= growth) cuts.push_back((uint32_t)c0);
     } else {
       cuts.push_back(0);
@@ -458,7 +461,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         DL_HIP(hipGetLastError());
         mark(DL_STAGE_PAIR_SORT);
         const bool in_tmp = radix_sort_pairs(s, pkeys.as<uint32_t>(), pvals.as<uint32_t>(), ptkeys.as<uint32_t>(),
-                                             ptvals.as<uint32_t>(), npairs, (int)key_bits, psort_ws.get());
+                                             ptvals.as<uint32_t>(), npairs, (int)key_bits, psort_ws.get(),
+                                             digit_bits());
         const uint32_t *skeys = in_tmp ? ptkeys.as<uint32_t>() : pkeys.as<uint32_t>();
         svals = in_tmp ? ptvals.as<uint32_t>() : pvals.as<uint32_t>();
         mark(DL_STAGE_RANGES);

@@ -38,6 +38,7 @@ class RendererOptions:
     first_pass: Optional[int] = None
     pass_growth: Optional[int] = None
     full_lists: bool = False
+    sort_digit_bits: Optional[int] = None   # radix digit width, 1..8 (default 8)
 
 
 class Device:
@@ -116,6 +117,7 @@ class Renderer:
         o.first_pass = int(opts.first_pass or 0)
         o.pass_growth = int(opts.pass_growth or 0)
         o.full_lists = 1 if opts.full_lists else 0
+        o.sort_digit_bits = int(opts.sort_digit_bits or 0)
         h = C.c_void_p()
         L.check(L.load().dl_renderer_create(device.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -177,6 +179,14 @@ class Renderer:
             "pairs_consumed": int(st.pairs_consumed), "list_entries": int(st.list_entries),
         }
 
+    def autotune(self, scene: "Scene", camera: RenderCamera) -> dict:
+        """Times candidate options (radix digit width, pass schedule) on this
+        scene and camera and keeps the fastest (dl_renderer_autotune)."""
+        res = L.DlTuneResult()
+        cam = L.camera_struct(camera)
+        L.check(L.load().dl_renderer_autotune(self.handle, scene._h, C.byref(cam), C.byref(res)))
+        return tune_result_dict(res)
+
     def tap(self, name: str) -> np.ndarray:
         which = L.TAPS.index(name)
         n = C.c_size_t()
@@ -197,6 +207,14 @@ class Renderer:
             self.destroy()
         except Exception:
             pass
+
+
+def tune_result_dict(res) -> dict:
+    o = res.chosen
+    return {"chosen": {"sort_digit_bits": o.sort_digit_bits or 8, "first_pass": o.first_pass,
+                       "pass_growth": o.pass_growth, "full_lists": bool(o.full_lists)},
+            "chosen_ms": float(res.chosen_ms), "default_ms": float(res.default_ms),
+            "candidates": int(res.candidates)}
 
 
 def sort_pairs_u32(device: Device, keys: np.ndarray, vals: np.ndarray, key_bits: int = 32):

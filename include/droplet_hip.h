@@ -91,13 +91,17 @@ typedef struct {
      * splats. After each pass, tiles whose pixels all have T < 1e-4 are
      * closed. Later passes bin nothing into closed tiles: the reference's
      * per-pixel loop breaks before it reads those entries, so the image is
-     * unchanged. first_pass is the number of splats in the first pass
-     * (0: automatic), and each later pass is pass_growth times larger
-     * (0: automatic). full_lists = 1 bins every pair in one pass, as the
-     * reference does. Debug taps always use full lists. */
+     * unchanged. first_pass is the number of splats in the first pass,
+     * and each later pass is pass_growth times larger. With both 0 the
+     * renderer uses passes (max(65536, visible/8) first, growth 8) only
+     * for frames with at least 2^26 pairs, and the batch always uses
+     * them. full_lists = 1 bins every pair in one pass, as the reference
+     * does. Debug taps always use full lists. */
     uint32_t first_pass;
     uint32_t pass_growth;
     uint32_t full_lists;
+    /* digit width of the LSD radix sorts, 1..8 bits (0: 8) */
+    uint32_t sort_digit_bits;
 } dl_renderer_opts;
 
 /* ---- device (replaces createDevice, src/gpu/device.ts:27-41) ---- */
@@ -168,6 +172,18 @@ int dl_renderer_set_taps(dl_renderer *r, int enabled);
 /* bytes: capacity of dst; returns DL_ERR_CAPACITY if too small */
 int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *written);
 
+/* ---- autotuner (no reference counterpart). It times candidate options on
+ * the caller's own workload and keeps the fastest for later calls. The
+ * candidates are the radix digit width and the front-to-back pass schedule.
+ * Every candidate produces the same image. */
+typedef struct {
+    dl_renderer_opts chosen;     /* options in effect after the call */
+    float chosen_ms;             /* wall ms per frame (per call for a batch) with them */
+    float default_ms;            /* the same with the options the renderer had before */
+    uint32_t candidates;         /* configurations timed */
+} dl_tune_result;
+int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, dl_tune_result *out);
+
 /* ---- batched views (no reference counterpart: it stands beside
  * Renderer.render, src/render/pipeline.ts:181-185, whose per-view image it
  * reproduces bit for bit). Many views over several resident scenes go
@@ -202,6 +218,8 @@ int dl_batch_render(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, c
 /* RGBA8 of view `view` of the last call (width*height*4 bytes) */
 int dl_batch_read_image(dl_batch *b, uint32_t view, uint8_t *out_rgba);
 int dl_batch_get_stats(dl_batch *b, dl_batch_stats *out);
+int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, const dl_view *views,
+                      uint32_t num_views, dl_tune_result *out);
 
 /* ---- stable u32 key/value sort (replaces GpuRadixSorter.sort,
  * src/gpu/radix-sort.ts:142-215, which is unstable; this one is stable).

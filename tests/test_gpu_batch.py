@@ -148,3 +148,17 @@ def test_batch_rejects_oversized_view(device, scenes):
             b.render(sc, [View(7, _orbit(0, 1))])
     finally:
         b.destroy()
+
+
+def test_batch_autotune_keeps_images(device, scenes):
+    datas, sc = scenes
+    b = BatchRenderer(device)
+    try:
+        views = _views(0.5, cams=3)
+        res = b.autotune(sc, views)
+        print(f"case: batch autotune {res}", flush=True)
+        assert res["candidates"] >= 5
+        b.render(sc, views)
+        _check_against_plain(device, datas, b, views)
+    finally:
+        b.destroy()

@@ -290,3 +290,32 @@ def test_front_to_back_passes_skip_closed_tiles(device):
     finally:
         r.destroy()
         full.destroy()
+
+
+def test_autotune_keeps_the_image(device):
+    data = S.gaussian_cloud(150_000, seed=41, pos_std=0.6, pos_clip=2.5)
+    cam = S.default_camera(640, 360)
+    scene = Scene(device, data)
+    r = Renderer(device)
+    try:
+        before = r.render_scene(scene, cam)
+        res = r.autotune(scene, cam)
+        print(f"case: autotune {res}", flush=True)
+        assert res["candidates"] >= 10
+        assert res["chosen_ms"] <= res["default_ms"]
+        after = r.render_scene(scene, cam)
+        np.testing.assert_array_equal(after, before)
+        np.testing.assert_array_equal(after, oracle.render(data, cam, taps=False)["rgba"])
+    finally:
+        r.destroy()
+        scene.destroy()
+
+
+@pytest.mark.parametrize("digit_bits", [4, 5, 7])
+def test_sort_digit_width_keeps_the_image(device, digit_bits):
+    data, cam = S.plain_render_workload(seed=42)
+    r = Renderer(device, RendererOptions(sort_digit_bits=digit_bits))
+    try:
+        assert_render_parity(r, data, cam)
+    finally:
+        r.destroy()
