@@ -241,6 +241,12 @@ def main() -> int:
     t1 = time.perf_counter()
     img = r.read_image()
     readback_ms = (time.perf_counter() - t1) * 1e3
+    # the reference's own boundary: host arrays in, RGBA8 out (PCIe both ways)
+    r.render(data, cam)
+    t1 = time.perf_counter()
+    r.render(data, cam)
+    host_ms = (time.perf_counter() - t1) * 1e3
+    note(f"host-array render (upload + frame + readback): {host_ms:.1f} ms")
     st = r.stats()
     note(f"timed {args.steps} frames: {ms_per_frame:.3f} ms/frame")
 
@@ -276,6 +282,7 @@ def main() -> int:
         "autotune": tune,
         "stage_ms": stage_ms,
         "readback_ms": round(readback_ms, 3),
+        "host_arrays_ms_per_frame": round(host_ms, 3),
         "image_nonblack_px": int(((img.reshape(-1, 4)[:, :3] > 5).any(1)).sum()),
     }
     if not args.no_voxel:

@@ -63,6 +63,11 @@ class DlRenderStats(C.Structure):
                 ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64)]
 
 
+class DlGrads(C.Structure):
+    _fields_ = [("positions", f32p), ("rotations", f32p), ("log_scales", f32p), ("opacity_logits", f32p),
+                ("colors_dc", f32p), ("colors_sh", f32p)]
+
+
 class DlTuneResult(C.Structure):
     _fields_ = [("chosen", DlRendererOpts), ("chosen_ms", C.c_float), ("default_ms", C.c_float),
                 ("candidates", C.c_uint32)]
@@ -100,6 +105,8 @@ SIGNATURES = [
     ("dl_renderer_stats", C.c_int, [C.c_void_p, C.POINTER(DlRenderStats)]),
     ("dl_renderer_set_taps", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_tap", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("dl_render_forward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlBackground), f32p]),
+    ("dl_render_backward", C.c_int, [C.c_void_p, f32p, C.POINTER(DlGrads)]),
     ("dl_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlTuneResult)]),
     ("dl_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DlView), C.c_uint32,
                                     C.POINTER(DlTuneResult)]),

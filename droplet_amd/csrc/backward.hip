@@ -1,0 +1,603 @@
+// Backward of the forward splat renderer on gfx950 (no reference counterpart:
+// droplet renders only; README.md:106). Differentiates the forward exactly
+// as render.hip evaluates it: the rasterizer loop of rasterize.ts:62-79 and
+// the EWA projection of project.ts:111-285, quirks included.
+//
+//   k_image_rgb          the differentiable image: clamp(C + T * bg, 0, 1)
+//                        per channel (finalize.ts:31-35 before quantisation)
+//   k_backward_raster    one wave per tile. Each pixel re-runs its list front
+//                        to back, recomputing T instead of storing it, and
+//                        takes dL/d(sx, sy, A, B, C, a, r, g, b) of every
+//                        splat it composited. The wave sums over its 256
+//                        pixels and adds once per (tile, splat).
+//   k_backward_project   per splat: chain rule from those 9 values to the
+//                        position, rotation, log-scale, opacity logit, DC
+//                        and SH coefficients, recomputing the projection.
+//
+// Compositing: out_c = sum_i col_ic a_i T_i + T_end bg_c with T_{i+1} =
+// T_i (1 - a_i). d out_c / d col_ic = a_i T_i and d out_c / d a_i =
+// col_ic T_i - S_ic / (1 - a_i), where S_ic = out_c - sum_{j<=i} col_jc a_j
+// T_j is what lies behind splat i (including the background).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "dl_common.h"
+#include "dl_internal.h"
+#include "render.h"
+#include "render_device.h"
+
+namespace dl {
+
+namespace {
+
+constexpr int kSplatGrads = 9;  // sx, sy, A, B, C, a, r, g, b
+
+__global__ __launch_bounds__(256) void k_image_rgb(const float4 *__restrict__ accum, size_t npx, float bg_r,
+                                                   float bg_g, float bg_b, float *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npx) return;
+  const float4 a = accum[i];
+  out[3 * i + 0] = wclamp(a.x + a.w * bg_r, 0.0f, 1.0f);
+  out[3 * i + 1] = wclamp(a.y + a.w * bg_g, 0.0f, 1.0f);
+  out[3 * i + 2] = wclamp(a.z + a.w * bg_b, 0.0f, 1.0f);
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__device__ inline float wave_sum_f(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(64) void k_backward_raster(const float4 *__restrict__ rec,
+                                                        const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ pvals,
+                                                        const float4 *__restrict__ accum,
+                                                        const float *__restrict__ grad_rgb, RastParams p,
+                                                        float *__restrict__ gsplat) {
+  __shared__ float4 s_rec[64 * 3];
+  __shared__ uint32_t s_id[64];
+  __shared__ uint32_t s_mask[64];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+  const int lane = threadIdx.x;
+  const uint32_t px = tx * kTile + (lane & 15);
+  const uint32_t py0 = ty * kTile + (lane >> 4);
+  const float pxf = (float)px + 0.5f;
+  const float bg[3] = {p.bg_r, p.bg_g, p.bg_b};
+  float T[4], pyf[4], G[4][3], S[4][3];
+  bool live[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t y = py0 + 4 * k;
+    const bool inb = px < p.width && y < p.height;
+    pyf[k] = (float)y + 0.5f;
+    T[k] = 1.0f;
+    live[k] = inb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      G[k][c] = 0.0f;
+      S[k][c] = 0.0f;
+    }
+    if (inb) {
+      const size_t pix = (size_t)y * p.width + px;
+      const float4 a = accum[pix];
+      const float fin[3] = {a.x, a.y, a.z};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        // clamp(x, 0, 1) passes the gradient only strictly inside (0, 1)
+        const float pre = fin[c] + a.w * bg[c];
+        G[k][c] = (pre > 0.0f && pre < 1.0f) ? grad_rgb[3 * pix + c] : 0.0f;
+        S[k][c] = pre;  // what lies behind the next splat: starts as the whole output
+      }
+    }
+  }
+  const uint32_t start = off[tile], end = off[tile + 1];
+  const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
+  for (uint32_t base = start; base < end; base += 64) {
+    uint32_t strips_live = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) strips_live |= (__ballot(live[k]) != 0 ? 1u : 0u) << k;
+    if (strips_live == 0) break;
+    const uint32_t i = base + lane;
+    if (i < end) {
+      const uint32_t j = pvals[i];
+      const float4 a = rec[3 * (size_t)j + 0];
+      const float4 b = rec[3 * (size_t)j + 1];
+      const float4 col = rec[3 * (size_t)j + 2];
+      s_rec[3 * lane + 0] = a;
+      s_rec[3 * lane + 1] = b;
+      s_rec[3 * lane + 2] = col;
+      s_id[lane] = j;
+      uint32_t m = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
+        if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+      }
+      s_mask[lane] = m;
+    }
+    __syncthreads();
+    const uint32_t cnt = min(64u, end - base);
+    for (uint32_t c = 0; c < cnt; c++) {
+      const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[c]) & strips_live;
+      if (m == 0) continue;
+      const float4 a = s_rec[3 * c + 0];
+      const float4 b = s_rec[3 * c + 1];
+      const float4 col = s_rec[3 * c + 2];
+      float g[kSplatGrads];
+#pragma unroll
+      for (int q = 0; q < kSplatGrads; q++) g[q] = 0.0f;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!((m >> k) & 1u)) continue;
+        if (!live[k]) continue;
+        const float dx = pxf - a.x;
+        const float dy = pyf[k] - a.y;
+        const float r = a.z;
+        if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
+        const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
+        if (power > 0.0f || power < col.w) continue;
+        const float e = dl_expf(power);
+        const float raw = b.w * wmax(0.0f, e - kGaussFloor);
+        const float alpha = wmin(kOpacityCap, raw);
+        if (alpha < kMinAlpha) continue;
+        any = true;
+        const float w = T[k] * alpha;
+        const float cc[3] = {col.x, col.y, col.z};
+        float dalpha = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          S[k][q] = S[k][q] - cc[q] * w;  // now: everything behind this splat
+          g[6 + q] += G[k][q] * w;
+          dalpha += G[k][q] * (cc[q] * T[k] - S[k][q] / (1.0f - alpha));
+        }
+        if (raw < kOpacityCap) {
+          // alpha = a * (e - floor), e = exp(power)
+          g[5] += dalpha * (e - kGaussFloor);
+          const float dpow = dalpha * b.w * e;
+          g[0] += dpow * (b.x * dx + b.y * dy);
+          g[1] += dpow * (b.y * dx + b.z * dy);
+          g[2] += dpow * (-0.5f * dx * dx);
+          g[3] += dpow * (-dx * dy);
+          g[4] += dpow * (-0.5f * dy * dy);
+        }
+        T[k] = T[k] * (1.0f - alpha);
+        live[k] = !(T[k] < kMinT);
+      }
+      if (__ballot(any) == 0) continue;
+#pragma unroll
+      for (int q = 0; q < kSplatGrads; q++) g[q] = wave_sum_f(g[q]);
+      if (lane == 0) {
+        float *dst = gsplat + (size_t)kSplatGrads * s_id[c];
+#pragma unroll
+        for (int q = 0; q < kSplatGrads; q++)
+          if (g[q] != 0.0f) atomicAdd(dst + q, g[q]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// d basis / d (x, y, z) of the SH terms of bands 1..3 in eval_sh's order
+// (coefficient k of a channel), accumulated as gd += w_k * grad b_k.
+template <int BANDS>
+__device__ inline void sh_dir_grad(const float *w, float x, float y, float z, float gd[3]) {
+  const float SH1 = (float)0.4886025119029199;
+  gd[1] += w[0] * (-SH1);
+  gd[2] += w[1] * SH1;
+  gd[0] += w[2] * (-SH1);
+  if constexpr (BANDS >= 2) {
+    const float K0 = (float)1.0925484305920792, K2 = (float)0.31539156525252005, K4 = (float)0.5462742152960396;
+    gd[0] += w[3] * (K0 * y);
+    gd[1] += w[3] * (K0 * x);
+    gd[1] += w[4] * (-K0 * z);
+    gd[2] += w[4] * (-K0 * y);
+    gd[0] += w[5] * (-2.0f * K2 * x);
+    gd[1] += w[5] * (-2.0f * K2 * y);
+    gd[2] += w[5] * (4.0f * K2 * z);
+    gd[0] += w[6] * (-K0 * z);
+    gd[2] += w[6] * (-K0 * x);
+    gd[0] += w[7] * (2.0f * K4 * x);
+    gd[1] += w[7] * (-2.0f * K4 * y);
+    if constexpr (BANDS >= 3) {
+      const float Q0 = (float)-0.5900435899266435, Q1 = (float)2.890611442640554;
+      const float Q2 = (float)-0.4570457994644658, Q3 = (float)0.3731763325901154, Q5 = (float)1.445305721320277;
+      const float xx = x * x, yy = y * y, zz = z * z;
+      gd[0] += w[8] * (Q0 * 6.0f * x * y);
+      gd[1] += w[8] * (Q0 * (3.0f * xx - 3.0f * yy));
+      gd[0] += w[9] * (Q1 * y * z);
+      gd[1] += w[9] * (Q1 * x * z);
+      gd[2] += w[9] * (Q1 * x * y);
+      gd[0] += w[10] * (Q2 * y * (-2.0f * x));
+      gd[1] += w[10] * (Q2 * (4.0f * zz - xx - 3.0f * yy));
+      gd[2] += w[10] * (Q2 * y * 8.0f * z);
+      gd[0] += w[11] * (Q3 * z * (-6.0f * x));
+      gd[1] += w[11] * (Q3 * z * (-6.0f * y));
+      gd[2] += w[11] * (Q3 * (6.0f * zz - 3.0f * xx - 3.0f * yy));
+      gd[0] += w[12] * (Q2 * (4.0f * zz - 3.0f * xx - yy));
+      gd[1] += w[12] * (Q2 * x * (-2.0f * y));
+      gd[2] += w[12] * (Q2 * x * 8.0f * z);
+      gd[0] += w[13] * (Q5 * z * 2.0f * x);
+      gd[1] += w[13] * (-Q5 * z * 2.0f * y);
+      gd[2] += w[13] * (Q5 * (xx - yy));
+      gd[0] += w[14] * (Q0 * (3.0f * xx - 3.0f * yy));
+      gd[1] += w[14] * (Q0 * x * (-6.0f * y));
+    }
+  }
+}
+
+// SH basis values in eval_sh's coefficient order
+template <int BANDS>
+__device__ inline void sh_basis(float x, float y, float z, float *bv) {
+  const float SH1 = (float)0.4886025119029199;
+  bv[0] = (-SH1) * y;
+  bv[1] = SH1 * z;
+  bv[2] = (-SH1) * x;
+  if constexpr (BANDS >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    bv[3] = (float)1.0925484305920792 * (x * y);
+    bv[4] = (float)-1.0925484305920792 * (y * z);
+    bv[5] = (float)0.31539156525252005 * ((2.0f * zz - xx) - yy);
+    bv[6] = (float)-1.0925484305920792 * (x * z);
+    bv[7] = (float)0.5462742152960396 * (xx - yy);
+    if constexpr (BANDS >= 3) {
+      bv[8] = ((float)-0.5900435899266435 * y) * (3.0f * xx - yy);
+      bv[9] = ((float)2.890611442640554 * (x * y)) * z;
+      bv[10] = ((float)-0.4570457994644658 * y) * ((4.0f * zz - xx) - yy);
+      bv[11] = ((float)0.3731763325901154 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+      bv[12] = ((float)-0.4570457994644658 * x) * ((4.0f * zz - xx) - yy);
+      bv[13] = ((float)1.445305721320277 * z) * (xx - yy);
+      bv[14] = ((float)-0.5900435899266435 * x) * (xx - 3.0f * yy);
+    }
+  }
+}
+
+// Gradient of o = quat_rot(q, v) (project.ts: o = v + w t + qv x t, t = 2 qv
+// x v, for any q, normalised or not) with respect to q, given go = dL/do.
+__device__ inline void quat_rot_grad(const float q[4], const float v[3], const float go[3], float gq[4]) {
+  const float qv[3] = {q[1], q[2], q[3]};
+  const float cx[3] = {qv[1] * v[2] - qv[2] * v[1], qv[2] * v[0] - qv[0] * v[2], qv[0] * v[1] - qv[1] * v[0]};
+  // o = v + 2 w (qv x v) + 2 qv (qv . v) - 2 v |qv|^2
+  gq[0] += 2.0f * (go[0] * cx[0] + go[1] * cx[1] + go[2] * cx[2]);
+  const float vxg[3] = {v[1] * go[2] - v[2] * go[1], v[2] * go[0] - v[0] * go[2], v[0] * go[1] - v[1] * go[0]};
+  const float qdv = qv[0] * v[0] + qv[1] * v[1] + qv[2] * v[2];
+  const float gdq = go[0] * qv[0] + go[1] * qv[1] + go[2] * qv[2];
+  const float gdv = go[0] * v[0] + go[1] * v[1] + go[2] * v[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    gq[1 + k] += 2.0f * q[0] * vxg[k] + 2.0f * (go[k] * qdv + v[k] * gdq) - 4.0f * gdv * qv[k];
+}
+
+template <int BANDS>
+__global__ __launch_bounds__(256) void k_backward_project(SceneView sc, ProjUniforms u,
+                                                          const float4 *__restrict__ rec,
+                                                          const float *__restrict__ gsplat,
+                                                          float *__restrict__ gpos, float *__restrict__ grot,
+                                                          float *__restrict__ gls, float *__restrict__ glogit,
+                                                          float *__restrict__ gdc, float *__restrict__ gsh) {
+  constexpr int M = BANDS == 0 ? 0 : BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+  const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= sc.count) return;
+  float gp[3] = {0, 0, 0}, gq[4] = {0, 0, 0, 0}, gl[3] = {0, 0, 0}, go = 0.0f, gd[3] = {0, 0, 0};
+  float gshv[3 * (M > 0 ? M : 1)];
+#pragma unroll
+  for (int k = 0; k < 3 * (M > 0 ? M : 1); k++) gshv[k] = 0.0f;
+  const float *g9 = gsplat + (size_t)kSplatGrads * id;
+  float g[kSplatGrads];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < kSplatGrads; q++) {
+    g[q] = g9[q];
+    any = any || g[q] != 0.0f;
+  }
+  const float4 r0 = rec[3 * (size_t)id];
+  if (any && r0.z > 0.0f) {
+    // ---- recompute the projection (project_splat) ----
+    const float *m = u.view;
+    const float px = sc.pos[3 * (size_t)id], py = sc.pos[3 * (size_t)id + 1], pz = sc.pos[3 * (size_t)id + 2];
+    const float4 qv4 = reinterpret_cast<const float4 *>(sc.rot)[id];
+    const float q[4] = {qv4.x, qv4.y, qv4.z, qv4.w};
+    const float s[3] = {dl_expf(sc.ls[3 * (size_t)id]), dl_expf(sc.ls[3 * (size_t)id + 1]),
+                        dl_expf(sc.ls[3 * (size_t)id + 2])};
+    const float logit = sc.op[id];
+    const float pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
+    const float pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
+    const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
+    const float f = u.focal;
+    float R[3][3];  // columns: quat_rot(q, e_j)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float e[3] = {0, 0, 0};
+      e[j] = 1.0f;
+      float o[3];
+      quat_rot(q, e[0], e[1], e[2], o);
+      R[0][j] = o[0];
+      R[1][j] = o[1];
+      R[2][j] = o[2];
+    }
+    float Mx[3][3];  // M = R diag(s)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) Mx[a][j] = R[a][j] * s[j];
+    float C3[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) C3[a][b] = (Mx[a][0] * Mx[b][0] + Mx[a][1] * Mx[b][1]) + Mx[a][2] * Mx[b][2];
+    const float W[3][3] = {{m[0], m[4], m[8]}, {m[1], m[5], m[9]}, {m[2], m[6], m[10]}};
+    float V[3][3];  // W C3 W^T
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+          for (int l = 0; l < 3; l++) acc += W[a][k] * C3[k][l] * W[b][l];
+        V[a][b] = acc;
+      }
+    const float iz = 1.0f / pcz;
+    const float j00 = f * iz, j02 = ((-f) * pcx) * iz * iz, j11 = f * iz, j12 = ((-f) * pcy) * iz * iz;
+    const float cov00 = j00 * j00 * V[0][0] + 2.0f * j00 * j02 * V[0][2] + j02 * j02 * V[2][2];
+    const float cov01 = j00 * j11 * V[0][1] + j00 * j12 * V[0][2] + j02 * j11 * V[1][2] + j02 * j12 * V[2][2];
+    const float cov11 = j11 * j11 * V[1][1] + 2.0f * j11 * j12 * V[1][2] + j12 * j12 * V[2][2];
+    const float det_orig = cov00 * cov11 - cov01 * cov01;
+    const float d00 = cov00 + kAA, d11 = cov11 + kAA;
+    const float det = d00 * d11 - cov01 * cov01;
+    const float ratio = det_orig / det;
+    const float aa = __builtin_sqrtf(wmax(ratio, 0.0f));
+    const float mid = 0.5f * (d00 + d11);
+    const float disc = mid * mid - det;
+    const float r_eig = __builtin_sqrtf(wmax(kEigMin, disc));
+    const float lambda1 = mid + r_eig;
+    const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
+    const float fade_s = kFadeStart * u.height, fade_e = kFadeEnd * u.height;
+    const float fade_arg = (fade_e - radius_raw) / (fade_e - fade_s);
+    const float fade = wclamp(fade_arg, 0.0f, 1.0f);
+    const float sig = dl_sigmoidf(logit);
+    // ---- colour ----
+    float len = 1.0f, dir[3] = {0, 0, 0};
+    float bv[M > 0 ? M : 1];
+    float colpre[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
+    if constexpr (BANDS > 0) {
+      len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+      dir[0] = pcx / len;
+      dir[1] = pcy / len;
+      dir[2] = pcz / len;
+      sh_basis<BANDS>(dir[0], dir[1], dir[2], bv);
+      const float *shc = sc.sh + (size_t)id * 3 * M;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < M; k++) colpre[c] = colpre[c] + bv[k] * shc[c * M + k];
+    }
+    // ---- backward ----
+    float gpc[3] = {0, 0, 0};
+    float gdir[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float gc = (colpre[c] + 0.5f > 0.0f) ? g[6 + c] : 0.0f;  // max(col + 0.5, 0)
+      gd[c] = gc;
+      if constexpr (BANDS > 0) {
+        const float *shc = sc.sh + (size_t)id * 3 * M;
+        float w[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+          gshv[c * M + k] = gc * bv[k];
+          w[k] = gc * shc[c * M + k];
+        }
+        sh_dir_grad<BANDS>(w, dir[0], dir[1], dir[2], gdir);
+      }
+    }
+    if constexpr (BANDS > 0) {
+      // dir = pc / |pc|
+      const float dd = gdir[0] * dir[0] + gdir[1] * dir[1] + gdir[2] * dir[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) gpc[k] += (gdir[k] - dd * dir[k]) / len;
+    }
+    // opacity a = (sigmoid(logit) * fade) * aa
+    const float ga = g[5];
+    go = ga * fade * aa * sig * (1.0f - sig);
+    const float gaa = ga * sig * fade;
+    const float gfade = ga * sig * aa;
+    float glambda = 0.0f;
+    if (fade_arg > 0.0f && fade_arg < 1.0f) {
+      const float grr = gfade * (-1.0f / (fade_e - fade_s));
+      glambda += grr * kSigma * 0.5f / __builtin_sqrtf(lambda1);
+    }
+    // conic A = d11 / det, B = -cov01 / det, C = d00 / det
+    float gd00 = g[4] / det, gd11 = g[2] / det, gc01 = -g[3] / det;
+    float gdet = -(g[2] * d11 - g[3] * cov01 + g[4] * d00) / (det * det);
+    float gdo = 0.0f;
+    if (ratio > 0.0f) {
+      const float gr = gaa * 0.5f / aa;  // d sqrt(ratio)
+      gdo += gr / det;
+      gdet += -gr * det_orig / (det * det);
+    }
+    // lambda1 = mid + sqrt(max(kEigMin, mid^2 - det))
+    float gmid = glambda;
+    if (disc > kEigMin) {
+      gmid += glambda * mid / r_eig;
+      gdet += -glambda * 0.5f / r_eig;
+    }
+    gd00 += 0.5f * gmid;
+    gd11 += 0.5f * gmid;
+    // det = d00 d11 - cov01^2 ; det_orig = cov00 cov11 - cov01^2
+    gd00 += gdet * d11;
+    gd11 += gdet * d00;
+    gc01 += -2.0f * gdet * cov01;
+    float gc00 = gd00 + gdo * cov11;
+    float gc11 = gd11 + gdo * cov00;
+    gc01 += -2.0f * gdo * cov01;
+    // cov2 = J V J^T (unique entries cov00, cov01, cov11)
+    float gV[3][3];
+    gV[0][0] = gc00 * j00 * j00;
+    gV[1][1] = gc11 * j11 * j11;
+    gV[2][2] = gc00 * j02 * j02 + gc01 * j02 * j12 + gc11 * j12 * j12;
+    gV[0][1] = gc01 * j00 * j11;                                  // unique-entry gradients (off-diagonal
+    gV[0][2] = 2.0f * gc00 * j00 * j02 + gc01 * j00 * j12;        // entries once)
+    gV[1][2] = gc01 * j02 * j11 + 2.0f * gc11 * j11 * j12;
+    const float gj00 = gc00 * (2.0f * j00 * V[0][0] + 2.0f * j02 * V[0][2]) + gc01 * (j11 * V[0][1] + j12 * V[0][2]);
+    const float gj02 = gc00 * (2.0f * j00 * V[0][2] + 2.0f * j02 * V[2][2]) + gc01 * (j11 * V[1][2] + j12 * V[2][2]);
+    const float gj11 = gc01 * (j00 * V[0][1] + j02 * V[1][2]) + gc11 * (2.0f * j11 * V[1][1] + 2.0f * j12 * V[1][2]);
+    const float gj12 = gc01 * (j00 * V[0][2] + j02 * V[2][2]) + gc11 * (2.0f * j11 * V[1][2] + 2.0f * j12 * V[2][2]);
+    // J(pc): j00 = j11 = f/z, j02 = -f x / z^2, j12 = -f y / z^2
+    gpc[0] += gj02 * (-f * iz * iz);
+    gpc[1] += gj12 * (-f * iz * iz);
+    gpc[2] += (gj00 + gj11) * (-f * iz * iz) + gj02 * (2.0f * f * pcx * iz * iz * iz) +
+              gj12 * (2.0f * f * pcy * iz * iz * iz);
+    // screen position: sx = -f x / z + cx, sy = f y / z + cy
+    gpc[0] += g[0] * (-f * iz);
+    gpc[2] += g[0] * (f * pcx * iz * iz);
+    gpc[1] += g[1] * (f * iz);
+    gpc[2] += g[1] * (-f * pcy * iz * iz);
+    // V = W C3 W^T: symmetric gradient Gh (off-diagonal halves), dC3 = W^T Gh W
+    float Gh[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        Gh[a][b] = (a == b) ? gV[a][a] : 0.5f * gV[lo][hi];
+      }
+    float GC[3][3];  // dL/dC3 as a full symmetric matrix
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) acc += W[a][k] * Gh[a][b] * W[b][l];
+        GC[k][l] = acc;
+      }
+    // C3 = M M^T: dL/dM = 2 GC M
+    float gs[3] = {0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float gm[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) gm[a] = 2.0f * ((GC[a][0] * Mx[0][j] + GC[a][1] * Mx[1][j]) + GC[a][2] * Mx[2][j]);
+      // M[:, j] = R e_j * s_j
+      gs[j] = gm[0] * R[0][j] + gm[1] * R[1][j] + gm[2] * R[2][j];
+      const float gr[3] = {gm[0] * s[j], gm[1] * s[j], gm[2] * s[j]};
+      float e[3] = {0, 0, 0};
+      e[j] = 1.0f;
+      quat_rot_grad(q, e, gr, gq);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) gl[j] = gs[j] * s[j];
+    // pc = W p + t
+#pragma unroll
+    for (int k = 0; k < 3; k++) gp[k] = (W[0][k] * gpc[0] + W[1][k] * gpc[1]) + W[2][k] * gpc[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    gpos[3 * (size_t)id + k] = gp[k];
+    gls[3 * (size_t)id + k] = gl[k];
+    gdc[3 * (size_t)id + k] = gd[k];
+  }
+  reinterpret_cast<float4 *>(grot)[id] = make_float4(gq[0], gq[1], gq[2], gq[3]);
+  glogit[id] = go;
+  if constexpr (M > 0) {
+#pragma unroll
+    for (int k = 0; k < 3 * M; k++) gsh[(size_t)id * 3 * M + k] = gshv[k];
+  }
+}
+
+}  // namespace
+
+void Renderer::forward_train(const SceneView &sc, const dl_camera &cam, const dl_background &bg, float *out_rgb) {
+  if (cam.projection != DL_PINHOLE)
+    fail(DL_ERR_INVALID, "the differentiable render supports pinhole cameras (equirect has no backward)");
+  train_mode = true;
+  try {
+    render(sc, cam, bg);
+  } catch (...) {
+    train_mode = false;
+    train_valid = false;
+    throw;
+  }
+  train_mode = false;
+  train_valid = true;
+  train_sc = sc;
+  train_u = make_uniforms(cam, max_cov);
+  train_bg = bg;
+  hipStream_t s = ctx->stream;
+  const size_t npx = (size_t)width * height;
+  img_rgb.reserve(npx * 12);
+  k_image_rgb<<<div_up(npx, 256), 256, 0, s>>>(accum.as<float4>(), npx, bg.r, bg.g, bg.b, img_rgb.as<float>());
+  DL_HIP(hipGetLastError());
+  if (out_rgb) DL_HIP(hipMemcpyAsync(out_rgb, img_rgb.get(), npx * 12, hipMemcpyDeviceToHost, s));
+  DL_HIP(hipStreamSynchronize(s));
+}
+
+void Renderer::backward(const float *grad_rgb, const dl_grads &out) {
+  if (!train_valid) fail(DL_ERR_INVALID, "dl_render_backward needs a preceding dl_render_forward");
+  const SceneView &sc = train_sc;
+  const uint32_t n = sc.count;
+  const size_t npx = (size_t)width * height;
+  const uint32_t M = sc.bands == 0 ? 0 : sc.bands == 1 ? 3 : sc.bands == 2 ? 8 : 15;
+  hipStream_t s = ctx->stream;
+  ctx->make_current();
+  grad_img.reserve(std::max<size_t>(npx, 1) * 12);
+  DL_HIP(hipMemcpyAsync(grad_img.get(), grad_rgb, npx * 12, hipMemcpyHostToDevice, s));
+  const size_t nn = std::max<uint32_t>(n, 1);
+  gsplat.reserve(nn * kSplatGrads * 4);
+  gparams.reserve(nn * (3 + 4 + 3 + 1 + 3 + 3 * (size_t)M) * 4 + 1024);
+  float *gpos = gparams.as<float>();
+  float *grot = gpos + 3 * nn;
+  float *gls = grot + 4 * nn;
+  float *glogit = gls + 3 * nn;
+  float *gdc = glogit + nn;
+  float *gsh = gdc + 3 * nn;
+  if (n) {
+    k_zero_u32<<<div_up((size_t)n * kSplatGrads, 256), 256, 0, s>>>(gsplat.as<uint32_t>(), (size_t)n * kSplatGrads);
+    DL_HIP(hipGetLastError());
+    if (stats.num_visible > 0 && stats.num_pairs > 0) {
+      RastParams p{};
+      p.tiles_x = stats.tiles_x;
+      p.width = width;
+      p.height = height;
+      p.bg_r = train_bg.r;
+      p.bg_g = train_bg.g;
+      p.bg_b = train_bg.b;
+      p.bg_a = train_bg.a;
+      k_backward_raster<<<stats.num_tiles, 64, 0, s>>>(rec.as<float4>(), ranges.as<uint32_t>(), train_vals, accum.as<float4>(),
+                                                      grad_img.as<float>(), p, gsplat.as<float>());
+      DL_HIP(hipGetLastError());
+    }
+    const uint32_t blocks = div_up(n, 256);
+    switch (sc.bands) {
+      case 0: k_backward_project<0><<<blocks, 256, 0, s>>>(sc, train_u, rec.as<float4>(), gsplat.as<float>(), gpos, grot, gls, glogit, gdc, gsh); break;
+      case 1: k_backward_project<1><<<blocks, 256, 0, s>>>(sc, train_u, rec.as<float4>(), gsplat.as<float>(), gpos, grot, gls, glogit, gdc, gsh); break;
+      case 2: k_backward_project<2><<<blocks, 256, 0, s>>>(sc, train_u, rec.as<float4>(), gsplat.as<float>(), gpos, grot, gls, glogit, gdc, gsh); break;
+      default: k_backward_project<3><<<blocks, 256, 0, s>>>(sc, train_u, rec.as<float4>(), gsplat.as<float>(), gpos, grot, gls, glogit, gdc, gsh); break;
+    }
+    DL_HIP(hipGetLastError());
+  }
+  auto down = [&](float *dst, const float *src, size_t count) {
+    if (dst && count) DL_HIP(hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToHost, s));
+  };
+  down(out.positions, gpos, 3 * (size_t)n);
+  down(out.rotations, grot, 4 * (size_t)n);
+  down(out.log_scales, gls, 3 * (size_t)n);
+  down(out.opacity_logits, glogit, n);
+  down(out.colors_dc, gdc, 3 * (size_t)n);
+  down(out.colors_sh, gsh, 3 * (size_t)M * n);
+  DL_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace dl

@@ -269,6 +269,22 @@ int dl_render_scene(dl_renderer *r, const dl_scene *scene, const dl_camera *cam,
   });
 }
 
+int dl_render_forward(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, const dl_background *bg,
+                      float *out_rgb) {
+  return guard([&] {
+    if (!r || !scene || !cam) fail(DL_ERR_INVALID, "null argument");
+    if (scene->s.ctx != r->r.ctx) fail(DL_ERR_INVALID, "scene and renderer belong to different contexts");
+    r->r.forward_train(scene->s.view(), *cam, bg_or_default(bg), out_rgb);
+  });
+}
+
+int dl_render_backward(dl_renderer *r, const float *grad_rgb, dl_grads *out) {
+  return guard([&] {
+    if (!r || !grad_rgb || !out) fail(DL_ERR_INVALID, "null argument");
+    r->r.backward(grad_rgb, *out);
+  });
+}
+
 int dl_renderer_read_image(dl_renderer *r, uint8_t *out_rgba) {
   return guard([&] {
     if (!r || !out_rgba) fail(DL_ERR_INVALID, "null argument");

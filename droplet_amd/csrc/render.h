@@ -83,8 +83,19 @@ struct Renderer {
   std::vector<int> ev_stage;
   size_t ev_used = 0;
 
+  // differentiable render (backward.hip): the forward keeps one pass of full
+  // lists, the accumulator and the projected records for the backward
+  bool train_mode = false, train_valid = false;
+  SceneView train_sc{};
+  ProjUniforms train_u{};
+  dl_background train_bg{};
+  const uint32_t *train_vals = nullptr;
+  DeviceBuffer img_rgb, grad_img, gsplat, gparams;
+
   ~Renderer();
   void render(const SceneView &sc, const dl_camera &cam, const dl_background &bg);
+  void forward_train(const SceneView &sc, const dl_camera &cam, const dl_background &bg, float *out_rgb);
+  void backward(const float *grad_rgb, const dl_grads &out);
   int digit_bits() const { return opts.sort_digit_bits ? (int)std::min<uint32_t>(opts.sort_digit_bits, 8) : 8; }
   void read_image(uint8_t *out);
 

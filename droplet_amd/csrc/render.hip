@@ -353,7 +353,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   }
   stats.num_visible = nvis;
 
-  const bool need_accum = keep_taps;
+  const bool need_accum = keep_taps || train_mode;
   bool accum_valid = false;
   if (nvis == 0) {
     // pipeline.ts:210-212: background only
@@ -370,10 +370,10 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     // Front-to-back passes that stop filling closed tiles (span binning, no
     // taps). A pass costs about 30 launches, so by default they are used only
     // when the frame has enough pairs for the skipped binning to pay for them.
-    const bool early_stop = span && !keep_taps && !opts.full_lists &&
+    const bool early_stop = span && !keep_taps && !train_mode && !opts.full_lists &&
                             (opts.first_pass || opts.pass_growth || pairs_all >= kPassMinPairs);
     std::vector<uint32_t> cuts;  // pass starts, then nvis
-    if (opts.chunk_cap && opts.chunk_cap < nvis) {
+    if (opts.chunk_cap && opts.chunk_cap < nvis && !train_mode) {
       for (uint64_t c0 = 0; c0 < nvis; c0 += opts.chunk_cap) cuts.push_back((uint32_t)c0);
     } else if (early_stop) {
       uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(kPassMinFirst, nvis / kPassFirstDiv);
@@ -483,6 +483,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         tap_append(taps[DL_TAP_TILE_OFFSETS], off.data(), off.size());
       }
       rp.has_pairs = 1;
+      train_vals = svals;
       mark(DL_STAGE_RASTERIZE);
       launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
                        rgba.as<uint32_t>(), rp, d_iters);

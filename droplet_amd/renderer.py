@@ -179,6 +179,37 @@ class Renderer:
             "pairs_consumed": int(st.pairs_consumed), "list_entries": int(st.list_entries),
         }
 
+    def forward(self, scene: "Scene", camera: RenderCamera,
+                background: Optional[BackgroundRGBA] = None) -> np.ndarray:
+        """Differentiable render (dl_render_forward): the image before
+        quantisation, clamp(C + T*bg, 0, 1), as float32 (H, W, 3). Keeps the
+        state that backward() needs."""
+        out = np.empty(camera.width * camera.height * 3, np.float32)
+        cam = L.camera_struct(camera)
+        bg = L.background_struct(background)
+        L.check(L.load().dl_render_forward(self.handle, scene._h, C.byref(cam), C.byref(bg), L.ptr(out)))
+        self._last_size = (camera.width, camera.height)
+        self._train_scene = (scene.count, scene.sh_bands)
+        return out.reshape(camera.height, camera.width, 3)
+
+    def backward(self, grad_rgb: np.ndarray) -> dict:
+        """dL/d(image) of the last forward() in, dL/d(scene floats) out, in
+        the SplatData layouts (dl_render_backward)."""
+        n, bands = self._train_scene
+        w, h = self._last_size
+        g = np.ascontiguousarray(grad_rgb, dtype=np.float32).reshape(-1)
+        if g.size != w * h * 3:
+            raise ValueError("grad_rgb must hold width*height*3 floats")
+        m = {0: 0, 1: 3, 2: 8, 3: 15}[bands]
+        out = {"positions": np.empty(3 * n, np.float32), "rotations": np.empty(4 * n, np.float32),
+               "log_scales": np.empty(3 * n, np.float32), "opacity_logits": np.empty(n, np.float32),
+               "colors_dc": np.empty(3 * n, np.float32), "colors_sh": np.empty(3 * m * n, np.float32)}
+        gr = L.DlGrads()
+        for k, a in out.items():
+            setattr(gr, k, L.ptr(a) if a.size else None)
+        L.check(L.load().dl_render_backward(self.handle, L.ptr(g), C.byref(gr)))
+        return out
+
     def autotune(self, scene: "Scene", camera: RenderCamera) -> dict:
         """Times candidate options (radix digit width, pass schedule) on this
         scene and camera and keeps the fastest (dl_renderer_autotune)."""

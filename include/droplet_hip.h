@@ -172,6 +172,26 @@ int dl_renderer_set_taps(dl_renderer *r, int enabled);
 /* bytes: capacity of dst; returns DL_ERR_CAPACITY if too small */
 int dl_renderer_tap(dl_renderer *r, int which, void *dst, size_t bytes, size_t *written);
 
+/* ---- differentiable render (no reference counterpart: droplet renders
+ * only, README.md:106). dl_render_forward renders like dl_render_scene and
+ * returns the image before quantisation, out_rgb[W*H*3] = clamp(C + T*bg,
+ * 0, 1) (finalize.ts:31-35). It keeps what the backward needs: the tile
+ * lists, the projected records and the per-pixel (C, T). dl_render_backward
+ * takes dL/d(out_rgb) and writes dL/d(every float of the scene) in the
+ * dl_splats layouts. Transmittance is recomputed per tile front to back,
+ * not stored per (pixel, splat). Pointers in dl_grads may be NULL. */
+typedef struct {
+    float *positions;            /* 3N */
+    float *rotations;            /* 4N */
+    float *log_scales;           /* 3N */
+    float *opacity_logits;       /* N */
+    float *colors_dc;            /* 3N */
+    float *colors_sh;            /* 3*M*N (M = 3, 8, 15 for bands 1..3) */
+} dl_grads;
+int dl_render_forward(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, const dl_background *bg,
+                      float *out_rgb);
+int dl_render_backward(dl_renderer *r, const float *grad_rgb, dl_grads *out);
+
 /* ---- autotuner (no reference counterpart). It times candidate options on
  * the caller's own workload and keeps the fastest for later calls. The
  * candidates are the radix digit width and the front-to-back pass schedule.

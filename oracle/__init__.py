@@ -97,6 +97,17 @@ def lib():
         L.orc_morton_encode3.restype = C.c_uint32
         L.orc_morton_encode3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_morton_decode3.argtypes = [C.c_uint32, u32p]
+        L.orc_gc_create.restype = C.c_void_p
+        L.orc_gc_create.argtypes = [C.POINTER(OrcSplats), C.POINTER(OrcCamera), f32p, f32p, C.c_uint32, C.c_int]
+        L.orc_gc_free.argtypes = [C.c_void_p]
+        L.orc_gc_num_params.restype = C.c_uint32
+        L.orc_gc_num_params.argtypes = [C.c_void_p]
+        L.orc_gc_record.restype = C.c_int
+        L.orc_gc_record.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]
+        L.orc_gc_image.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.orc_gc_fd.restype = C.c_int
+        L.orc_gc_fd.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, C.c_double, C.POINTER(C.c_double),
+                                C.POINTER(C.c_int32), C.c_int]
         _lib = L
     return _lib
 
@@ -318,3 +329,64 @@ def morton_decode3(m: int):
     o = (C.c_uint32 * 3)()
     lib().orc_morton_decode3(m, o)
     return tuple(o)
+
+
+# ---------------------------------------------------------------------------
+# FP64 central differences of the differentiable render (gradcheck.c)
+# ---------------------------------------------------------------------------
+GC_NOT_VISIBLE, GC_RADIUS, GC_ORDER, GC_CULL, GC_BRANCH, GC_PROJ_KINK = 1, 2, 4, 8, 16, 32
+# parameter index ranges of one splat (then SH, channel-major)
+GC_POS, GC_ROT, GC_LOGSCALE, GC_LOGIT, GC_DC, GC_SH = 0, 3, 7, 10, 11, 14
+
+
+class GradCheck:
+    """L = sum (clamp(C + T*bg, 0, 1) - target)^2 over RGB, evaluated in f64
+    with the f32 pipeline's lists; central differences per parameter."""
+
+    def __init__(self, splats, cam, target_rgb: np.ndarray, background=None, max_coverage: int = 0,
+                 threads: Optional[int] = None):
+        self._keep = (splats, np.ascontiguousarray(target_rgb, dtype=np.float32).reshape(-1))
+        self.threads = threads or min(16, cpu_threads())
+        s = _splats(splats)
+        c = camera_struct(cam)
+        bg = None if background is None else np.array([background.r, background.g, background.b, background.a],
+                                                        np.float32)
+        self._bg = bg
+        self._h = lib().orc_gc_create(C.byref(s), C.byref(c), _p(bg, f32p), _p(self._keep[1], f32p),
+                                      max_coverage, self.threads)
+        if not self._h:
+            raise ValueError("gradient check needs a pinhole camera and shBands <= 3")
+        self.width, self.height = cam.width, cam.height
+        self.num_params = lib().orc_gc_num_params(self._h)
+
+    def image(self):
+        out = np.empty(self.width * self.height * 3, np.float64)
+        loss = C.c_double()
+        lib().orc_gc_image(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(loss), self.threads)
+        return out.reshape(self.height, self.width, 3), loss.value
+
+    def record(self, splat: int):
+        out = np.empty(10, np.float64)
+        rc = lib().orc_gc_record(self._h, splat, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return None if rc else out
+
+    def fd(self, ids, params, h: float = 1e-4):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        params = np.ascontiguousarray(params, dtype=np.uint32)
+        out = np.zeros(ids.size, np.float64)
+        flags = np.zeros(ids.size, np.int32)
+        lib().orc_gc_fd(self._h, ids.size, _p(ids, u32p), _p(params, u32p), h,
+                        out.ctypes.data_as(C.POINTER(C.c_double)), flags.ctypes.data_as(C.POINTER(C.c_int32)),
+                        self.threads)
+        return out, flags
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orc_gc_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -158,6 +158,30 @@ typedef struct {
 
 int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_result *res,
                  int num_threads);
+
+/* --- FP64 central-difference checker of the differentiable render
+ * (gradcheck.c; pinhole). The forward is evaluated in double from the f32
+ * inputs, with the f32 pipeline's lists, order and integer radii.
+ * L = sum_px sum_rgb (clamp(C + T*bg, 0, 1) - target)^2. Parameters of a
+ * splat are indexed 0-2 position, 3-6 rotation, 7-9 log-scale, 10 opacity
+ * logit, 11-13 DC, 14.. SH (channel-major, as colors_sh). orc_gc_fd
+ * returns the difference quotients plus flags: 1 not visible, 2 integer
+ * radius changes, 4 depth order changes, 8 culled, 16 on some pixel of the
+ * splat's tiles the f64 values at p+h or p-h branch differently from the
+ * f32 forward (which decides the branches of the checked function), 32 a
+ * kink of the projection (fade, eigenvalue floor, colour clamp) lies within
+ * the stencil. */
+typedef struct orc_gradcheck orc_gradcheck;
+orc_gradcheck *orc_gc_create(const orc_splats *s, const orc_camera *cam, const float *bg,
+                             const float *target_rgb, uint32_t max_coverage, int num_threads);
+void orc_gc_free(orc_gradcheck *g);
+uint32_t orc_gc_num_params(const orc_gradcheck *g);
+/* f64 record of a visible splat: sx sy A B C alpha r g b, then the radius */
+int orc_gc_record(const orc_gradcheck *g, uint32_t id, double out[10]);
+/* f64 image (W*H*3, may be NULL) and loss */
+void orc_gc_image(const orc_gradcheck *g, double *out_rgb, double *loss, int num_threads);
+int orc_gc_fd(const orc_gradcheck *g, uint32_t n, const uint32_t *ids, const uint32_t *params, double h,
+              double *out, int32_t *flags, int num_threads);
 void orc_aabb_extents(const orc_splats *s, float *out_ext);
 /* BVH DFS leaf order (right child first) of all Gaussians, written to out_perm[N] */
 int orc_bvh_order(const float *positions, const float *extents, uint32_t n, uint32_t *out_perm);

@@ -1,0 +1,77 @@
+"""CPU checks of the FP64 central-difference checker (oracle/gradcheck.c)
+that the device backward is compared against: its f64 forward equals the
+bit-exact f32 oracle to f32 precision, and its difference quotients equal a
+closed-form derivative on a single splat."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from droplet_amd import synthetic as S
+
+
+@pytest.mark.parametrize("bands", [0, 1, 3])
+def test_f64_forward_matches_f32_oracle(bands):
+    data = S.gaussian_cloud(4000, seed=50 + bands, sh_bands=bands)
+    cam = S.default_camera(160, 96)
+    ref = oracle.render(data, cam, taps=True)
+    acc = ref["accum"].reshape(96, 160, 4)
+    img32 = np.clip(acc[..., :3], 0.0, 1.0)
+    gc = oracle.GradCheck(data, cam, np.zeros((96, 160, 3), np.float32))
+    img64, loss = gc.image()
+    assert np.abs(img64 - img32).max() < 2e-5
+    assert math.isclose(loss, float((img32.astype(np.float64) ** 2).sum()), rel_tol=1e-5)
+    # records against the f32 projection (same splats, depth order)
+    proj = ref["projected"]
+    for j in range(0, ref["num_visible"], 97):
+        sid = int(ref["sorted_indices"][j])
+        rec = gc.record(sid)
+        p = proj[j]
+        if p[2] <= 0:
+            continue
+        f32 = np.array([p[0], p[1], p[4], p[5], p[6], p[7], p[8], p[9], p[10]], np.float64)
+        np.testing.assert_allclose(rec[:9], f32, rtol=2e-5, atol=2e-5)
+        assert rec[9] == p[2]
+
+
+def test_fd_matches_closed_form_on_one_splat():
+    """One splat, background 0: out_c = clamp(col_c * alpha_px); dL/d(dc_c) =
+    sum_px 2 (out_c - t_c) alpha_px where 0 < out_c < 1."""
+    data = S.one_splat((0.05, -0.03, 0.0), color=(0.7, 0.4, 0.2), log_scale=-2.2, op_logit=1.0)
+    cam = S.default_camera(96, 64)
+    tgt = np.random.default_rng(0).random((64, 96, 3)).astype(np.float32) * 0.3
+    gc = oracle.GradCheck(data, cam, tgt)
+    rec = gc.record(0)
+    sx, sy, A, B, Cc, a = rec[:6]
+    col = rec[6:9]
+    r = rec[9]
+    ys, xs = np.mgrid[0:64, 0:96]
+    dx, dy = xs + 0.5 - sx, ys + 0.5 - sy
+    power = -0.5 * (A * dx * dx + 2 * B * dx * dy + Cc * dy * dy)
+    floor = float(np.float32(1.1108996538e-2))
+    alpha = np.minimum(float(np.float32(0.999)), a * np.maximum(0.0, np.exp(power) - floor))
+    ok = (np.abs(dx) <= r) & (np.abs(dy) <= r) & (power <= 0) & (alpha >= float(np.float32(0.00392156862745098)))
+    alpha = np.where(ok, alpha, 0.0)
+    out = col[None, None, :] * alpha[..., None]
+    inside = (out > 0) & (out < 1)
+    expect = (2 * (np.clip(out, 0, 1) - tgt) * alpha[..., None] * inside).sum(axis=(0, 1))
+    g, flags = gc.fd([0, 0, 0], [oracle.GC_DC, oracle.GC_DC + 1, oracle.GC_DC + 2], h=1e-4)
+    assert (flags == 0).all()
+    np.testing.assert_allclose(g, expect, rtol=1e-7, atol=1e-10)
+    # position: a move of the splat shifts the image; the quotient is finite and flag-free
+    gp, fp = gc.fd([0, 0], [oracle.GC_POS, oracle.GC_POS + 1], h=1e-5)
+    assert (fp == 0).all() and np.isfinite(gp).all() and np.abs(gp).max() > 0
+
+
+def test_fd_flags_order_changes():
+    """Two splats at nearly the same depth: a position step that reorders them
+    is flagged, not reported as a derivative."""
+    pos = np.array([0.0, 0.0, 0.0, 0.01, 0.0, 1e-6], np.float32)
+    from droplet_amd.types import SplatData
+    data = SplatData(2, pos, np.tile(np.array([1, 0, 0, 0], np.float32), 2), np.full(6, -2.5, np.float32),
+                     np.zeros(2, np.float32), np.zeros(6, np.float32))
+    cam = S.default_camera(64, 64)
+    gc = oracle.GradCheck(data, cam, np.zeros((64, 64, 3), np.float32))
+    _, flags = gc.fd([0, 1], [2, 2], h=1e-4)
+    assert (flags & oracle.GC_ORDER).any()
