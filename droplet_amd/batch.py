@@ -9,6 +9,7 @@ with new cameras."""
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -96,6 +97,10 @@ class BatchRenderer:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be going away: the
+        # process releases the device memory by itself
+        if sys.is_finalizing():
+            return
         try:
             self.destroy()
         except Exception:

@@ -189,14 +189,14 @@ __global__ __launch_bounds__(64) void k_backward_raster(const float4 *__restrict
 
 // d basis / d (x, y, z) of the SH terms of bands 1..3 in eval_sh's order
 // (coefficient k of a channel), accumulated as gd += w_k * grad b_k.
-template <int BANDS>
-__device__ inline void sh_dir_grad(const float *w, float x, float y, float z, float gd[3]) {
-  const float SH1 = (float)0.4886025119029199;
+template <int BANDS, class R>
+__device__ inline void sh_dir_grad(const R *w, R x, R y, R z, R gd[3]) {
+  const R SH1 = (R)(float)0.4886025119029199;
   gd[1] += w[0] * (-SH1);
   gd[2] += w[1] * SH1;
   gd[0] += w[2] * (-SH1);
   if constexpr (BANDS >= 2) {
-    const float K0 = (float)1.0925484305920792, K2 = (float)0.31539156525252005, K4 = (float)0.5462742152960396;
+    const R K0 = (R)(float)1.0925484305920792, K2 = (R)(float)0.31539156525252005, K4 = (R)(float)0.5462742152960396;
     gd[0] += w[3] * (K0 * y);
     gd[1] += w[3] * (K0 * x);
     gd[1] += w[4] * (-K0 * z);
@@ -209,9 +209,9 @@ __device__ inline void sh_dir_grad(const float *w, float x, float y, float z, fl
     gd[0] += w[7] * (2.0f * K4 * x);
     gd[1] += w[7] * (-2.0f * K4 * y);
     if constexpr (BANDS >= 3) {
-      const float Q0 = (float)-0.5900435899266435, Q1 = (float)2.890611442640554;
-      const float Q2 = (float)-0.4570457994644658, Q3 = (float)0.3731763325901154, Q5 = (float)1.445305721320277;
-      const float xx = x * x, yy = y * y, zz = z * z;
+      const R Q0 = (R)(float)-0.5900435899266435, Q1 = (R)(float)2.890611442640554;
+      const R Q2 = (R)(float)-0.4570457994644658, Q3 = (R)(float)0.3731763325901154, Q5 = (R)(float)1.445305721320277;
+      const R xx = x * x, yy = y * y, zz = z * z;
       gd[0] += w[8] * (Q0 * 6.0f * x * y);
       gd[1] += w[8] * (Q0 * (3.0f * xx - 3.0f * yy));
       gd[0] += w[9] * (Q1 * y * z);
@@ -236,47 +236,62 @@ __device__ inline void sh_dir_grad(const float *w, float x, float y, float z, fl
 }
 
 // SH basis values in eval_sh's coefficient order
-template <int BANDS>
-__device__ inline void sh_basis(float x, float y, float z, float *bv) {
-  const float SH1 = (float)0.4886025119029199;
+template <int BANDS, class R>
+__device__ inline void sh_basis(R x, R y, R z, R *bv) {
+  const R SH1 = (R)(float)0.4886025119029199;
   bv[0] = (-SH1) * y;
   bv[1] = SH1 * z;
   bv[2] = (-SH1) * x;
   if constexpr (BANDS >= 2) {
-    const float xx = x * x, yy = y * y, zz = z * z;
-    bv[3] = (float)1.0925484305920792 * (x * y);
-    bv[4] = (float)-1.0925484305920792 * (y * z);
-    bv[5] = (float)0.31539156525252005 * ((2.0f * zz - xx) - yy);
-    bv[6] = (float)-1.0925484305920792 * (x * z);
-    bv[7] = (float)0.5462742152960396 * (xx - yy);
+    const R xx = x * x, yy = y * y, zz = z * z;
+    bv[3] = (R)(float)1.0925484305920792 * (x * y);
+    bv[4] = (R)(float)-1.0925484305920792 * (y * z);
+    bv[5] = (R)(float)0.31539156525252005 * ((2 * zz - xx) - yy);
+    bv[6] = (R)(float)-1.0925484305920792 * (x * z);
+    bv[7] = (R)(float)0.5462742152960396 * (xx - yy);
     if constexpr (BANDS >= 3) {
-      bv[8] = ((float)-0.5900435899266435 * y) * (3.0f * xx - yy);
-      bv[9] = ((float)2.890611442640554 * (x * y)) * z;
-      bv[10] = ((float)-0.4570457994644658 * y) * ((4.0f * zz - xx) - yy);
-      bv[11] = ((float)0.3731763325901154 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
-      bv[12] = ((float)-0.4570457994644658 * x) * ((4.0f * zz - xx) - yy);
-      bv[13] = ((float)1.445305721320277 * z) * (xx - yy);
-      bv[14] = ((float)-0.5900435899266435 * x) * (xx - 3.0f * yy);
+      bv[8] = ((R)(float)-0.5900435899266435 * y) * (3 * xx - yy);
+      bv[9] = ((R)(float)2.890611442640554 * (x * y)) * z;
+      bv[10] = ((R)(float)-0.4570457994644658 * y) * ((4 * zz - xx) - yy);
+      bv[11] = ((R)(float)0.3731763325901154 * z) * ((2 * zz - 3 * xx) - 3 * yy);
+      bv[12] = ((R)(float)-0.4570457994644658 * x) * ((4 * zz - xx) - yy);
+      bv[13] = ((R)(float)1.445305721320277 * z) * (xx - yy);
+      bv[14] = ((R)(float)-0.5900435899266435 * x) * (xx - 3 * yy);
     }
   }
 }
 
 // Gradient of o = quat_rot(q, v) (project.ts: o = v + w t + qv x t, t = 2 qv
 // x v, for any q, normalised or not) with respect to q, given go = dL/do.
-__device__ inline void quat_rot_grad(const float q[4], const float v[3], const float go[3], float gq[4]) {
-  const float qv[3] = {q[1], q[2], q[3]};
-  const float cx[3] = {qv[1] * v[2] - qv[2] * v[1], qv[2] * v[0] - qv[0] * v[2], qv[0] * v[1] - qv[1] * v[0]};
+template <class R>
+__device__ inline void quat_rot_grad(const R q[4], const R v[3], const R go[3], R gq[4]) {
+  const R qv[3] = {q[1], q[2], q[3]};
+  const R cx[3] = {qv[1] * v[2] - qv[2] * v[1], qv[2] * v[0] - qv[0] * v[2], qv[0] * v[1] - qv[1] * v[0]};
   // o = v + 2 w (qv x v) + 2 qv (qv . v) - 2 v |qv|^2
-  gq[0] += 2.0f * (go[0] * cx[0] + go[1] * cx[1] + go[2] * cx[2]);
-  const float vxg[3] = {v[1] * go[2] - v[2] * go[1], v[2] * go[0] - v[0] * go[2], v[0] * go[1] - v[1] * go[0]};
-  const float qdv = qv[0] * v[0] + qv[1] * v[1] + qv[2] * v[2];
-  const float gdq = go[0] * qv[0] + go[1] * qv[1] + go[2] * qv[2];
-  const float gdv = go[0] * v[0] + go[1] * v[1] + go[2] * v[2];
+  gq[0] += 2 * (go[0] * cx[0] + go[1] * cx[1] + go[2] * cx[2]);
+  const R vxg[3] = {v[1] * go[2] - v[2] * go[1], v[2] * go[0] - v[0] * go[2], v[0] * go[1] - v[1] * go[0]};
+  const R qdv = qv[0] * v[0] + qv[1] * v[1] + qv[2] * v[2];
+  const R gdq = go[0] * qv[0] + go[1] * qv[1] + go[2] * qv[2];
+  const R gdv = go[0] * v[0] + go[1] * v[1] + go[2] * v[2];
 #pragma unroll
-  for (int k = 0; k < 3; k++)
-    gq[1 + k] += 2.0f * q[0] * vxg[k] + 2.0f * (go[k] * qdv + v[k] * gdq) - 4.0f * gdv * qv[k];
+  for (int k = 0; k < 3; k++) gq[1 + k] += 2 * q[0] * vxg[k] + 2 * (go[k] * qdv + v[k] * gdq) - 4 * gdv * qv[k];
 }
 
+// quat_rot (project.ts:101-104) for any real type
+template <class R>
+__device__ inline void quat_rot_r(const R q[4], const R v[3], R o[3]) {
+  const R c0 = q[2] * v[2] - q[3] * v[1], c1 = q[3] * v[0] - q[1] * v[2], c2 = q[1] * v[1] - q[2] * v[0];
+  const R t0 = 2 * c0, t1 = 2 * c1, t2 = 2 * c2;
+  o[0] = (v[0] + q[0] * t0) + (q[2] * t2 - q[3] * t1);
+  o[1] = (v[1] + q[0] * t1) + (q[3] * t0 - q[1] * t2);
+  o[2] = (v[2] + q[0] * t2) + (q[1] * t1 - q[2] * t0);
+}
+
+// The chain rule of project.ts:111-285 per splat. The forward is f32, but
+// near-singular projected covariances make f32 cancellation (det_orig =
+// cov00 cov11 - cov01^2, the antialiasing factor) matter at the 1e-3 level.
+// So the derivative of the projection is taken in f64, from the same f32
+// inputs.
 template <int BANDS>
 __global__ __launch_bounds__(256) void k_backward_project(SceneView sc, ProjUniforms u,
                                                           const float4 *__restrict__ rec,
@@ -284,236 +299,237 @@ __global__ __launch_bounds__(256) void k_backward_project(SceneView sc, ProjUnif
                                                           float *__restrict__ gpos, float *__restrict__ grot,
                                                           float *__restrict__ gls, float *__restrict__ glogit,
                                                           float *__restrict__ gdc, float *__restrict__ gsh) {
+  using R = double;
   constexpr int M = BANDS == 0 ? 0 : BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
   const uint32_t id = blockIdx.x * 256 + threadIdx.x;
   if (id >= sc.count) return;
-  float gp[3] = {0, 0, 0}, gq[4] = {0, 0, 0, 0}, gl[3] = {0, 0, 0}, go = 0.0f, gd[3] = {0, 0, 0};
-  float gshv[3 * (M > 0 ? M : 1)];
+  R gp[3] = {0, 0, 0}, gq[4] = {0, 0, 0, 0}, gl[3] = {0, 0, 0}, go = 0, gd[3] = {0, 0, 0};
+  R gshv[3 * (M > 0 ? M : 1)];
 #pragma unroll
-  for (int k = 0; k < 3 * (M > 0 ? M : 1); k++) gshv[k] = 0.0f;
+  for (int k = 0; k < 3 * (M > 0 ? M : 1); k++) gshv[k] = 0;
   const float *g9 = gsplat + (size_t)kSplatGrads * id;
-  float g[kSplatGrads];
+  R g[kSplatGrads];
   bool any = false;
 #pragma unroll
   for (int q = 0; q < kSplatGrads; q++) {
     g[q] = g9[q];
-    any = any || g[q] != 0.0f;
+    any = any || g9[q] != 0.0f;
   }
   const float4 r0 = rec[3 * (size_t)id];
   if (any && r0.z > 0.0f) {
-    // ---- recompute the projection (project_splat) ----
+    // ---- recompute the projection ----
     const float *m = u.view;
-    const float px = sc.pos[3 * (size_t)id], py = sc.pos[3 * (size_t)id + 1], pz = sc.pos[3 * (size_t)id + 2];
+    const R p3[3] = {sc.pos[3 * (size_t)id], sc.pos[3 * (size_t)id + 1], sc.pos[3 * (size_t)id + 2]};
     const float4 qv4 = reinterpret_cast<const float4 *>(sc.rot)[id];
-    const float q[4] = {qv4.x, qv4.y, qv4.z, qv4.w};
-    const float s[3] = {dl_expf(sc.ls[3 * (size_t)id]), dl_expf(sc.ls[3 * (size_t)id + 1]),
-                        dl_expf(sc.ls[3 * (size_t)id + 2])};
-    const float logit = sc.op[id];
-    const float pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
-    const float pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
-    const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
-    const float f = u.focal;
-    float R[3][3];  // columns: quat_rot(q, e_j)
+    const R q[4] = {qv4.x, qv4.y, qv4.z, qv4.w};
+    const R s[3] = {exp((R)sc.ls[3 * (size_t)id]), exp((R)sc.ls[3 * (size_t)id + 1]),
+                    exp((R)sc.ls[3 * (size_t)id + 2])};
+    const R logit = sc.op[id];
+    const R W[3][3] = {{m[0], m[4], m[8]}, {m[1], m[5], m[9]}, {m[2], m[6], m[10]}};
+    const R pcx = W[0][0] * p3[0] + W[0][1] * p3[1] + W[0][2] * p3[2] + (R)m[12];
+    const R pcy = W[1][0] * p3[0] + W[1][1] * p3[1] + W[1][2] * p3[2] + (R)m[13];
+    const R pcz = W[2][0] * p3[0] + W[2][1] * p3[1] + W[2][2] * p3[2] + (R)m[14];
+    const R f = u.focal;
+    R Rm[3][3];  // columns: quat_rot(q, e_j)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      float e[3] = {0, 0, 0};
-      e[j] = 1.0f;
-      float o[3];
-      quat_rot(q, e[0], e[1], e[2], o);
-      R[0][j] = o[0];
-      R[1][j] = o[1];
-      R[2][j] = o[2];
+      R e[3] = {0, 0, 0};
+      e[j] = 1;
+      R o[3];
+      quat_rot_r(q, e, o);
+      Rm[0][j] = o[0];
+      Rm[1][j] = o[1];
+      Rm[2][j] = o[2];
     }
-    float Mx[3][3];  // M = R diag(s)
+    R Mx[3][3];  // M = R diag(s)
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int j = 0; j < 3; j++) Mx[a][j] = R[a][j] * s[j];
-    float C3[3][3];
+      for (int j = 0; j < 3; j++) Mx[a][j] = Rm[a][j] * s[j];
+    R C3[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int b = 0; b < 3; b++) C3[a][b] = (Mx[a][0] * Mx[b][0] + Mx[a][1] * Mx[b][1]) + Mx[a][2] * Mx[b][2];
-    const float W[3][3] = {{m[0], m[4], m[8]}, {m[1], m[5], m[9]}, {m[2], m[6], m[10]}};
-    float V[3][3];  // W C3 W^T
+      for (int b = 0; b < 3; b++) C3[a][b] = Mx[a][0] * Mx[b][0] + Mx[a][1] * Mx[b][1] + Mx[a][2] * Mx[b][2];
+    R V[3][3];  // W C3 W^T
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
-        float acc = 0.0f;
+        R acc = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++)
 #pragma unroll
           for (int l = 0; l < 3; l++) acc += W[a][k] * C3[k][l] * W[b][l];
         V[a][b] = acc;
       }
-    const float iz = 1.0f / pcz;
-    const float j00 = f * iz, j02 = ((-f) * pcx) * iz * iz, j11 = f * iz, j12 = ((-f) * pcy) * iz * iz;
-    const float cov00 = j00 * j00 * V[0][0] + 2.0f * j00 * j02 * V[0][2] + j02 * j02 * V[2][2];
-    const float cov01 = j00 * j11 * V[0][1] + j00 * j12 * V[0][2] + j02 * j11 * V[1][2] + j02 * j12 * V[2][2];
-    const float cov11 = j11 * j11 * V[1][1] + 2.0f * j11 * j12 * V[1][2] + j12 * j12 * V[2][2];
-    const float det_orig = cov00 * cov11 - cov01 * cov01;
-    const float d00 = cov00 + kAA, d11 = cov11 + kAA;
-    const float det = d00 * d11 - cov01 * cov01;
-    const float ratio = det_orig / det;
-    const float aa = __builtin_sqrtf(wmax(ratio, 0.0f));
-    const float mid = 0.5f * (d00 + d11);
-    const float disc = mid * mid - det;
-    const float r_eig = __builtin_sqrtf(wmax(kEigMin, disc));
-    const float lambda1 = mid + r_eig;
-    const float radius_raw = kSigma * __builtin_sqrtf(lambda1);
-    const float fade_s = kFadeStart * u.height, fade_e = kFadeEnd * u.height;
-    const float fade_arg = (fade_e - radius_raw) / (fade_e - fade_s);
-    const float fade = wclamp(fade_arg, 0.0f, 1.0f);
-    const float sig = dl_sigmoidf(logit);
+    const R iz = 1 / pcz;
+    const R j00 = f * iz, j02 = -f * pcx * iz * iz, j11 = f * iz, j12 = -f * pcy * iz * iz;
+    const R cov00 = j00 * j00 * V[0][0] + 2 * j00 * j02 * V[0][2] + j02 * j02 * V[2][2];
+    const R cov01 = j00 * j11 * V[0][1] + j00 * j12 * V[0][2] + j02 * j11 * V[1][2] + j02 * j12 * V[2][2];
+    const R cov11 = j11 * j11 * V[1][1] + 2 * j11 * j12 * V[1][2] + j12 * j12 * V[2][2];
+    const R det_orig = cov00 * cov11 - cov01 * cov01;
+    const R kaa = (R)kAA;
+    const R d00 = cov00 + kaa, d11 = cov11 + kaa;
+    const R det = d00 * d11 - cov01 * cov01;
+    const R ratio = det_orig / det;
+    const R aa = ratio > 0 ? sqrt(ratio) : 0;
+    const R mid = 0.5 * (d00 + d11);
+    const R disc = mid * mid - det;
+    const R r_eig = sqrt(disc > (R)kEigMin ? disc : (R)kEigMin);
+    const R lambda1 = mid + r_eig;
+    const R radius_raw = (R)kSigma * sqrt(lambda1);
+    const R fade_s = (R)kFadeStart * (R)u.height, fade_e = (R)kFadeEnd * (R)u.height;
+    const R fade_arg = (fade_e - radius_raw) / (fade_e - fade_s);
+    const R fade = fade_arg < 0 ? 0 : fade_arg > 1 ? 1 : fade_arg;
+    const R sig = 1 / (1 + exp(-logit));
     // ---- colour ----
-    float len = 1.0f, dir[3] = {0, 0, 0};
-    float bv[M > 0 ? M : 1];
-    float colpre[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
+    R len = 1, dir[3] = {0, 0, 0};
+    R bv[M > 0 ? M : 1];
+    R colpre[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
     if constexpr (BANDS > 0) {
-      len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+      len = sqrt(pcx * pcx + pcy * pcy + pcz * pcz);
       dir[0] = pcx / len;
       dir[1] = pcy / len;
       dir[2] = pcz / len;
-      sh_basis<BANDS>(dir[0], dir[1], dir[2], bv);
+      sh_basis<BANDS, R>(dir[0], dir[1], dir[2], bv);
       const float *shc = sc.sh + (size_t)id * 3 * M;
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int k = 0; k < M; k++) colpre[c] = colpre[c] + bv[k] * shc[c * M + k];
+        for (int k = 0; k < M; k++) colpre[c] += bv[k] * (R)shc[c * M + k];
     }
     // ---- backward ----
-    float gpc[3] = {0, 0, 0};
-    float gdir[3] = {0, 0, 0};
+    R gpc[3] = {0, 0, 0};
+    R gdir[3] = {0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const float gc = (colpre[c] + 0.5f > 0.0f) ? g[6 + c] : 0.0f;  // max(col + 0.5, 0)
+      const R gc = (colpre[c] + 0.5 > 0) ? g[6 + c] : 0;  // max(col + 0.5, 0)
       gd[c] = gc;
       if constexpr (BANDS > 0) {
         const float *shc = sc.sh + (size_t)id * 3 * M;
-        float w[M];
+        R w[M];
 #pragma unroll
         for (int k = 0; k < M; k++) {
           gshv[c * M + k] = gc * bv[k];
-          w[k] = gc * shc[c * M + k];
+          w[k] = gc * (R)shc[c * M + k];
         }
-        sh_dir_grad<BANDS>(w, dir[0], dir[1], dir[2], gdir);
+        sh_dir_grad<BANDS, R>(w, dir[0], dir[1], dir[2], gdir);
       }
     }
     if constexpr (BANDS > 0) {
       // dir = pc / |pc|
-      const float dd = gdir[0] * dir[0] + gdir[1] * dir[1] + gdir[2] * dir[2];
+      const R dd = gdir[0] * dir[0] + gdir[1] * dir[1] + gdir[2] * dir[2];
 #pragma unroll
       for (int k = 0; k < 3; k++) gpc[k] += (gdir[k] - dd * dir[k]) / len;
     }
     // opacity a = (sigmoid(logit) * fade) * aa
-    const float ga = g[5];
-    go = ga * fade * aa * sig * (1.0f - sig);
-    const float gaa = ga * sig * fade;
-    const float gfade = ga * sig * aa;
-    float glambda = 0.0f;
-    if (fade_arg > 0.0f && fade_arg < 1.0f) {
-      const float grr = gfade * (-1.0f / (fade_e - fade_s));
-      glambda += grr * kSigma * 0.5f / __builtin_sqrtf(lambda1);
+    const R ga = g[5];
+    go = ga * fade * aa * sig * (1 - sig);
+    const R gaa = ga * sig * fade;
+    const R gfade = ga * sig * aa;
+    R glambda = 0;
+    if (fade_arg > 0 && fade_arg < 1) {
+      const R grr = gfade * (-1 / (fade_e - fade_s));
+      glambda += grr * (R)kSigma * 0.5 / sqrt(lambda1);
     }
     // conic A = d11 / det, B = -cov01 / det, C = d00 / det
-    float gd00 = g[4] / det, gd11 = g[2] / det, gc01 = -g[3] / det;
-    float gdet = -(g[2] * d11 - g[3] * cov01 + g[4] * d00) / (det * det);
-    float gdo = 0.0f;
-    if (ratio > 0.0f) {
-      const float gr = gaa * 0.5f / aa;  // d sqrt(ratio)
+    R gd00 = g[4] / det, gd11 = g[2] / det, gc01 = -g[3] / det;
+    R gdet = -(g[2] * d11 - g[3] * cov01 + g[4] * d00) / (det * det);
+    R gdo = 0;
+    if (ratio > 0) {
+      const R gr = gaa * 0.5 / aa;  // d sqrt(ratio)
       gdo += gr / det;
       gdet += -gr * det_orig / (det * det);
     }
     // lambda1 = mid + sqrt(max(kEigMin, mid^2 - det))
-    float gmid = glambda;
-    if (disc > kEigMin) {
+    R gmid = glambda;
+    if (disc > (R)kEigMin) {
       gmid += glambda * mid / r_eig;
-      gdet += -glambda * 0.5f / r_eig;
+      gdet += -glambda * 0.5 / r_eig;
     }
-    gd00 += 0.5f * gmid;
-    gd11 += 0.5f * gmid;
+    gd00 += 0.5 * gmid;
+    gd11 += 0.5 * gmid;
     // det = d00 d11 - cov01^2 ; det_orig = cov00 cov11 - cov01^2
     gd00 += gdet * d11;
     gd11 += gdet * d00;
-    gc01 += -2.0f * gdet * cov01;
-    float gc00 = gd00 + gdo * cov11;
-    float gc11 = gd11 + gdo * cov00;
-    gc01 += -2.0f * gdo * cov01;
-    // cov2 = J V J^T (unique entries cov00, cov01, cov11)
-    float gV[3][3];
+    gc01 += -2 * gdet * cov01;
+    const R gc00 = gd00 + gdo * cov11;
+    const R gc11 = gd11 + gdo * cov00;
+    gc01 += -2 * gdo * cov01;
+    // cov2 = J V J^T over the unique entries cov00, cov01, cov11
+    R gV[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     gV[0][0] = gc00 * j00 * j00;
     gV[1][1] = gc11 * j11 * j11;
     gV[2][2] = gc00 * j02 * j02 + gc01 * j02 * j12 + gc11 * j12 * j12;
-    gV[0][1] = gc01 * j00 * j11;                                  // unique-entry gradients (off-diagonal
-    gV[0][2] = 2.0f * gc00 * j00 * j02 + gc01 * j00 * j12;        // entries once)
-    gV[1][2] = gc01 * j02 * j11 + 2.0f * gc11 * j11 * j12;
-    const float gj00 = gc00 * (2.0f * j00 * V[0][0] + 2.0f * j02 * V[0][2]) + gc01 * (j11 * V[0][1] + j12 * V[0][2]);
-    const float gj02 = gc00 * (2.0f * j00 * V[0][2] + 2.0f * j02 * V[2][2]) + gc01 * (j11 * V[1][2] + j12 * V[2][2]);
-    const float gj11 = gc01 * (j00 * V[0][1] + j02 * V[1][2]) + gc11 * (2.0f * j11 * V[1][1] + 2.0f * j12 * V[1][2]);
-    const float gj12 = gc01 * (j00 * V[0][2] + j02 * V[2][2]) + gc11 * (2.0f * j11 * V[1][2] + 2.0f * j12 * V[2][2]);
+    gV[0][1] = gc01 * j00 * j11;                            // unique-entry gradients
+    gV[0][2] = 2 * gc00 * j00 * j02 + gc01 * j00 * j12;     // (each off-diagonal once)
+    gV[1][2] = gc01 * j02 * j11 + 2 * gc11 * j11 * j12;
+    const R gj00 = gc00 * (2 * j00 * V[0][0] + 2 * j02 * V[0][2]) + gc01 * (j11 * V[0][1] + j12 * V[0][2]);
+    const R gj02 = gc00 * (2 * j00 * V[0][2] + 2 * j02 * V[2][2]) + gc01 * (j11 * V[1][2] + j12 * V[2][2]);
+    const R gj11 = gc01 * (j00 * V[0][1] + j02 * V[1][2]) + gc11 * (2 * j11 * V[1][1] + 2 * j12 * V[1][2]);
+    const R gj12 = gc01 * (j00 * V[0][2] + j02 * V[2][2]) + gc11 * (2 * j11 * V[1][2] + 2 * j12 * V[2][2]);
     // J(pc): j00 = j11 = f/z, j02 = -f x / z^2, j12 = -f y / z^2
     gpc[0] += gj02 * (-f * iz * iz);
     gpc[1] += gj12 * (-f * iz * iz);
-    gpc[2] += (gj00 + gj11) * (-f * iz * iz) + gj02 * (2.0f * f * pcx * iz * iz * iz) +
-              gj12 * (2.0f * f * pcy * iz * iz * iz);
+    gpc[2] += (gj00 + gj11) * (-f * iz * iz) + gj02 * (2 * f * pcx * iz * iz * iz) +
+              gj12 * (2 * f * pcy * iz * iz * iz);
     // screen position: sx = -f x / z + cx, sy = f y / z + cy
     gpc[0] += g[0] * (-f * iz);
     gpc[2] += g[0] * (f * pcx * iz * iz);
     gpc[1] += g[1] * (f * iz);
     gpc[2] += g[1] * (-f * pcy * iz * iz);
     // V = W C3 W^T: symmetric gradient Gh (off-diagonal halves), dC3 = W^T Gh W
-    float Gh[3][3];
+    R Gh[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
         const int lo = a < b ? a : b, hi = a < b ? b : a;
-        Gh[a][b] = (a == b) ? gV[a][a] : 0.5f * gV[lo][hi];
+        Gh[a][b] = (a == b) ? gV[a][a] : 0.5 * gV[lo][hi];
       }
-    float GC[3][3];  // dL/dC3 as a full symmetric matrix
+    R GC[3][3];  // dL/dC3 as a full symmetric matrix
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
       for (int l = 0; l < 3; l++) {
-        float acc = 0.0f;
+        R acc = 0;
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
           for (int b = 0; b < 3; b++) acc += W[a][k] * Gh[a][b] * W[b][l];
         GC[k][l] = acc;
       }
-    // C3 = M M^T: dL/dM = 2 GC M
-    float gs[3] = {0, 0, 0};
+    // C3 = M M^T: dL/dM = 2 GC M ; M[:, j] = R e_j * s_j
+    R gs[3] = {0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      float gm[3];
+      R gm[3];
 #pragma unroll
-      for (int a = 0; a < 3; a++) gm[a] = 2.0f * ((GC[a][0] * Mx[0][j] + GC[a][1] * Mx[1][j]) + GC[a][2] * Mx[2][j]);
-      // M[:, j] = R e_j * s_j
-      gs[j] = gm[0] * R[0][j] + gm[1] * R[1][j] + gm[2] * R[2][j];
-      const float gr[3] = {gm[0] * s[j], gm[1] * s[j], gm[2] * s[j]};
-      float e[3] = {0, 0, 0};
-      e[j] = 1.0f;
-      quat_rot_grad(q, e, gr, gq);
+      for (int a = 0; a < 3; a++) gm[a] = 2 * (GC[a][0] * Mx[0][j] + GC[a][1] * Mx[1][j] + GC[a][2] * Mx[2][j]);
+      gs[j] = gm[0] * Rm[0][j] + gm[1] * Rm[1][j] + gm[2] * Rm[2][j];
+      const R gr[3] = {gm[0] * s[j], gm[1] * s[j], gm[2] * s[j]};
+      R e[3] = {0, 0, 0};
+      e[j] = 1;
+      quat_rot_grad<R>(q, e, gr, gq);
     }
 #pragma unroll
     for (int j = 0; j < 3; j++) gl[j] = gs[j] * s[j];
     // pc = W p + t
 #pragma unroll
-    for (int k = 0; k < 3; k++) gp[k] = (W[0][k] * gpc[0] + W[1][k] * gpc[1]) + W[2][k] * gpc[2];
+    for (int k = 0; k < 3; k++) gp[k] = W[0][k] * gpc[0] + W[1][k] * gpc[1] + W[2][k] * gpc[2];
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    gpos[3 * (size_t)id + k] = gp[k];
-    gls[3 * (size_t)id + k] = gl[k];
-    gdc[3 * (size_t)id + k] = gd[k];
+    gpos[3 * (size_t)id + k] = (float)gp[k];
+    gls[3 * (size_t)id + k] = (float)gl[k];
+    gdc[3 * (size_t)id + k] = (float)gd[k];
   }
-  reinterpret_cast<float4 *>(grot)[id] = make_float4(gq[0], gq[1], gq[2], gq[3]);
-  glogit[id] = go;
+  reinterpret_cast<float4 *>(grot)[id] = make_float4((float)gq[0], (float)gq[1], (float)gq[2], (float)gq[3]);
+  glogit[id] = (float)go;
   if constexpr (M > 0) {
 #pragma unroll
-    for (int k = 0; k < 3 * M; k++) gsh[(size_t)id * 3 * M + k] = gshv[k];
+    for (int k = 0; k < 3 * M; k++) gsh[(size_t)id * 3 * M + k] = (float)gshv[k];
   }
 }
 

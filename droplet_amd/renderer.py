@@ -8,6 +8,7 @@ libdroplet_hip.so on the GPU.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass
 from typing import Optional
 
@@ -67,6 +68,10 @@ class Device:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be going away: the
+        # process releases the device memory by itself
+        if sys.is_finalizing():
+            return
         try:
             self.destroy()
         except Exception:
@@ -96,6 +101,10 @@ class Scene:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be going away: the
+        # process releases the device memory by itself
+        if sys.is_finalizing():
+            return
         try:
             self.destroy()
         except Exception:
@@ -234,6 +243,10 @@ class Renderer:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be going away: the
+        # process releases the device memory by itself
+        if sys.is_finalizing():
+            return
         try:
             self.destroy()
         except Exception:

@@ -105,6 +105,8 @@ def lib():
         L.orc_gc_record.restype = C.c_int
         L.orc_gc_record.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]
         L.orc_gc_image.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.orc_gc_fd_record.restype = C.c_int
+        L.orc_gc_fd_record.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double, C.POINTER(C.c_double)]
         L.orc_gc_fd.restype = C.c_int
         L.orc_gc_fd.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, C.c_double, C.POINTER(C.c_double),
                                 C.POINTER(C.c_int32), C.c_int]
@@ -379,6 +381,11 @@ class GradCheck:
                         out.ctypes.data_as(C.POINTER(C.c_double)), flags.ctypes.data_as(C.POINTER(C.c_int32)),
                         self.threads)
         return out, flags
+
+    def fd_record(self, splat: int, field: int, h: float = 1e-6):
+        out = C.c_double()
+        flags = lib().orc_gc_fd_record(self._h, splat, field, h, C.byref(out))
+        return out.value, flags
 
     def close(self):
         if getattr(self, "_h", None):

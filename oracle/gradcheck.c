@@ -17,10 +17,19 @@
  *   - a visibility change;
  *   - a piecewise part of the projection (fade, eigenvalue floor, colour
  *     clamp) whose side differs between p+h and p-h.
+ *   - a per-pixel branch that depends on the perturbed splat (its own test,
+ *     the early stop, an output clamp) and whose f64 outcome differs
+ *     between p+h and p-h.
  * The f32 forward runs in lockstep with the f64 values and decides every
- * per-pixel branch (pixel_loss_forced), so both differentiate the same
- * function. A parameter is also flagged when, on any pixel of its tiles,
- * the f64 values would branch differently from the f32 forward.
+ * per-pixel branch (pixel_loss_forced). The other splats of a pixel enter
+ * with their f32 records, exactly as the forward computed them: they are
+ * constants of the function being differenced. The perturbed splat enters
+ * as its f32 record plus the exact f64 change of its projection at p +- h.
+ * So the quotients differentiate the function the device evaluates: the
+ * projection's derivative in f64, and the compositing around the f32
+ * values. (For very elongated splats the f32 conic is ill-conditioned at
+ * pixels far along the long axis, and exp(power) there can differ from
+ * the exact-math value by about 1%.)
  */
 #include <math.h>
 #include <stdlib.h>
@@ -344,21 +353,21 @@ void orc_gc_image(const orc_gradcheck *g, double *out_rgb, double *loss, int nth
  * runs in lockstep and decides every branch: which entries composite, where
  * alpha saturates at the cap, where the loop stops and which output channels
  * clamp. The f64 values follow those decisions, giving the smooth function
- * whose derivative the device backward computes. *cross is set when the f64
- * values themselves would branch differently (a threshold lies within the
- * difference stencil or within rounding). */
+ * whose derivative the device backward computes. *sig hashes what the f64
+ * values alone would decide where they depend on the perturbed splat (its
+ * own branch, the early stop, the output clamps): a difference between p+h
+ * and p-h means a threshold lies inside the stencil. */
 static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, uint32_t t, uint32_t sub_rank,
-                                const double *sub, double sub_radius, int *cross) {
+                                const double *sub, double sub_radius, uint64_t *sig) {
     float T32 = 1.0f, C32[3] = {0, 0, 0};
     double T = 1.0, C[3] = {0, 0, 0};
     const float px32 = (float)x + 0.5f, py32 = (float)y + 0.5f;
     const double px = x + 0.5, py = y + 0.5;
-    for (uint32_t e = g->toff[t]; e < g->toff[t + 1]; e++) {
-        if (T32 < (float)kMinT_) {
-            if (!(T < kMinT_)) *cross = 1;
-            break;
-        }
-        if (T < kMinT_) *cross = 1;
+    uint64_t h = 1469598103934665603ull;
+    uint32_t step = 0;
+    for (uint32_t e = g->toff[t]; e < g->toff[t + 1]; e++, step++) {
+        if (T < kMinT_) h = (h ^ (uint64_t)(0x100000u + step)) * 1099511628211ull;
+        if (T32 < (float)kMinT_) break;
         uint32_t k = g->tlist[e];
         const float *r = g->proj32 + 12 * (size_t)k;
         int code = 1;
@@ -376,8 +385,17 @@ static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, 
                 }
             }
         }
-        const double *q = k == sub_rank ? sub : g->rec + 9 * (size_t)k;
-        double a64 = 0.0;
+        if (code < 4 && k != sub_rank) continue;
+        /* other splats are constants of the checked function: their f32
+         * records exactly as the forward computed them; only the perturbed
+         * splat's record is recomputed (in f64) from its parameters */
+        double other[9];
+        if (k != sub_rank) {
+            other[0] = r[0]; other[1] = r[1]; other[2] = r[4]; other[3] = r[5]; other[4] = r[6];
+            other[5] = r[7]; other[6] = r[8]; other[7] = r[9]; other[8] = r[10];
+        }
+        const double *q = k == sub_rank ? sub : other;
+        double raw64 = 0.0;
         int code64 = 1;
         {
             double rad = k == sub_rank ? sub_radius : g->radius[k];
@@ -387,18 +405,18 @@ static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, 
                 if (power > 0) code64 = 2;
                 else {
                     double ge = exp(power) - kFloor;
-                    double raw = q[5] * (ge > 0 ? ge : 0);
-                    a64 = raw < kCap ? raw : kCap;
-                    code64 = a64 < kMinA ? 3 : (raw < kCap ? 4 : 5);
+                    raw64 = q[5] * (ge > 0 ? ge : 0);
+                    double a64 = raw64 < kCap ? raw64 : kCap;
+                    code64 = a64 < kMinA ? 3 : (raw64 < kCap ? 4 : 5);
                 }
             }
         }
-        const int comp32 = code >= 4, comp64 = code64 >= 4;
-        if (comp32 != comp64 || (comp32 && code != code64)) *cross = 1;
-        if (comp32) {
+        if (k == sub_rank) h = (h ^ (uint64_t)(code64 + 8 * step)) * 1099511628211ull;
+        if (code >= 4) {
             for (int c = 0; c < 3; c++) C32[c] = C32[c] + (T32 * a32) * r[8 + c];
             T32 = T32 * (1.0f - a32);
-            double a = code == 5 ? kCap : a64;
+            /* the f32 decision: saturated at the cap, or the unclamped value */
+            double a = code == 5 ? kCap : raw64;
             for (int c = 0; c < 3; c++) C[c] += T * a * q[6 + c];
             T *= 1.0 - a;
         }
@@ -408,24 +426,31 @@ static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, 
         float o32 = C32[c] + T32 * (float)g->bg[c];
         double o = C[c] + T * g->bg[c];
         int in32 = o32 > 0.0f && o32 < 1.0f, in64 = o > 0 && o < 1;
-        if (in32 != in64) *cross = 1;
+        h = (h ^ (uint64_t)(16 + in64)) * 1099511628211ull;
         if (!in32) o = o32 <= 0.0f ? 0.0 : 1.0;
         double d = o - (double)g->target[3 * ((size_t)y * g->W + x) + c];
         loss += d * d;
     }
+    *sig = h;
     return loss;
 }
 
-/* loss over the tiles of rank k with record rec for rank k */
-static double local_loss(const orc_gradcheck *g, uint32_t k, const double *rec, double rad, int *cross) {
+/* loss over the tiles of rank k with record rec for rank k; pixel
+ * signatures combined in pixel order */
+static double local_loss(const orc_gradcheck *g, uint32_t k, const double *rec, double rad, uint64_t *sig_out) {
     double total = 0;
+    uint64_t sig = 0;
     for (uint32_t e = g->stoff[k]; e < g->stoff[k + 1]; e++) {
         uint32_t t = g->stiles[e];
         uint32_t tx = t % g->tiles_x, ty = t / g->tiles_x;
         for (uint32_t y = ty * GC_TILE; y < ty * GC_TILE + GC_TILE && y < g->H; y++)
-            for (uint32_t x = tx * GC_TILE; x < tx * GC_TILE + GC_TILE && x < g->W; x++)
-                total += pixel_loss_forced(g, x, y, t, k, rec, rad, cross);
+            for (uint32_t x = tx * GC_TILE; x < tx * GC_TILE + GC_TILE && x < g->W; x++) {
+                uint64_t ps;
+                total += pixel_loss_forced(g, x, y, t, k, rec, rad, &ps);
+                sig = (sig ^ ps) * 1099511628211ull + (x * 73856093u ^ y * 19349663u);
+            }
     }
+    *sig_out = sig;
     return total;
 }
 
@@ -445,6 +470,7 @@ static int gc_fd_one(const orc_gradcheck *g, uint32_t id, uint32_t param, double
     if (g->radius[k] <= 0) return GC_NOT_VISIBLE;
     double p[14 + 45], rec[2][9], rad[2], dep[2];
     uint32_t st[2] = {0, 0};
+    uint64_t sg[2] = {0, 0};
     double L[2] = {0, 0};
     int flags = 0;
     for (int side = 0; side < 2; side++) {
@@ -455,6 +481,15 @@ static int gc_fd_one(const orc_gradcheck *g, uint32_t id, uint32_t param, double
             continue;
         }
         if (rad[side] != g->radius[k]) flags |= GC_RADIUS;
+        /* the f32 forward's record moved by the exact f64 change of the
+         * projection: the function differenced is the forward the device
+         * evaluates (its f32 values) and its change is the f64 projection's */
+        {
+            const float *r32 = g->proj32 + 12 * (size_t)k;
+            const double base32[9] = {r32[0], r32[1], r32[4], r32[5], r32[6], r32[7], r32[8], r32[9], r32[10]};
+            const double *b64 = g->rec + 9 * (size_t)k;
+            for (int f = 0; f < 9; f++) rec[side][f] = base32[f] + (rec[side][f] - b64[f]);
+        }
         /* depth order: the f32 key of the f64 depth against the neighbours (ties by id) */
         if (param < 3) {
             uint32_t key = orc_float_to_sortable((float)dep[side]);
@@ -467,14 +502,134 @@ static int gc_fd_one(const orc_gradcheck *g, uint32_t id, uint32_t param, double
                 if (key > nk || (key == nk && id > nid)) flags |= GC_ORDER;
             }
         }
-        int cross = 0;
-        L[side] = local_loss(g, k, rec[side], g->radius[k], &cross);
-        if (cross) flags |= GC_BRANCH;
+        L[side] = local_loss(g, k, rec[side], g->radius[k], &sg[side]);
     }
     if (flags & GC_CULL) return flags;
+    if (sg[0] != sg[1]) flags |= GC_BRANCH;
     if (st[0] != st[1]) flags |= GC_PROJ_KINK;
     *out = (L[1] - L[0]) / (2 * h);
     return flags;
+}
+
+/* Difference quotient of L with respect to one field of the splat's f64
+ * record (0 sx, 1 sy, 2 A, 3 B, 4 C, 5 alpha, 6-8 colour): the
+ * screen-space gradients the backward's rasterizer pass produces. */
+int orc_gc_fd_record(const orc_gradcheck *g, uint32_t id, uint32_t field, double h, double *out) {
+    *out = 0;
+    if (!g || id >= g->s.count || field >= 9) return -1;
+    uint32_t k = g->rank[id];
+    if (k == 0xFFFFFFFFu || g->radius[k] <= 0) return GC_NOT_VISIBLE;
+    double rec[9], L[2];
+    uint64_t sg[2];
+    for (int side = 0; side < 2; side++) {
+        memcpy(rec, g->rec + 9 * (size_t)k, sizeof(rec));
+        rec[field] += side ? h : -h;
+        L[side] = local_loss(g, k, rec, g->radius[k], &sg[side]);
+    }
+    *out = (L[1] - L[0]) / (2 * h);
+    return sg[0] != sg[1] ? GC_BRANCH : 0;
+}
+
+/* dL/d(record alpha) of one splat by the backward's own recurrence (front
+ * to back, S = what lies behind = output minus the running sum), in f64
+ * (f64 = 1) or in f32 (f64 = 0), on the f32 forward's records and
+ * decisions: separates the algorithm from its f32 evaluation. */
+double orc_gc_dalpha(const orc_gradcheck *g, uint32_t id, int f64) {
+    uint32_t k0 = g->rank[id];
+    if (k0 == 0xFFFFFFFFu) return 0;
+    double total = 0;
+    for (uint32_t e = g->stoff[k0]; e < g->stoff[k0 + 1]; e++) {
+        uint32_t t = g->stiles[e];
+        uint32_t tx = t % g->tiles_x, ty = t / g->tiles_x;
+        for (uint32_t y = ty * GC_TILE; y < ty * GC_TILE + GC_TILE && y < g->H; y++)
+            for (uint32_t x = tx * GC_TILE; x < tx * GC_TILE + GC_TILE && x < g->W; x++) {
+                /* forward in f32 (the device's values) */
+                float T = 1.0f, Cc[3] = {0, 0, 0};
+                const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+                uint32_t n = 0;
+                for (uint32_t q = g->toff[t]; q < g->toff[t + 1]; q++, n++) {
+                    if (T < (float)kMinT_) break;
+                    const float *r = g->proj32 + 12 * (size_t)g->tlist[q];
+                    float dx = px - r[0], dy = py - r[1];
+                    if (r[2] <= 0.0f || fabsf(dx) > r[2] || fabsf(dy) > r[2]) continue;
+                    float power = -0.5f * (((r[4] * dx) * dx + ((2.0f * r[5]) * dx) * dy) + (r[6] * dy) * dy);
+                    if (power > 0.0f) continue;
+                    float ge = orc_expf(power) - (float)kFloor;
+                    float raw = r[7] * (ge > 0.0f ? ge : 0.0f);
+                    float a = raw < (float)kCap ? raw : (float)kCap;
+                    if (a < (float)kMinA) continue;
+                    for (int c = 0; c < 3; c++) Cc[c] = Cc[c] + (T * a) * r[8 + c];
+                    T = T * (1.0f - a);
+                }
+                double G[3];
+                for (int c = 0; c < 3; c++) {
+                    float pre = Cc[c] + T * (float)g->bg[c];
+                    float img = pre < 0.0f ? 0.0f : pre > 1.0f ? 1.0f : pre;
+                    float grad = 2.0f * (img - g->target[3 * ((size_t)y * g->W + x) + c]);
+                    G[c] = (pre > 0.0f && pre < 1.0f) ? grad : 0.0;
+                }
+                /* backward recurrence */
+                double Td = 1.0, S[3];
+                float Tf = 1.0f, Sf[3];
+                for (int c = 0; c < 3; c++) { S[c] = Cc[c] + T * (float)g->bg[c]; Sf[c] = Cc[c] + T * (float)g->bg[c]; }
+                for (uint32_t q = g->toff[t]; q < g->toff[t + 1]; q++) {
+                    if (f64 ? Tf < (float)kMinT_ : Tf < (float)kMinT_) break;
+                    uint32_t k = g->tlist[q];
+                    const float *r = g->proj32 + 12 * (size_t)k;
+                    float dx = px - r[0], dy = py - r[1];
+                    if (r[2] <= 0.0f || fabsf(dx) > r[2] || fabsf(dy) > r[2]) continue;
+                    float power = -0.5f * (((r[4] * dx) * dx + ((2.0f * r[5]) * dx) * dy) + (r[6] * dy) * dy);
+                    if (power > 0.0f) continue;
+                    float ef = orc_expf(power);
+                    float ge = ef - (float)kFloor;
+                    float raw = r[7] * (ge > 0.0f ? ge : 0.0f);
+                    float a = raw < (float)kCap ? raw : (float)kCap;
+                    if (a < (float)kMinA) continue;
+                    if (f64) {
+                        double w = Td * a, da = 0;
+                        for (int c = 0; c < 3; c++) {
+                            S[c] -= r[8 + c] * w;
+                            da += G[c] * (r[8 + c] * Td - S[c] / (1.0 - a));
+                        }
+                        if (k == k0 && raw < (float)kCap) total += da * ((double)ef - kFloor);
+                        Td *= 1.0 - a;
+                    } else {
+                        float w = Tf * a, da = 0.0f;
+                        for (int c = 0; c < 3; c++) {
+                            Sf[c] = Sf[c] - r[8 + c] * w;
+                            da += (float)G[c] * (r[8 + c] * Tf - Sf[c] / (1.0f - a));
+                        }
+                        if (k == k0 && raw < (float)kCap) total += (double)(da * (ef - (float)kFloor));
+                    }
+                    Tf = Tf * (1.0f - a);
+                }
+            }
+    }
+    return total;
+}
+
+/* per pixel of one tile: the forced-loss quotient for the splat's alpha
+ * (out_fd[256]) and the pixel's backward-recurrence term (out_an[256]) */
+int orc_gc_pixel_terms(const orc_gradcheck *g, uint32_t id, uint32_t t, double h, double *out_fd, double *out_an) {
+    uint32_t k0 = g->rank[id];
+    if (k0 == 0xFFFFFFFFu) return -1;
+    uint32_t tx = t % g->tiles_x, ty = t / g->tiles_x;
+    for (uint32_t i = 0; i < 256; i++) {
+        uint32_t x = tx * GC_TILE + (i & 15), y = ty * GC_TILE + (i >> 4);
+        out_fd[i] = out_an[i] = 0;
+        if (x >= g->W || y >= g->H) continue;
+        double rec[9];
+        memcpy(rec, g->rec + 9 * (size_t)k0, sizeof(rec));
+        uint64_t sg;
+        double base5 = rec[5];
+        rec[5] = base5 + h;
+        double lp = pixel_loss_forced(g, x, y, t, k0, rec, g->radius[k0], &sg);
+        rec[5] = base5 - h;
+        double lm = pixel_loss_forced(g, x, y, t, k0, rec, g->radius[k0], &sg);
+        out_fd[i] = (lp - lm) / (2 * h);
+    }
+    (void)out_an;
+    return 0;
 }
 
 int orc_gc_fd(const orc_gradcheck *g, uint32_t n, const uint32_t *ids, const uint32_t *params, double h,

@@ -166,11 +166,11 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
  * splat are indexed 0-2 position, 3-6 rotation, 7-9 log-scale, 10 opacity
  * logit, 11-13 DC, 14.. SH (channel-major, as colors_sh). orc_gc_fd
  * returns the difference quotients plus flags: 1 not visible, 2 integer
- * radius changes, 4 depth order changes, 8 culled, 16 on some pixel of the
- * splat's tiles the f64 values at p+h or p-h branch differently from the
- * f32 forward (which decides the branches of the checked function), 32 a
- * kink of the projection (fade, eigenvalue floor, colour clamp) lies within
- * the stencil. */
+ * radius changes, 4 depth order changes, 8 culled, 16 a per-pixel branch
+ * that depends on the splat (its own test, the early stop, an output
+ * clamp) differs between p+h and p-h, 32 a kink of the projection (fade,
+ * eigenvalue floor, colour clamp) lies within the stencil. The f32 forward
+ * decides the per-pixel branches of the checked function. */
 typedef struct orc_gradcheck orc_gradcheck;
 orc_gradcheck *orc_gc_create(const orc_splats *s, const orc_camera *cam, const float *bg,
                              const float *target_rgb, uint32_t max_coverage, int num_threads);
@@ -182,6 +182,9 @@ int orc_gc_record(const orc_gradcheck *g, uint32_t id, double out[10]);
 void orc_gc_image(const orc_gradcheck *g, double *out_rgb, double *loss, int num_threads);
 int orc_gc_fd(const orc_gradcheck *g, uint32_t n, const uint32_t *ids, const uint32_t *params, double h,
               double *out, int32_t *flags, int num_threads);
+/* the same for one field of the splat's f64 record (0 sx, 1 sy, 2 A, 3 B,
+ * 4 C, 5 alpha, 6-8 colour) */
+int orc_gc_fd_record(const orc_gradcheck *g, uint32_t id, uint32_t field, double h, double *out);
 void orc_aabb_extents(const orc_splats *s, float *out_ext);
 /* BVH DFS leaf order (right child first) of all Gaussians, written to out_perm[N] */
 int orc_bvh_order(const float *positions, const float *extents, uint32_t n, uint32_t *out_perm);

@@ -37,15 +37,16 @@ def test_f64_forward_matches_f32_oracle(bands):
 
 def test_fd_matches_closed_form_on_one_splat():
     """One splat, background 0: out_c = clamp(col_c * alpha_px); dL/d(dc_c) =
-    sum_px 2 (out_c - t_c) alpha_px where 0 < out_c < 1."""
+    sum_px 2 (out_c - t_c) alpha_px where 0 < out_c < 1 (alpha_px from the
+    f32 forward's record, in f64)."""
     data = S.one_splat((0.05, -0.03, 0.0), color=(0.7, 0.4, 0.2), log_scale=-2.2, op_logit=1.0)
     cam = S.default_camera(96, 64)
     tgt = np.random.default_rng(0).random((64, 96, 3)).astype(np.float32) * 0.3
     gc = oracle.GradCheck(data, cam, tgt)
-    rec = gc.record(0)
-    sx, sy, A, B, Cc, a = rec[:6]
-    col = rec[6:9]
-    r = rec[9]
+    # the checked function is the f32 forward's: its record of the splat
+    p = oracle.render(data, cam, taps=True)["projected"][0].astype(np.float64)
+    sx, sy, r, A, B, Cc, a = p[0], p[1], p[2], p[4], p[5], p[6], p[7]
+    col = p[8:11]
     ys, xs = np.mgrid[0:64, 0:96]
     dx, dy = xs + 0.5 - sx, ys + 0.5 - sy
     power = -0.5 * (A * dx * dx + 2 * B * dx * dy + Cc * dy * dy)
