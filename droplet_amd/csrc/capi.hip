@@ -115,6 +115,7 @@ __global__ void k_debug_math(int op, const float *a, const float *b, float *out,
     case 2: r = a[i] / b[i]; break;
     case 3: r = bits_f32(sortable_key((float)((double)a[i] * (double)b[i]))); break;
     case 4: r = dl_sigmoidf(a[i]); break;
+    case 5: r = dl_expf_nonpos(a[i]); break;
     default: break;
   }
   out[i] = r;
@@ -484,7 +485,7 @@ int dl_debug_math(dl_context *ctx, int op, const float *a, const float *b, float
   return guard([&] {
     if (!ctx || !a || !out) fail(DL_ERR_INVALID, "null argument");
     if ((op == 2 || op == 3) && !b) fail(DL_ERR_INVALID, "op needs b");
-    if (op < 0 || op > 4) fail(DL_ERR_INVALID, "unknown op");
+    if (op < 0 || op > 5) fail(DL_ERR_INVALID, "unknown op");
     if (n == 0) return;
     Context &c = ctx->c;
     c.make_current();

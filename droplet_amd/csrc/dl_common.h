@@ -65,6 +65,28 @@ __host__ __device__ inline float dl_expf(float x) {
   return (p * s1) * s2;
 }
 
+// dl_expf for x in [-87, 0] or NaN, bit-identical there. The clamps are no-ops
+// on that range. 2^k (-126 <= k <= 0) is a normal power of two, so one exact
+// scaling by ldexp equals the two exact steps. NaN propagates through p.
+// The rasterizer's exponent always lies in [tau, 0], and tau > -4.6.
+__host__ __device__ inline float dl_expf_nonpos(float x) {
+  float kf = __builtin_rintf(x * 0x1.715476p+0f);
+  float r = __builtin_fmaf(kf, -0x1.62e430p-1f, x);
+  r = __builtin_fmaf(kf, 0x1.05c610p-29f, r);
+  float p = 0x1.a01a02p-13f;
+  p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+  p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+  p = __builtin_fmaf(p, r, 0.5f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  // k from the bits of kf + 1.5 * 2^23 (exact for integral |kf| < 2^22);
+  // defined for NaN too, where the result is NaN whatever k is
+  const int k = (int)(f32_bits(kf + 0x1.8p23f) - 0x4B400000u);
+  return __builtin_ldexpf(p, k);
+}
+
 // sigmoid (project.ts:106, voxelize.ts:388)
 __host__ __device__ inline float dl_sigmoidf(float x) { return 1.0f / (1.0f + dl_expf(-x)); }
 

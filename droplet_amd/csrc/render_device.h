@@ -404,7 +404,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
           const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
           if (power > 0.0f || power < col.w) continue;
-          const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf(power) - kGaussFloor));
+          const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf_nonpos(power) - kGaussFloor));
           if (alpha < kMinAlpha) continue;
           const float w = T[k] * alpha;
           cr[k] = cr[k] + w * col.x;

@@ -273,7 +273,7 @@ def sort_pairs_u32(device: Device, keys: np.ndarray, vals: np.ndarray, key_bits:
 
 
 def debug_math(device: Device, op: str, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:
-    ops = {"exp": 0, "sqrt": 1, "div": 2, "depth_key": 3, "sigmoid": 4}
+    ops = {"exp": 0, "sqrt": 1, "div": 2, "depth_key": 3, "sigmoid": 4, "exp_nonpos": 5}
     a = np.ascontiguousarray(a, dtype=np.float32)
     bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
     out = np.empty_like(a)

@@ -270,7 +270,8 @@ void dl_voxel_result_free(dl_voxel_result *res);
 
 /* ---- numerics self-check: evaluates the device's transcendental and IEEE
  * primitives on n inputs (used by the parity tests).
- * op: 0 exp, 1 sqrt, 2 div (a/b), 3 f64->f32 depth key, 4 sigmoid. */
+ * op: 0 exp, 1 sqrt, 2 div (a/b), 3 f64->f32 depth key, 4 sigmoid,
+ * 5 the rasterizer's exp for arguments in [-87, 0]. */
 int dl_debug_math(dl_context *ctx, int op, const float *a, const float *b, float *out, uint32_t n);
 
 #ifdef __cplusplus

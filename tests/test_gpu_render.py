@@ -66,6 +66,18 @@ def test_exp_matches_oracle_bitexact(device):
     np.testing.assert_array_equal(got.view(np.uint32), oracle.expf(x).view(np.uint32))
 
 
+def test_rasterizer_exp_matches_oracle_bitexact(device):
+    """The rasterizer's exp (one-step 2^k, arguments in [-87, 0] or NaN) is
+    bit-identical to the exp specification there."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-87, 0, 300_000), rng.uniform(-5, 0, 300_000),
+                        -np.abs(rng.standard_normal(100_000)) * 1e-6,
+                        np.array([0.0, -0.0, -1e-45, -4.6, -86.99, -87.0])]).astype(np.float32)
+    got = debug_math(device, "exp_nonpos", x)
+    np.testing.assert_array_equal(got.view(np.uint32), oracle.expf(x).view(np.uint32))
+    assert np.isnan(debug_math(device, "exp_nonpos", np.array([np.nan], np.float32))[0])
+
+
 def test_ieee_primitives_match_host(device):
     rng = np.random.default_rng(1)
     a = (rng.standard_normal(300_000) * 10.0 ** rng.uniform(-20, 20, 300_000)).astype(np.float32)
