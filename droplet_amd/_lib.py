@@ -98,6 +98,10 @@ SIGNATURES = [
     ("dl_render", C.c_int, [C.c_void_p, C.POINTER(DlSplats), C.POINTER(DlCamera), C.POINTER(DlBackground), u8p]),
     ("dl_scene_create", C.c_int, [C.c_void_p, C.POINTER(DlSplats), C.POINTER(C.c_void_p)]),
     ("dl_scene_destroy", None, [C.c_void_p]),
+    ("dl_scene_create_from_ply", C.c_int, [C.c_void_p, u8p, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    ("dl_scene_info", C.c_int, [C.c_void_p, u32p, u32p, u32p]),
+    ("dl_scene_read", C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p, f32p, f32p]),
+    ("dl_scene_bounds", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("dl_render_scene", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlBackground), u8p]),
     ("dl_renderer_read_image", C.c_int, [C.c_void_p, u8p]),
     ("dl_renderer_set_timing", C.c_int, [C.c_void_p, C.c_int]),
@@ -117,6 +121,7 @@ SIGNATURES = [
     ("dl_batch_read_image", C.c_int, [C.c_void_p, C.c_uint32, u8p]),
     ("dl_batch_get_stats", C.c_int, [C.c_void_p, C.POINTER(DlBatchStats)]),
     ("dl_sort_pairs_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint32, C.c_uint32]),
+    ("dl_scan_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint64, C.c_uint32, u32p]),
     ("dl_voxelize", C.c_int, [C.c_void_p, C.POINTER(DlSplats), C.POINTER(DlVoxelOpts), C.POINTER(C.c_void_p)]),
     ("dl_voxel_result_count", C.c_uint64, [C.c_void_p]),
     ("dl_voxel_result_blocks", C.c_int, [C.c_void_p, u32p, u32p, u32p]),

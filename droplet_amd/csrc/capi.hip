@@ -95,6 +95,7 @@ void Scene::upload(const dl_splats &sp) {
   DL_HIP(hipStreamSynchronize(s));
   count = sp.count;
   bands = sp.sh_bands;
+  sh_coeffs = M;
 }
 
 SceneView Scene::view() const {
@@ -258,6 +259,54 @@ void dl_scene_destroy(dl_scene *scene) {
   (void)hipSetDevice(scene->s.ctx->device);
   (void)hipStreamSynchronize(scene->s.ctx->stream);
   delete scene;
+}
+
+int dl_scene_create_from_ply(dl_context *ctx, const uint8_t *bytes, uint64_t len, int flip_y, dl_scene **out) {
+  return guard([&] {
+    if (!ctx || !out) fail(DL_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<dl_scene> sc(new dl_scene());
+    sc->s.ctx = &ctx->c;
+    sc->s.load_ply(bytes, (size_t)len, flip_y != 0);
+    *out = sc.release();
+  });
+}
+
+int dl_scene_info(const dl_scene *scene, uint32_t *count, uint32_t *sh_bands, uint32_t *sh_coeffs_per_channel) {
+  return guard([&] {
+    if (!scene) fail(DL_ERR_INVALID, "null scene");
+    if (count) *count = scene->s.count;
+    if (sh_bands) *sh_bands = scene->s.bands;
+    if (sh_coeffs_per_channel) *sh_coeffs_per_channel = scene->s.sh_coeffs;
+  });
+}
+
+int dl_scene_read(dl_scene *scene, float *positions, float *rotations, float *log_scales, float *opacity_logits,
+                  float *colors_dc, float *colors_sh) {
+  return guard([&] {
+    if (!scene) fail(DL_ERR_INVALID, "null scene");
+    Scene &s = scene->s;
+    s.ctx->make_current();
+    hipStream_t st = s.ctx->stream;
+    const size_t n = s.count;
+    auto get = [&](float *dst, const DeviceBuffer &b, size_t floats) {
+      if (dst && floats) DL_HIP(hipMemcpyAsync(dst, b.get(), floats * 4, hipMemcpyDeviceToHost, st));
+    };
+    get(positions, s.pos, 3 * n);
+    get(rotations, s.rot, 4 * n);
+    get(log_scales, s.ls, 3 * n);
+    get(opacity_logits, s.op, n);
+    get(colors_dc, s.dc, 3 * n);
+    get(colors_sh, s.sh, 3 * (size_t)s.sh_coeffs * n);
+    DL_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int dl_scene_bounds(dl_scene *scene, double bounds_min[3], double bounds_max[3]) {
+  return guard([&] {
+    if (!scene || !bounds_min || !bounds_max) fail(DL_ERR_INVALID, "null argument");
+    scene->s.bounds(bounds_min, bounds_max);
+  });
 }
 
 int dl_render_scene(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, const dl_background *bg,
@@ -477,6 +526,29 @@ int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t 
                                          (int)key_bits, c.sort_ws.get());
     DL_HIP(hipMemcpyAsync(keys, in_tmp ? c.sort_tmp_keys.get() : c.sort_keys.get(), bytes, hipMemcpyDeviceToHost, s));
     DL_HIP(hipMemcpyAsync(vals, in_tmp ? c.sort_tmp_vals.get() : c.sort_vals.get(), bytes, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dl_scan_u32(dl_context *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t repeat, uint32_t *total) {
+  return guard([&] {
+    if (!ctx || !total) fail(DL_ERR_INVALID, "null argument");
+    if (n && (!in || !out)) fail(DL_ERR_INVALID, "null in/out");
+    if (repeat == 0) fail(DL_ERR_INVALID, "repeat must be >= 1");
+    Context &c = ctx->c;
+    c.make_current();
+    hipStream_t s = c.stream;
+    const size_t bytes = (size_t)n * 4;
+    DeviceBuffer d_in, d_out, d_total, ws;
+    d_in.reserve(std::max<size_t>(bytes, 4));
+    d_out.reserve(std::max<size_t>(bytes, 4));
+    d_total.reserve(4);
+    ws.reserve(scan_workspace_bytes(n));
+    if (n) DL_HIP(hipMemcpyAsync(d_in.get(), in, bytes, hipMemcpyHostToDevice, s));
+    for (uint32_t r = 0; r < repeat; r++)
+      exclusive_scan_u32(s, d_in.as<uint32_t>(), d_out.as<uint32_t>(), n, d_total.as<uint32_t>(), ws.get());
+    if (n) DL_HIP(hipMemcpyAsync(out, d_out.get(), bytes, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(total, d_total.get(), 4, hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
   });
 }

@@ -52,8 +52,16 @@ struct RastParams {
 struct Scene {
   Context *ctx = nullptr;
   uint32_t count = 0, bands = 0;
+  // SH coefficients per channel held per splat: M(bands) for uploaded
+  // SplatData; loadPlySplats's floor(k_rest / 3) for a PLY scene, whose
+  // colorsSH array the renderer reads with stride 3 * M like the reference
+  uint32_t sh_coeffs = 0;
   DeviceBuffer pos, rot, ls, op, dc, sh;
   void upload(const dl_splats &s);
+  // loadPlySplats (ply.ts:120-239) into this scene (ply.hip)
+  void load_ply(const uint8_t *bytes, size_t len, bool flip_y);
+  // splatBounds (ply.ts:242-255) of the resident positions
+  void bounds(double lo[3], double hi[3]);
   SceneView view() const;
 };
 
@@ -77,6 +85,15 @@ struct Renderer {
   bool timing = false;
   bool keep_taps = false;
   bool counting = false;
+  // Frames of span binning normally run without a host read of the
+  // visible-splat and pair totals after the depth sort: passes cover all
+  // sorted splats (invisible ones have no tiles) and the pair capacity is
+  // the largest total seen so far plus headroom. The totals are read at the
+  // end of the frame; a frame that needed more capacity is rendered again
+  // with the exact totals. spec_cap is 0 until a frame has run.
+  uint64_t spec_cap = 0;
+  uint64_t last_pairs_all = 0;
+  bool force_exact = false;
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)
   std::vector<hipEvent_t> events;

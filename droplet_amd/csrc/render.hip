@@ -302,6 +302,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   uint32_t nvis = 0;
   unsigned long long pairs_all = 0;
   const uint32_t *sorted_ids = nullptr;
+  const bool speculative = !force_exact && spec_cap > 0 && n > 0 && !keep_taps && !train_mode &&
+                           opts.binning == 0 && span_binning_supported(tiles_x, tiles_y) &&
+                           (opts.chunk_cap == 0 || opts.chunk_cap >= n);
   if (n > 0) {
     const uint32_t nparts = div_up(n, kProjBlock);
     keys.reserve((size_t)n * 4);
@@ -327,9 +330,16 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
                                          tids.as<uint32_t>(), n, 32, sort_ws.get(), digit_bits());
     sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
     const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
-    DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
-    DL_HIP(hipStreamSynchronize(s));
+    if (speculative) {
+      // invisible splats sort last and have no tiles: passes over all n
+      // splats build the same lists; the capacity is checked after the frame
+      nvis = n;
+      pairs_all = spec_cap;
+    } else {
+      DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
+      DL_HIP(hipStreamSynchronize(s));
+    }
     // every per-chunk u32 scan below sums a subset of these pairs
     if (pairs_all > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (tile, splat) pairs in one frame");
     if (keep_taps) {
@@ -371,7 +381,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     // taps). A pass costs about 30 launches, so by default they are used only
     // when the frame has enough pairs for the skipped binning to pay for them.
     const bool early_stop = span && !keep_taps && !train_mode && !opts.full_lists &&
-                            (opts.first_pass || opts.pass_growth || pairs_all >= kPassMinPairs);
+                            (opts.first_pass || opts.pass_growth ||
+                             (speculative ? last_pairs_all : pairs_all) >= kPassMinPairs);
     std::vector<uint32_t> cuts;  // pass starts, then nvis
     if (opts.chunk_cap && opts.chunk_cap < nvis && !train_mode) {
       for (uint64_t c0 = 0; c0 < nvis; c0 += opts.chunk_cap) cuts.push_back((uint32_t)c0);
@@ -496,14 +507,36 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   }
   mark(-1);
   {
-    unsigned long long cnt[3] = {0, 0, 0};
-    DL_HIP(hipMemcpyAsync(cnt, d_iters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    // small: [0] visible, [8..9] all pairs, [10..15] loop counters
+    uint32_t sm[16] = {};
+    DL_HIP(hipMemcpyAsync(sm, d_nvis, sizeof(sm), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
+    auto u64_at = [&](int k) { return (unsigned long long)sm[k] | ((unsigned long long)sm[k + 1] << 32); };
     if (counting) {
-      stats.pixel_iters = cnt[0];
-      stats.pairs_consumed = cnt[1];
+      stats.pixel_iters = u64_at(10);
+      stats.pairs_consumed = u64_at(12);
     }
-    stats.list_entries = cnt[2];
+    stats.list_entries = u64_at(14);
+    if (n > 0) {
+      const unsigned long long all = u64_at(8);
+      last_pairs_all = all;
+      if (all <= 0xFFFFFFF0ull)
+        spec_cap = std::max<uint64_t>(spec_cap, std::min<uint64_t>(all + all / 8 + 4096, 0xFFFFFFF0ull));
+      if (speculative) {
+        stats.num_visible = sm[0];
+        stats.num_pairs = all;
+        if (all > pairs_all) {
+          // the lists were cut at the capacity: render again with exact totals
+          struct Reset {
+            bool &f;
+            ~Reset() { f = false; }
+          } reset{force_exact};
+          force_exact = true;
+          render(sc, cam, bg);
+          return;
+        }
+      }
+    }
   }
   DL_HIP(hipStreamSynchronize(s));
   if (keep_taps) {

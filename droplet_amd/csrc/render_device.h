@@ -342,19 +342,20 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
   const uint32_t px = tx * kTile + (lane & 15);
   const uint32_t py0 = ty * kTile + (lane >> 4);
   const float pxf = (float)px + 0.5f;
+  // A pixel is live while !(T < kMinT); pixels outside the image start
+  // with T = 0, so they are never live and never written.
   float cr[4], cg[4], cb[4], T[4], pyf[4];
-  bool live[4], inb[4];
+  bool inb[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const uint32_t y = py0 + 4 * k;
     inb[k] = px < p.width && y < p.height;
     pyf[k] = (float)y + 0.5f;
-    cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = 1.0f;
+    cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = inb[k] ? 1.0f : 0.0f;
     if (inb[k] && p.resume) {
       const float4 a = accum[(size_t)y * p.width + px];
       cr[k] = a.x; cg[k] = a.y; cb[k] = a.z; T[k] = a.w;
     }
-    live[k] = inb[k] && !(T[k] < kMinT);
   }
   uint32_t consumed = 0;
   if (start < end) {
@@ -363,7 +364,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
     for (uint32_t base = start; base < end; base += 64) {
       uint32_t strips_live = 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) strips_live |= (__ballot(live[k]) != 0 ? 1u : 0u) << k;
+      for (int k = 0; k < 4; k++) strips_live |= (__ballot(!(T[k] < kMinT)) != 0 ? 1u : 0u) << k;
       if (strips_live == 0) {
         stop = base;
         break;
@@ -388,7 +389,9 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
       __syncthreads();
       const uint32_t cnt = min(64u, end - base);
       for (uint32_t c = 0; c < cnt; c++) {
-        if (COUNT) iters += (uint32_t)live[0] + (uint32_t)live[1] + (uint32_t)live[2] + (uint32_t)live[3];
+        if (COUNT)
+          iters += (uint32_t)!(T[0] < kMinT) + (uint32_t)!(T[1] < kMinT) + (uint32_t)!(T[2] < kMinT) +
+                   (uint32_t)!(T[3] < kMinT);
         const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[c]) & strips_live;
         if (m == 0) continue;
         const float4 a = s_rec[3 * c + 0];
@@ -397,7 +400,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           if (!((m >> k) & 1u)) continue;
-          if (!live[k]) continue;
+          if (T[k] < kMinT) continue;
           const float dx = pxf - a.x;
           const float dy = pyf[k] - a.y;
           const float r = a.z;
@@ -411,7 +414,6 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           cg[k] = cg[k] + w * col.y;
           cb[k] = cb[k] + w * col.z;
           T[k] = T[k] * (1.0f - alpha);
-          live[k] = !(T[k] < kMinT);
         }
       }
       __syncthreads();
@@ -426,7 +428,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
     if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }
   if (p.closed && !p.finalize) {
-    const bool open = __ballot(live[0] || live[1] || live[2] || live[3]) != 0;
+    const bool open = __ballot(!(T[0] < kMinT) || !(T[1] < kMinT) || !(T[2] < kMinT) || !(T[3] < kMinT)) != 0;
     if (lane == 0) p.closed[vtile] = open ? 0 : 1;
   }
   if (COUNT) {

@@ -8,6 +8,7 @@ libdroplet_hip.so on the GPU.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import sys
 from dataclasses import dataclass
 from typing import Optional
@@ -86,14 +87,43 @@ def create_device(opts: Optional[DeviceOptions] = None) -> Device:
 class Scene:
     """Device-resident SplatData: uploaded once, rendered many times."""
 
-    def __init__(self, device: Device, data: SplatData):
+    def __init__(self, device: Device, data: Optional[SplatData] = None, *, _handle=None):
         self.device = device
-        self.count = data.count
-        self.sh_bands = data.sh_bands
+        if _handle is None:
+            h = C.c_void_p()
+            s = L.splats_struct(data)
+            L.check(L.load().dl_scene_create(device.handle, C.byref(s), C.byref(h)))
+            _handle = h
+        self._h = _handle
+        n, b, k = (C.c_uint32(), C.c_uint32(), C.c_uint32())
+        L.check(L.load().dl_scene_info(self._h, C.byref(n), C.byref(b), C.byref(k)))
+        self.count, self.sh_bands, self.sh_coeffs_per_channel = n.value, b.value, k.value
+
+    @classmethod
+    def from_ply(cls, device: Device, source, flip_y: bool = True) -> "Scene":
+        """loadPlySplats (src/io/ply.ts:120-239) straight into device memory:
+        `source` is the file's bytes (bytes, bytearray, memoryview, a uint8
+        array) or a path, which is memory-mapped."""
+        buf = _ply_bytes(source)
         h = C.c_void_p()
-        s = L.splats_struct(data)
-        L.check(L.load().dl_scene_create(device.handle, C.byref(s), C.byref(h)))
-        self._h = h
+        L.check(L.load().dl_scene_create_from_ply(device.handle, L.ptr(buf, L.u8p), buf.size, 1 if flip_y else 0,
+                                                  C.byref(h)))
+        return cls(device, _handle=h)
+
+    def read(self) -> SplatData:
+        """The scene's SplatData arrays, copied back to the host."""
+        n, k = self.count, self.sh_coeffs_per_channel
+        out = [np.empty(c * n, np.float32) for c in (3, 4, 3, 1, 3, 3 * k)]
+        L.check(L.load().dl_scene_read(self._h, *(L.ptr(a, L.f32p) for a in out)))
+        return SplatData(n, out[0], out[1], out[2], out[3], out[4], out[5] if self.sh_bands else None,
+                         self.sh_bands)
+
+    def bounds(self):
+        """splatBounds (src/io/ply.ts:242-255) of the resident positions:
+        ((minx, miny, minz), (maxx, maxy, maxz))."""
+        lo, hi = (C.c_double * 3)(), (C.c_double * 3)()
+        L.check(L.load().dl_scene_bounds(self._h, lo, hi))
+        return tuple(lo), tuple(hi)
 
     def destroy(self) -> None:
         if getattr(self, "_h", None) is not None:
@@ -261,6 +291,33 @@ def tune_result_dict(res) -> dict:
             "candidates": int(res.candidates)}
 
 
+def _ply_bytes(source) -> np.ndarray:
+    if isinstance(source, (str, os.PathLike)):
+        if os.path.getsize(source) == 0:
+            return np.zeros(0, np.uint8)
+        return np.memmap(source, dtype=np.uint8, mode="r")
+    if isinstance(source, np.ndarray):
+        return np.ascontiguousarray(source.reshape(-1).view(np.uint8))
+    return np.frombuffer(source, dtype=np.uint8)
+
+
+def load_ply_splats(source, flip_y: bool = True, device: Optional[Device] = None) -> SplatData:
+    """`loadPlySplats` (src/io/ply.ts:120-239): parses the file on the device
+    and returns the SplatData arrays on the host. colors_sh holds loadPlySplats's
+    3 * floor(k_rest / 3) coefficients per splat."""
+    dev = device or create_device()
+    sc = Scene.from_ply(dev, source, flip_y)
+    try:
+        return sc.read()
+    finally:
+        sc.destroy()
+
+
+def splat_bounds(scene: Scene):
+    """`splatBounds` (src/io/ply.ts:242-255) of a resident scene."""
+    return scene.bounds()
+
+
 def sort_pairs_u32(device: Device, keys: np.ndarray, vals: np.ndarray, key_bits: int = 32):
     """Stable GPU sort of u32 (key, value) pairs on the low key_bits bits;
     the stable replacement of `GpuRadixSorter.sort` (src/gpu/radix-sort.ts:142-215)."""
@@ -270,6 +327,18 @@ def sort_pairs_u32(device: Device, keys: np.ndarray, vals: np.ndarray, key_bits:
         raise ValueError("keys and vals differ in length")
     L.check(L.load().dl_sort_pairs_u32(device.handle, L.ptr(k, L.u32p), L.ptr(v, L.u32p), k.size, key_bits))
     return k, v
+
+
+def scan_u32(device: Device, values: np.ndarray, repeat: int = 1):
+    """Exclusive prefix sum (wrapping u32) on the device, run `repeat` times
+    back to back; returns (prefix, total). The device-wide form of the
+    histogram scan in src/gpu/radix-sort.ts:172-187."""
+    a = np.ascontiguousarray(values, dtype=np.uint32)
+    out = np.empty_like(a)
+    total = np.zeros(1, np.uint32)
+    L.check(L.load().dl_scan_u32(device.handle, L.ptr(a, L.u32p), L.ptr(out, L.u32p), a.size, repeat,
+                                 L.ptr(total, L.u32p)))
+    return out, int(total[0])
 
 
 def debug_math(device: Device, op: str, a: np.ndarray, b: Optional[np.ndarray] = None) -> np.ndarray:

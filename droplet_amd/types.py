@@ -65,7 +65,13 @@ class SplatData:
         if m > 0:
             if self.colors_sh is None:
                 raise ValueError("colorsSH is required when shBands > 0")
-            self.colors_sh = _f32(self.colors_sh, 3 * m * n, "colorsSH")
+            sh = np.ascontiguousarray(np.asarray(self.colors_sh, dtype=np.float32).reshape(-1))
+            # loadPlySplats (src/io/ply.ts:181-188) can hold more coefficients
+            # per splat than shBands uses; the renderer reads the first 3*M*N
+            # values with stride 3*M, as the reference's packer does
+            if sh.size < 3 * m * n:
+                raise ValueError(f"colorsSH: expected at least {3 * m * n} float32 values, got {sh.size}")
+            self.colors_sh = sh
         elif self.colors_sh is not None:
             self.colors_sh = np.ascontiguousarray(np.asarray(self.colors_sh, dtype=np.float32).reshape(-1))
 

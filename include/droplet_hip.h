@@ -8,7 +8,8 @@
  *   src/gpu/radix-sort.ts    GpuRadixSorter.sort
  *   src/render/pipeline.ts   Renderer (constructor, render, destroy)
  *   src/voxel/voxelize.ts    voxelize
- *   src/render/shaders/*, src/voxel/shaders/voxelize.ts   (every WGSL kernel)
+ *   src/render/shaders/, src/voxel/shaders/voxelize.ts   (every WGSL kernel)
+ *   src/io/ply.ts            loadPlySplats, splatBounds
  *
  * Conventions:
  *   - Every call returns DL_OK (0) or an error code; dl_last_error() gives a
@@ -123,6 +124,27 @@ int dl_render(dl_renderer *r, const dl_splats *splats, const dl_camera *cam,
 /* Device-resident scene: upload once, render many frames. */
 int dl_scene_create(dl_context *ctx, const dl_splats *splats, dl_scene **out);
 void dl_scene_destroy(dl_scene *scene);
+
+/* ---- PLY ingestion: loadPlySplats (src/io/ply.ts:120-239) straight into a
+ * resident scene. bytes/len: the whole file (already read or mapped); the
+ * header is parsed on the host with the reference's rules and messages
+ * (parseHeader, ply.ts:40-91); the binary vertex section is copied to the
+ * device as it lies and de-interleaved there. flip_y != 0 is the
+ * reference's default (PlyLoadOptions.flipY, ply.ts:96-111). */
+int dl_scene_create_from_ply(dl_context *ctx, const uint8_t *bytes, uint64_t len, int flip_y, dl_scene **out);
+/* count, shBands and SH coefficients per channel held per splat: M(bands)
+ * for a dl_splats scene, loadPlySplats's floor(k_rest / 3) for a PLY
+ * scene (its colorsSH array, ply.ts:181-188, which the renderer reads with
+ * the stride of shBands like the reference). Any pointer may be NULL. */
+int dl_scene_info(const dl_scene *scene, uint32_t *count, uint32_t *sh_bands, uint32_t *sh_coeffs_per_channel);
+/* The scene's SplatData arrays back on the host (loadPlySplats's result for a
+ * PLY scene). Any pointer may be NULL to skip it; colors_sh takes count * 3 *
+ * sh_coeffs_per_channel floats. */
+int dl_scene_read(dl_scene *scene, float *positions, float *rotations, float *log_scales,
+                  float *opacity_logits, float *colors_dc, float *colors_sh);
+/* splatBounds (src/io/ply.ts:242-255) of the resident positions: +Infinity /
+ * -Infinity per axis for an empty scene. */
+int dl_scene_bounds(dl_scene *scene, double bounds_min[3], double bounds_max[3]);
 /* Render a resident scene. out_rgba may be NULL: the image then stays on the
  * device (dl_renderer_read_image copies it later). */
 int dl_render_scene(dl_renderer *r, const dl_scene *scene, const dl_camera *cam,
@@ -245,6 +267,14 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
  * src/gpu/radix-sort.ts:142-215, which is unstable; this one is stable).
  * Host arrays in/out; only the low key_bits bits of each key are compared. */
 int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t key_bits);
+
+/* ---- exclusive prefix sum of u32 (wrapping) values: the device-wide
+ * replacement for the CPU scan of the sort's histogram
+ * (src/gpu/radix-sort.ts:172-187), and the scan under the renderer's
+ * binning and the voxelizer. Host arrays; out may alias in. The scan runs
+ * `repeat` (>= 1) times back to back on the device (the same result each
+ * time); *total gets the sum. */
+int dl_scan_u32(dl_context *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t repeat, uint32_t *total);
 
 /* ---- voxelizer (replaces voxelize, src/voxel/voxelize.ts:59-248) ---- */
 typedef struct {

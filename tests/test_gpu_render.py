@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 from droplet_amd import synthetic as S
-from droplet_amd.renderer import Renderer, RendererOptions, Scene, debug_math, sort_pairs_u32
+from droplet_amd.renderer import Renderer, RendererOptions, Scene, debug_math, scan_u32, sort_pairs_u32
 from droplet_amd.types import BackgroundRGBA
 
 pytestmark = pytest.mark.gpu
@@ -98,6 +98,22 @@ def test_sort_pairs_stable(device):
         order = np.argsort(masked, kind="stable")
         np.testing.assert_array_equal(v, vals[order])
         np.testing.assert_array_equal(k, keys[order])
+
+
+def test_scan_u32(device):
+    """The device-wide scan against a wrapping u32 cumulative sum: empty, one
+    tile, tile edges, several levels of partials and 20M values; three scans
+    back to back on one workspace."""
+    rng = np.random.default_rng(3)
+    for n, hi in [(0, 10), (1, 10), (4095, 2 ** 32), (4096, 1000), (4097, 2 ** 32), (64 * 4096 + 5, 2 ** 32),
+                  (300_001, 3), (20_000_000, 2 ** 32)]:
+        print(f"case: scan n={n}", flush=True)
+        a = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+        out, total = scan_u32(device, a, repeat=3)
+        inc = np.cumsum(a, dtype=np.uint32)
+        excl = np.concatenate([np.zeros(1, np.uint32), inc[:-1]]) if n else inc
+        np.testing.assert_array_equal(out, excl)
+        assert total == (int(inc[-1]) if n else 0)
 
 
 def test_render_plain_workload_parity(renderer):
