@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dl_common.h"
 #include "dl_internal.h"
 
 namespace dl {
@@ -10,6 +11,56 @@ namespace dl {
 // LDS-bounded grid limits of the span path (larger grids use emit + sort)
 constexpr uint32_t kMaxBinTilesX = 256;
 constexpr uint32_t kMaxBinTilesY = 1024;
+
+struct Rect {
+  uint32_t x0, x1, y0;
+  uint32_t nrows;               // rows y0 .. y0+nrows-1 (0: no pairs)
+  uint32_t full_rows;           // rows y0 .. y0+full_rows-1 span [x0, x1]
+  uint32_t rem;                 // row y0+full_rows spans [x0, x0+rem-1] when rem > 0
+};
+
+// tile rectangle and maxCoverage truncation of one projected splat, with the
+// exact arithmetic of project.ts:275-283 / emit-pairs.ts:164-181
+__device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov) {
+  Rect R{0, 0, 0, 0, 0, 0};
+  const float sx = v0.x, sy = v0.y, radius = v0.z;
+  if (!(radius > 0.0f)) return R;
+  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
+  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
+  const uint32_t max_tx = u32sat(wmin((float)tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
+  const uint32_t max_ty = u32sat(wmin((float)tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
+  if (max_tx < min_tx || max_ty < min_ty) return R;
+  const uint32_t w = max_tx - min_tx + 1u, h = max_ty - min_ty + 1u;
+  R.x0 = min_tx;
+  R.x1 = max_tx;
+  R.y0 = min_ty;
+  if ((uint64_t)w * h <= max_cov) {
+    R.full_rows = h;
+    R.rem = 0;
+  } else {
+    R.full_rows = max_cov / w;
+    R.rem = max_cov - R.full_rows * w;
+  }
+  R.nrows = R.full_rows + (R.rem > 0 ? 1u : 0u);
+  return R;
+}
+
+// Rect of a span-binnable grid (x < 256, y < 1024, rem < 256) in 8 bytes:
+// (x0 | x1 << 8 | rem << 16, y0 | full_rows << 16)
+__device__ inline uint2 pack_rect8(const Rect &R) {
+  return make_uint2(R.x0 | (R.x1 << 8) | (R.rem << 16), R.y0 | (R.full_rows << 16));
+}
+
+__device__ inline Rect unpack_rect8(const uint2 p) {
+  Rect R;
+  R.x0 = p.x & 0xFFu;
+  R.x1 = (p.x >> 8) & 0xFFu;
+  R.rem = p.x >> 16;
+  R.y0 = p.y & 0xFFFFu;
+  R.full_rows = p.y >> 16;
+  R.nrows = R.full_rows + (R.rem > 0 ? 1u : 0u);
+  return R;
+}
 
 struct BinGrid {
   uint32_t tiles_x, tiles_y, max_cov;
@@ -21,6 +72,9 @@ struct BinGrid {
 struct LiveMap {
   const uint16_t *pre = nullptr;
   uint32_t stride = 0;
+  // summed-area table of the open tiles, same stride, rows + 1 rows (set by
+  // the live-splat compaction)
+  const uint32_t *sat = nullptr;
 };
 
 // Level-1 geometry of one view. A batch stacks the tile rows of its views
@@ -46,6 +100,9 @@ struct BinWorkspace {
   DeviceBuffer tmp, chunk_base_a, seg_of_a, c2, base2, gspans;
   DeviceBuffer seg_start_b, chunk_base_b, seg_of_b, c3, base3;
   DeviceBuffer scan_ws;
+  // live-splat compaction of later passes: id-order live bits, per-block
+  // counts and offsets, [0] live items and [1] their level-1 blocks
+  DeviceBuffer live_bits, live_cnt, live_off, live_small, live_sat;
 };
 
 bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y);
@@ -64,6 +121,10 @@ struct SpanBinArgs {
   uint32_t max_view_rows = 0;    // LDS rows of a level-1 block (<= kMaxBinTilesY)
   uint32_t pair_cap = 0;         // capacity of the span, group-span and pvals arrays
   LiveMap live;                  // only open tiles receive entries
+  // single view only: the item count on the device (items of `one` are then
+  // min(*d_items, one.n)) and ceil(items / 256); null: one.n items
+  const uint32_t *d_items = nullptr;
+  const uint32_t *d_nb1 = nullptr;
 };
 
 // Per-tile lists of every virtual tile: d_off[0..rows*cols] (lower-bound
@@ -75,8 +136,11 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
 
 // One view: ids[0..cn) are the chunk's splat ids in depth order, rec the
 // projected records indexed by splat id; npairs bounds the chunk's (tile,
-// splat) pairs.
+// splat) pairs. With a live map and rect8 (the packed rects of all n_all
+// splats in id order), only splats with a live tile in some row of their
+// rect enter level 1 (a stable compaction); the lists are the same.
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
-              const LiveMap &live, BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small);
+              const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
+              uint32_t *pvals, uint32_t *d_small);
 
 }  // namespace dl

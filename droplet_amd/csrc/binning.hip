@@ -35,39 +35,6 @@ constexpr int kW1 = kB1 / 32;            // mask words per row
 constexpr int kChunk = 1024;             // items per MBS chunk: 4 waves x 4 steps x 64 lanes
 constexpr int kK = 16;                   // buckets per MBS segment
 
-struct Rect {
-  uint32_t x0, x1, y0;
-  uint32_t nrows;               // rows y0 .. y0+nrows-1 (0: no pairs)
-  uint32_t full_rows;           // rows y0 .. y0+full_rows-1 span [x0, x1]
-  uint32_t rem;                 // row y0+full_rows spans [x0, x0+rem-1] when rem > 0
-};
-
-// tile rectangle and maxCoverage truncation of one projected splat, with the
-// exact arithmetic of project.ts:275-283 / emit-pairs.ts:164-181
-__device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov) {
-  Rect R{0, 0, 0, 0, 0, 0};
-  const float sx = v0.x, sy = v0.y, radius = v0.z;
-  if (!(radius > 0.0f)) return R;
-  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
-  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
-  const uint32_t max_tx = u32sat(wmin((float)tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
-  const uint32_t max_ty = u32sat(wmin((float)tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
-  if (max_tx < min_tx || max_ty < min_ty) return R;
-  const uint32_t w = max_tx - min_tx + 1u, h = max_ty - min_ty + 1u;
-  R.x0 = min_tx;
-  R.x1 = max_tx;
-  R.y0 = min_ty;
-  if ((uint64_t)w * h <= max_cov) {
-    R.full_rows = h;
-    R.rem = 0;
-  } else {
-    R.full_rows = max_cov / w;
-    R.rem = max_cov - R.full_rows * w;
-  }
-  R.nrows = R.full_rows + (R.rem > 0 ? 1u : 0u);
-  return R;
-}
-
 // Depth-ordered splat records of one chunk: (id, x0 | x1 << 16, y0 |
 // full_rows << 16, rem). One gather of the id-ordered projection records, so
 // that both level-1 passes read coalesced 16-byte records.
@@ -102,6 +69,139 @@ __device__ inline bool live_range(const LiveMap &L, uint32_t y, uint32_t xa, uin
   return r[xb + 1] != r[xa];
 }
 
+// ------------------------ live-splat compaction ----------------------------
+// In a pass after the first, a splat enters level 1 only if some row of its
+// rect holds a live tile in the row's column span: exactly the splats that
+// would give level 1 at least one span. Removing the others from the
+// depth-ordered input (a stable compaction) leaves every list unchanged.
+
+// Summed-area table of the open tiles from the row prefixes: sat[y * (cols
+// + 1) + x] = open tiles in rows < y and columns < x, for y <= rows, x <=
+// cols. One thread per column accumulates down the rows.
+__global__ __launch_bounds__(256) void k_open_sat(const uint16_t *__restrict__ pre, uint32_t rows, uint32_t cols,
+                                                  uint32_t *__restrict__ sat) {
+  const uint32_t stride = cols + 1;
+  for (uint32_t x = threadIdx.x; x <= cols; x += 256) {
+    uint32_t acc = 0;
+    sat[x] = 0;
+#pragma unroll 8
+    for (uint32_t y = 0; y < rows; y++) {
+      acc += pre[(size_t)y * stride + x];
+      sat[(size_t)(y + 1) * stride + x] = acc;
+    }
+  }
+}
+
+// The compaction kernels give each thread kLiveItems items (one per 256-item
+// sub-block, loads issued together): the per-item work is a few loads, so
+// one item per thread leaves them latency-bound.
+constexpr int kLiveItems = 8;
+
+// bit i of bits: splat i (id order, packed rects of the projection) is live:
+// an open tile in its full rows (the summed-area table) or in its partial
+// last row (the row prefix)
+__global__ __launch_bounds__(256) void k_live_bits(const uint2 *__restrict__ rect8, uint32_t n, LiveMap live,
+                                                   unsigned long long *__restrict__ bits) {
+  const size_t base = (size_t)blockIdx.x * 256 * kLiveItems + threadIdx.x;
+  const uint32_t st = live.stride;
+  uint2 r[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const size_t i = base + (size_t)k * 256;
+    r[k] = i < n ? rect8[i] : make_uint2(0u, 0u);
+  }
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const size_t i = base + (size_t)k * 256;
+    const Rect R = unpack_rect8(r[k]);
+    bool lv = false;
+    if (R.full_rows > 0) {
+      const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
+      const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
+                            live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
+      lv = open != 0;
+    }
+    if (!lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
+    const unsigned long long b = __ballot(lv);
+    const size_t first = i - (size_t)__lane_id();
+    if (__lane_id() == 0 && first < n) bits[first >> 6] = b;
+  }
+}
+
+__device__ inline bool live_bit(const unsigned long long *__restrict__ bits, uint32_t id) {
+  return (bits[id >> 6] >> (id & 63u)) & 1ull;
+}
+
+// live splats per 256-item sub-block of the pass (nsub = ceil(cn / 256))
+__global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__ ids, uint32_t cn,
+                                                    const unsigned long long *__restrict__ bits,
+                                                    uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t s_w[kLiveItems][4];
+  const uint32_t sub0 = blockIdx.x * kLiveItems;
+  uint32_t id[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const uint32_t jj = (sub0 + k) * 256 + threadIdx.x;
+    id[k] = jj < cn ? ids[jj] : 0xFFFFFFFFu;
+  }
+  unsigned long long w[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const bool lv = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
+    const unsigned long long b = __ballot(lv);
+    if (__lane_id() == 0) s_w[k][threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  }
+  __syncthreads();
+  const uint32_t nsub = (cn + 255) / 256;
+  if (threadIdx.x < kLiveItems && sub0 + threadIdx.x < nsub) {
+    const uint32_t k = threadIdx.x;
+    cnt[sub0 + k] = s_w[k][0] + s_w[k][1] + s_w[k][2] + s_w[k][3];
+  }
+}
+
+// the pass's live splats as packed depth-ordered rects, compacted: sub-block
+// q's live items start at off[q]; also live_small[1] = level-1 blocks of
+// the live_small[0] live items
+__global__ __launch_bounds__(256) void k_rects_live(const uint2 *__restrict__ rect8, const uint32_t *__restrict__ ids,
+                                                    uint32_t cn, const unsigned long long *__restrict__ bits,
+                                                    const uint32_t *__restrict__ off, uint32_t *__restrict__ live_small,
+                                                    uint4 *__restrict__ rects) {
+  __shared__ uint32_t s_w[kLiveItems][4];
+  const uint32_t sub0 = blockIdx.x * kLiveItems;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  uint32_t id[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const uint32_t jj = (sub0 + k) * 256 + threadIdx.x;
+    id[k] = jj < cn ? ids[jj] : 0xFFFFFFFFu;
+  }
+  unsigned long long w[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
+  unsigned long long bal[kLiveItems];
+  uint2 r[kLiveItems];
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    const bool lv = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
+    bal[k] = __ballot(lv);
+    r[k] = lv ? rect8[id[k]] : make_uint2(0u, 0u);
+    if (lane == 0) s_w[k][wave] = (uint32_t)__popcll(bal[k]);
+  }
+  __syncthreads();
+  const uint32_t nsub = (cn + 255) / 256;
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++) {
+    if (sub0 + k >= nsub || !((bal[k] >> lane) & 1ull)) continue;
+    uint32_t pos = off[sub0 + k] + (uint32_t)__popcll(bal[k] & below);
+    for (int q = 0; q < wave; q++) pos += s_w[k][q];
+    rects[pos] = pack_rect(id[k], unpack_rect8(r[k]));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) live_small[1] = (live_small[0] + kB1 - 1) / kB1;
+}
+
 // The level-1 view that owns level-1 block b: the single view `one`, or a
 // binary search of the batch table by first block.
 __device__ inline L1View l1_view(const L1View &one, const L1View *__restrict__ tab, uint32_t ntab, uint32_t b) {
@@ -128,14 +228,25 @@ __global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ 
 // Level-1 block b of view v covers items [item0 + 256 k, ...) with k = b -
 // blk0, and rows [row0, row0 + rows); its count of row r sits at c1_base +
 // (r - row0) * nb1 + k, so the flat scan orders (view, row, block).
+// With d_items (single view) the view's item count is read on the device:
+// the grid covers the capacity and blocks past the count exit at once.
+__device__ inline L1View l1_view_dev(const L1View &one, const L1View *__restrict__ tab, uint32_t ntab, uint32_t b,
+                                     const uint32_t *__restrict__ d_items) {
+  L1View v = l1_view(one, tab, ntab, b);
+  if (d_items) v.n = min(*d_items, v.n);
+  return v;
+}
+
 __global__ __launch_bounds__(kB1) void k_rows_count(const uint4 *__restrict__ rects, L1View one,
                                                     const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
+                                                    const uint32_t *__restrict__ d_items,
                                                     uint32_t *__restrict__ c1) {
   extern __shared__ uint32_t s_mask[];  // [rows][kW1]
-  const L1View v = l1_view(one, tab, ntab, blockIdx.x);
+  const L1View v = l1_view_dev(one, tab, ntab, blockIdx.x, d_items);
   const uint32_t t = threadIdx.x;
   const uint32_t k_blk = blockIdx.x - v.blk0;
   const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  if (k_blk >= nb1) return;
   const uint32_t jj = k_blk * kB1 + t;
   for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
   __syncthreads();
@@ -158,16 +269,18 @@ __global__ __launch_bounds__(kB1) void k_rows_count(const uint4 *__restrict__ re
 
 __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ rects, L1View one,
                                                       const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
+                                                      const uint32_t *__restrict__ d_items,
                                                       const uint32_t *__restrict__ base1, uint32_t cap,
                                                       uint2 *__restrict__ spans) {
   extern __shared__ uint32_t s_dyn[];
-  const L1View v = l1_view(one, tab, ntab, blockIdx.x);
+  const L1View v = l1_view_dev(one, tab, ntab, blockIdx.x, d_items);
   uint32_t *s_mask = s_dyn;                                                  // [rows][kW1]
   uint32_t *s_base = s_dyn + v.rows * kW1;                                   // [rows]
   uint16_t *s_pre = reinterpret_cast<uint16_t *>(s_base + v.rows);           // [rows][kW1]
   const uint32_t t = threadIdx.x;
   const uint32_t k_blk = blockIdx.x - v.blk0;
   const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  if (k_blk >= nb1) return;
   const uint32_t jj = k_blk * kB1 + t;
   for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
   for (uint32_t y = t; y < v.rows; y += kB1) s_base[y] = base1[(size_t)v.c1_base + (size_t)y * nb1 + k_blk];
@@ -215,6 +328,7 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ 
 // row_start[rows] = all spans; clamped to the span capacity cap
 __global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, size_t n1, L1View one,
                                                    const L1View *__restrict__ tab, uint32_t ntab, uint32_t rows,
+                                                   const uint32_t *__restrict__ d_items,
                                                    const uint32_t *__restrict__ d_num_spans, uint32_t cap,
                                                    uint32_t *__restrict__ row_start) {
   const uint32_t r = blockIdx.x * 256 + threadIdx.x;
@@ -230,9 +344,12 @@ __global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ 
     }
     v = tab[lo];
   }
+  if (d_items) v.n = min(*d_items, v.n);
   const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
+  // with a device count the level-1 counts fill rows * nb1 entries
+  const size_t n1e = d_items ? (size_t)v.rows * nb1 : n1;
   const size_t idx = (size_t)v.c1_base + (size_t)(r - v.row0) * nb1;
-  row_start[r] = idx < n1 ? min(base1[idx], cap) : total;
+  row_start[r] = idx < n1e ? min(base1[idx], cap) : total;
 }
 
 // ----------------------- MBS chunk tables ---------------------------------
@@ -439,6 +556,50 @@ __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restri
   out[o] = pos < ncounts ? min(base[pos], total) : total;
 }
 
+// The whole chunk table of a level with at most kSmallSegs segments in one
+// 1024-thread workgroup: each thread owns 4 consecutive segments.
+constexpr uint32_t kSmallSegs = 4096;
+
+__global__ __launch_bounds__(1024) void k_chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg,
+                                                            uint32_t cap, uint32_t *__restrict__ chunk_base,
+                                                            uint32_t *__restrict__ seg_of) {
+  __shared__ uint32_t s_wave[16];
+  const uint32_t t = threadIdx.x, s0 = 4 * t;
+  uint32_t nch[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t sg = s0 + k;
+    nch[k] = sg < nseg ? (seg_start[sg + 1] - seg_start[sg] + kChunk - 1) / kChunk : 0u;
+    sum += nch[k];
+  }
+  // exclusive scan of the per-thread sums over the 16 waves
+  const int lane = __lane_id(), wave = t >> 6;
+  uint32_t inc = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  uint32_t run = inc - sum, total = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wave) run += s_wave[w];
+    total += s_wave[w];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t sg = s0 + k;
+    if (sg < nseg) {
+      chunk_base[sg] = run;
+      const uint32_t e = min(run + nch[k], cap);
+      for (uint32_t c = run; c < e; c++) seg_of[c] = sg;
+    }
+    run += nch[k];
+  }
+  if (t == 0) chunk_base[nseg] = total;
+}
+
 // Chunk table of one MBS level, built on the device: chunks per segment,
 // their scan (chunk_base, with the total in chunk_base[nseg]) and chunk ->
 // segment. max_chunks bounds the total: every segment of len items has
@@ -448,6 +609,12 @@ void build_chunk_table(hipStream_t s, const uint32_t *seg_start, uint32_t nseg, 
   tmp.reserve((size_t)nseg * 4 + 4);
   chunk_base.reserve(((size_t)nseg + 1) * 4);
   seg_of.reserve((size_t)std::max<uint32_t>(max_chunks, 1) * 4);
+  if (nseg <= kSmallSegs) {
+    k_chunk_table_small<<<1, 1024, 0, s>>>(seg_start, nseg, max_chunks, chunk_base.as<uint32_t>(),
+                                           seg_of.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+    return;
+  }
   scan_ws.reserve(scan_workspace_bytes(nseg));
   k_seg_chunks<<<div_up(nseg, 256), 256, 0, s>>>(seg_start, nseg, tmp.as<uint32_t>());
   DL_HIP(hipGetLastError());
@@ -482,20 +649,26 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   ws.scan_ws.reserve(scan_workspace_bytes(n1));
   if (a.l1_blocks) {
     k_rows_count<<<a.l1_blocks, kB1, (size_t)a.max_view_rows * kW1 * 4, s>>>(a.rects, a.one, a.tab, a.ntab, a.live,
-                                                                            ws.c1.as<uint32_t>());
+                                                                            a.d_items, ws.c1.as<uint32_t>());
     DL_HIP(hipGetLastError());
   }
-  exclusive_scan_u32(s, ws.c1.as<uint32_t>(), ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, d_num_spans,
-                     ws.scan_ws.get());
+  if (a.d_items) {
+    // single view with a device item count: rows * ceil(items / 256) counts
+    exclusive_scan_u32_dev(s, ws.c1.as<uint32_t>(), ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, a.d_nb1,
+                           a.one.rows, d_num_spans, ws.scan_ws.get());
+  } else {
+    exclusive_scan_u32(s, ws.c1.as<uint32_t>(), ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, d_num_spans,
+                       ws.scan_ws.get());
+  }
   if (a.l1_blocks) {
     const size_t lds1 = (size_t)a.max_view_rows * (kW1 * 4 + 4 + kW1 * 2);
-    k_rows_scatter<<<a.l1_blocks, kB1, lds1, s>>>(a.rects, a.one, a.tab, a.ntab, a.live, ws.base1.as<uint32_t>(), cap,
-                                                  ws.spans.as<uint2>());
+    k_rows_scatter<<<a.l1_blocks, kB1, lds1, s>>>(a.rects, a.one, a.tab, a.ntab, a.live, a.d_items,
+                                                  ws.base1.as<uint32_t>(), cap, ws.spans.as<uint2>());
     DL_HIP(hipGetLastError());
   }
   ws.row_start.reserve(((size_t)a.rows + 1) * 4);
   k_row_start<<<div_up((size_t)a.rows + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, a.one,
-                                                              a.tab, a.ntab, a.rows, d_num_spans, cap,
+                                                              a.tab, a.ntab, a.rows, a.d_items, d_num_spans, cap,
                                                               ws.row_start.as<uint32_t>());
   DL_HIP(hipGetLastError());
 
@@ -555,10 +728,36 @@ void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1
 }
 
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
-              const LiveMap &live, BinWorkspace &ws, uint32_t *d_off, uint32_t *pvals, uint32_t *d_small) {
+              const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
+              uint32_t *pvals, uint32_t *d_small) {
   ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
-  if (cn) {
-    k_rects<<<div_up(cn, 256), 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());
+  const bool compact = live.pre && rect8 && cn > 0;
+  const uint32_t nblk = div_up(cn, 256);
+  if (compact) {
+    // live splats only, in depth order
+    ws.live_bits.reserve(((size_t)div_up(n_all, 64) + 1) * 8);
+    ws.live_cnt.reserve((size_t)nblk * 4);
+    ws.live_off.reserve((size_t)nblk * 4);
+    ws.live_small.reserve(256);
+    ws.scan_ws.reserve(scan_workspace_bytes(nblk));
+    uint32_t *ls = ws.live_small.as<uint32_t>();
+    ws.live_sat.reserve(((size_t)g.tiles_y + 1) * (g.tiles_x + 1) * 4);
+    k_open_sat<<<1, 256, 0, s>>>(live.pre, g.tiles_y, g.tiles_x, ws.live_sat.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+    LiveMap lm = live;
+    lm.sat = ws.live_sat.as<uint32_t>();
+    k_live_bits<<<div_up(n_all, 256 * kLiveItems), 256, 0, s>>>(rect8, n_all, lm,
+                                                                ws.live_bits.as<unsigned long long>());
+    DL_HIP(hipGetLastError());
+    const uint32_t grid = div_up(nblk, kLiveItems);
+    k_live_count<<<grid, 256, 0, s>>>(ids, cn, ws.live_bits.as<unsigned long long>(), ws.live_cnt.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+    exclusive_scan_u32(s, ws.live_cnt.as<uint32_t>(), ws.live_off.as<uint32_t>(), nblk, ls, ws.scan_ws.get());
+    k_rects_live<<<grid, 256, 0, s>>>(rect8, ids, cn, ws.live_bits.as<unsigned long long>(),
+                                      ws.live_off.as<uint32_t>(), ls, ws.rects.as<uint4>());
+    DL_HIP(hipGetLastError());
+  } else if (cn) {
+    k_rects<<<nblk, 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());
     DL_HIP(hipGetLastError());
   }
   SpanBinArgs a;
@@ -573,6 +772,10 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
   a.cols = g.tiles_x;
   a.pair_cap = npairs;
   a.live = live;
+  if (compact) {
+    a.d_items = ws.live_small.as<uint32_t>();
+    a.d_nb1 = ws.live_small.as<uint32_t>() + 1;
+  }
   span_bin_rects(s, a, ws, d_off, pvals, d_small);
 }
 

@@ -72,6 +72,8 @@ struct Renderer {
   Scene own_scene;
   // per splat (id order): depth keys/ids, projected records, tile counts
   DeviceBuffer keys, ids, tkeys, tids, sort_ws, rec, counts, parts;
+  // per splat (id order): packed tile rect for span binning's later passes
+  DeviceBuffer rect8;
   // per chunk
   DeviceBuffer chunk_counts, offsets, scan_ws, ranges;
   // pairs

@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
                                                              double far_plane, uint32_t *__restrict__ keys,
                                                              uint32_t *__restrict__ ids, float4 *__restrict__ rec,
                                                              uint32_t *__restrict__ counts,
+                                                             uint2 *__restrict__ rect8,
                                                              uint32_t *__restrict__ part_vis,
                                                              unsigned long long *__restrict__ part_pairs) {
   __shared__ uint32_t s_vis[kProjBlock / 64];
@@ -49,7 +50,11 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
   const uint32_t i = blockIdx.x * kProjBlock + threadIdx.x;
   bool vis = false;
   uint32_t count = 0;
-  if (i < sc.count) cull_project_one<BANDS>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
+  if (i < sc.count) {
+    cull_project_one<BANDS>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
+    // packed tile rect for the live-splat compaction of span binning
+    if (rect8) rect8[i] = pack_rect8(splat_rect(rec[3 * (size_t)i], u.tiles_x, u.tiles_y, u.max_cov));
+  }
   uint32_t v = vis ? 1u : 0u;
   unsigned long long pc = count;
 #pragma unroll
@@ -185,7 +190,7 @@ void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float
 }
 
 using CullProjFn = void (*)(SceneView, ProjUniforms, double, double, uint32_t *, uint32_t *, float4 *, uint32_t *,
-                            uint32_t *, unsigned long long *);
+                            uint2 *, uint32_t *, unsigned long long *);
 
 CullProjFn cull_project_for(uint32_t bands) {
   switch (bands) {
@@ -302,6 +307,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   uint32_t nvis = 0;
   unsigned long long pairs_all = 0;
   const uint32_t *sorted_ids = nullptr;
+  // packed rects (id order) for the live-splat compaction of later passes
+  const bool want_rect8 = opts.binning == 0 && span_binning_supported(tiles_x, tiles_y) && !keep_taps &&
+                          !train_mode && !opts.full_lists;
   const bool speculative = !force_exact && spec_cap > 0 && n > 0 && !keep_taps && !train_mode &&
                            opts.binning == 0 && span_binning_supported(tiles_x, tiles_y) &&
                            (opts.chunk_cap == 0 || opts.chunk_cap >= n);
@@ -312,6 +320,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     tkeys.reserve((size_t)n * 4);
     tids.reserve((size_t)n * 4);
     rec.reserve((size_t)n * 48);
+    if (want_rect8) rect8.reserve((size_t)n * 8);
     counts.reserve((size_t)n * 4);
     parts.reserve((size_t)nparts * 12 + 256);
     sort_ws.reserve(radix_workspace_bytes(n, 32));
@@ -321,7 +330,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     mark(DL_STAGE_CULL);
     cull_project_for(sc.bands)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
                                                              keys.as<uint32_t>(), ids.as<uint32_t>(), rec.as<float4>(),
-                                                             counts.as<uint32_t>(), part_vis, part_pairs);
+                                                             counts.as<uint32_t>(),
+                                                             want_rect8 ? rect8.as<uint2>() : nullptr, part_vis,
+                                                             part_pairs);
     DL_HIP(hipGetLastError());
     k_reduce_totals<<<1, 1024, 0, s>>>(part_vis, part_pairs, nparts, d_nvis, d_pairs_all);
     DL_HIP(hipGetLastError());
@@ -457,8 +468,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         // stable two-level bucketing straight into per-tile lists (binning.hip)
         mark(DL_STAGE_EMIT);
         const LiveMap live = (track_closed && c > 0) ? LiveMap{open_pre.as<uint16_t>(), tiles_x + 1} : LiveMap{};
-        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, live, bin_ws,
-                 ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin);
+        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, live,
+                 want_rect8 ? rect8.as<uint2>() : nullptr, n, bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(),
+                 d_bin);
         k_add_entries<<<1, 64, 0, s>>>(d_bin + 2, d_entries);
         DL_HIP(hipGetLastError());
       } else {
