@@ -37,7 +37,7 @@ constexpr int kProjBlock = 256;
 // One thread per splat in id order, so every scene array is read coalesced.
 // Per-block partial sums of visible splats and tile counts replace any
 // same-address atomics.
-template <int BANDS>
+template <int BANDS, bool COLOR>
 __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjUniforms u, double near_plane,
                                                              double far_plane, uint32_t *__restrict__ keys,
                                                              uint32_t *__restrict__ ids, float4 *__restrict__ rec,
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
   bool vis = false;
   uint32_t count = 0;
   if (i < sc.count) {
-    cull_project_one<BANDS>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
+    cull_project_one<BANDS, COLOR>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
     // packed tile rect for the live-splat compaction of span binning
     if (rect8) rect8[i] = pack_rect8(splat_rect(rec[3 * (size_t)i], u.tiles_x, u.tiles_y, u.max_cov));
   }
@@ -192,13 +192,24 @@ void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float
 using CullProjFn = void (*)(SceneView, ProjUniforms, double, double, uint32_t *, uint32_t *, float4 *, uint32_t *,
                             uint2 *, uint32_t *, unsigned long long *);
 
-CullProjFn cull_project_for(uint32_t bands) {
+CullProjFn cull_project_for(uint32_t bands, bool color) {
   switch (bands) {
-    case 0: return k_cull_project<0>;
-    case 1: return k_cull_project<1>;
-    case 2: return k_cull_project<2>;
-    default: return k_cull_project<3>;
+    case 0: return color ? k_cull_project<0, true> : k_cull_project<0, false>;
+    case 1: return color ? k_cull_project<1, true> : k_cull_project<1, false>;
+    case 2: return color ? k_cull_project<2, true> : k_cull_project<2, false>;
+    default: return color ? k_cull_project<3, true> : k_cull_project<3, false>;
   }
+}
+
+using ColorFn = void (*)(SceneView, const uint32_t *, const uint4 *, uint32_t, const uint32_t *, float4 *);
+
+void launch_colors(hipStream_t s, const SceneView &sc, const uint32_t *ids, const uint4 *rects, uint32_t n_cap,
+                   const uint32_t *d_n, float4 *rec) {
+  if (n_cap == 0) return;
+  ColorFn f = sc.bands == 0 ? k_splat_colors<0> : sc.bands == 1 ? k_splat_colors<1>
+            : sc.bands == 2 ? k_splat_colors<2> : k_splat_colors<3>;
+  f<<<div_up(n_cap, 256), 256, 0, s>>>(sc, ids, rects, n_cap, d_n, rec);
+  DL_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -313,6 +324,15 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   const bool speculative = !force_exact && spec_cap > 0 && n > 0 && !keep_taps && !train_mode &&
                            opts.binning == 0 && span_binning_supported(tiles_x, tiles_y) &&
                            (opts.chunk_cap == 0 || opts.chunk_cap >= n);
+  // A speculative frame that will run in passes leaves the colours out of the
+  // projection; each pass computes them for the splats that reach its tile
+  // lists (all splats of the first pass, the live ones of later passes).
+  bool defer_color = false;
+  if (speculative && !opts.full_lists &&
+      (opts.first_pass || opts.pass_growth || last_pairs_all >= kPassMinPairs)) {
+    const uint64_t len = opts.first_pass ? opts.first_pass : std::max<uint64_t>(kPassMinFirst, n / kPassFirstDiv);
+    defer_color = len < n;
+  }
   if (n > 0) {
     const uint32_t nparts = div_up(n, kProjBlock);
     keys.reserve((size_t)n * 4);
@@ -328,7 +348,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     unsigned long long *part_pairs =
         reinterpret_cast<unsigned long long *>(parts.as<uint8_t>() + (((size_t)nparts * 4 + 255) & ~(size_t)255));
     mark(DL_STAGE_CULL);
-    cull_project_for(sc.bands)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
+    cull_project_for(sc.bands, !defer_color)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
                                                              keys.as<uint32_t>(), ids.as<uint32_t>(), rec.as<float4>(),
                                                              counts.as<uint32_t>(),
                                                              want_rect8 ? rect8.as<uint2>() : nullptr, part_vis,
@@ -473,6 +493,14 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
                  d_bin);
         k_add_entries<<<1, 64, 0, s>>>(d_bin + 2, d_entries);
         DL_HIP(hipGetLastError());
+        if (defer_color) {
+          mark(DL_STAGE_PROJECT);
+          if (c == 0)
+            launch_colors(s, sc, cids, nullptr, cn, nullptr, rec.as<float4>());
+          else
+            launch_colors(s, sc, nullptr, bin_ws.rects.as<uint4>(), cn, bin_ws.live_small.as<uint32_t>(),
+                          rec.as<float4>());
+        }
       } else {
         pkeys.reserve((size_t)npairs * 4);
         ptkeys.reserve((size_t)npairs * 4);

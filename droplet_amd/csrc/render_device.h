@@ -77,9 +77,43 @@ __device__ inline void eval_sh(const float *__restrict__ sh, float x, float y, f
   out[2] = b;
 }
 
+// Camera-space position of a splat centre (project.ts:111-113).
+__device__ inline void cam_pos(const float *m, float px, float py, float pz, float &pcx, float &pcy, float &pcz) {
+  pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
+  pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
+  pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
+}
+
+// Colour of a splat (project.ts:253-266): DC + view-dependent SH along the
+// camera-space direction, + 0.5, clamped at 0.
+template <int BANDS>
+__device__ inline void color_from(const float *dc, const float *shc, float pcx, float pcy, float pcz, float col[3]) {
+  col[0] = dc[0];
+  col[1] = dc[1];
+  col[2] = dc[2];
+  if constexpr (BANDS > 0) {
+    const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
+    float sh[3];
+    eval_sh<BANDS>(shc, pcx / len, pcy / len, pcz / len, sh);
+    col[0] = col[0] + sh[0];
+    col[1] = col[1] + sh[1];
+    col[2] = col[2] + sh[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
+}
+
+template <int BANDS>
+__device__ inline void splat_color(const SceneView &sc, uint32_t id, float pcx, float pcy, float pcz, float col[3]) {
+  constexpr int M = BANDS == 0 ? 0 : BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
+  color_from<BANDS>(sc.dc + 3 * (size_t)id, sc.sh + (size_t)id * 3 * M, pcx, pcy, pcz, col);
+}
+
 // One splat through project.ts:111-285. Returns the tile count; a culled
 // splat gets slot 0 = (0, 0, -1, 0) and count 0 (project.ts:216-219, 247-250).
-template <int BANDS>
+// Without COLOR the colour lanes of slot 2 stay 0: k_splat_colors fills them
+// later for the splats that reach a tile list.
+template <int BANDS, bool COLOR = true>
 __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const ProjUniforms &u, float4 &o0,
                                          float4 &o1, float4 &o2) {
   const float *m = u.view;
@@ -94,9 +128,8 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
   const float s2 = dl_expf(sc.ls[3 * (size_t)id + 2]);
   const float logit = sc.op[id];
 
-  const float pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
-  const float pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
-  const float pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
+  float pcx, pcy, pcz;
+  cam_pos(m, px, py, pz, pcx, pcy, pcz);
   const float depth = -pcz;
   float sx, sy;
   if (u.proj == DL_PINHOLE) {
@@ -179,18 +212,10 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
   const float fade = wclamp((fade_e - radius_raw) / (fade_e - fade_s), 0.0f, 1.0f);
   if (fade <= 0.0f) return 0;
   const float radius = __builtin_ceilf(wmin(radius_raw, fade_e));
-  float col[3] = {sc.dc[3 * (size_t)id], sc.dc[3 * (size_t)id + 1], sc.dc[3 * (size_t)id + 2]};
-  if constexpr (BANDS > 0) {
-    constexpr int M = BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
-    const float len = __builtin_sqrtf((pcx * pcx + pcy * pcy) + pcz * pcz);
-    float sh[3];
-    eval_sh<BANDS>(sc.sh + (size_t)id * 3 * M, pcx / len, pcy / len, pcz / len, sh);
-    col[0] = col[0] + sh[0];
-    col[1] = col[1] + sh[1];
-    col[2] = col[2] + sh[2];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; k++) col[k] = wmax(col[k] + 0.5f, 0.0f);
+  // without COLOR the colour lanes carry the camera-space position for
+  // k_splat_colors
+  float col[3] = {pcx, pcy, pcz};
+  if constexpr (COLOR) splat_color<BANDS>(sc, id, pcx, pcy, pcz, col);
   const float alpha = (dl_sigmoidf(logit) * fade) * aa;
   // Rasterizer skip threshold in the record's padding lane (0 in the
   // reference, project.ts:272): for power < tau the reference's pixel alpha,
@@ -222,7 +247,7 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
 // f64 cull (camera.ts:56-99) and sortable depth key of splat i, then
 // project() when it is visible; the outputs go to slot o of keys / vals /
 // rec / counts (counts may be null), with vals[o] = val.
-template <int BANDS>
+template <int BANDS, bool COLOR = true>
 __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms &u, double near_plane,
                                         double far_plane, uint32_t i, size_t o, uint32_t val,
                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
@@ -257,11 +282,33 @@ __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms 
   vals[o] = val;
   float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f), o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o2 = o1;
   count = 0;
-  if (vis) count = project_splat<BANDS>(sc, i, u, o0, o1, o2);
+  if (vis) count = project_splat<BANDS, COLOR>(sc, i, u, o0, o1, o2);
   rec[3 * o + 0] = o0;
   rec[3 * o + 1] = o1;
   rec[3 * o + 2] = o2;
   if (counts) counts[o] = count;
+}
+
+// Colours of the splats of one pass, computed after a projection without
+// COLOR with the same arithmetic, from the camera-space position that the
+// projection left in the colour lanes: item j < n names splat ids[j] (or
+// rects[j].x), n = the static n_cap or min(n_cap, *d_n). The colours of
+// culled splats (in no tile list) are meaningless and never read.
+template <int BANDS>
+__global__ __launch_bounds__(256) void k_splat_colors(SceneView sc, const uint32_t *__restrict__ ids,
+                                                      const uint4 *__restrict__ rects, uint32_t n_cap,
+                                                      const uint32_t *__restrict__ d_n, float4 *__restrict__ rec) {
+  const uint32_t n = d_n ? min(*d_n, n_cap) : n_cap;
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t id = ids ? ids[j] : rects[j].x;
+  float *slot = reinterpret_cast<float *>(rec + 3 * (size_t)id + 2);
+  const float pcx = slot[0], pcy = slot[1], pcz = slot[2];
+  float col[3];
+  splat_color<BANDS>(sc, id, pcx, pcy, pcz, col);
+  slot[0] = col[0];
+  slot[1] = col[1];
+  slot[2] = col[2];
 }
 
 // ---------------------------------------------------------------------------

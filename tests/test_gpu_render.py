@@ -299,6 +299,33 @@ def test_front_to_back_passes_match_oracle(device, first_pass, growth):
         r.destroy()
 
 
+@pytest.mark.parametrize("bands", [0, 1, 3])
+def test_repeated_frames_match_oracle(device, bands):
+    """The first frame of a renderer reads the visible and pair totals after
+    the depth sort; later frames of the same size do not (capacity from the
+    earlier frames, checked at the end) and, in passes, compute colours only
+    for the splats that reach a tile list. Every frame equals the oracle, and
+    a scene with more pairs than the capacity is rendered again exactly."""
+    data = S.gaussian_cloud(80_000, seed=70 + bands, sh_bands=bands, pos_std=0.75, pos_clip=3.0)
+    cam = S.default_camera(640, 360)
+    big = S.gaussian_cloud(160_000, seed=80 + bands, sh_bands=bands, pos_std=0.75, pos_clip=3.0)
+    r = Renderer(device, RendererOptions(first_pass=8_000, pass_growth=4))
+    try:
+        ref = oracle.render(data, cam, taps=False)
+        for frame in range(3):
+            img = r.render(data, cam)
+            st = r.stats()
+            print(f"case: bands={bands} frame={frame} passes={st['num_chunks']}", flush=True)
+            np.testing.assert_array_equal(img, ref["rgba"])
+            assert st["num_pairs"] == ref["num_pairs"]
+            assert st["num_chunks"] > 1
+        img = r.render(big, cam)
+        np.testing.assert_array_equal(img, oracle.render(big, cam, taps=False)["rgba"])
+        assert r.stats()["num_pairs"] > ref["num_pairs"]
+    finally:
+        r.destroy()
+
+
 def test_front_to_back_passes_skip_closed_tiles(device):
     """On a dense scene many (tile, splat) pairs lie behind saturated pixels:
     the passes build fewer list entries than the full lists hold (tiles at
