@@ -129,16 +129,18 @@ using namespace dl;
 // ---- autotuner --------------------------------------------------------------
 namespace {
 
-// Candidate options around `base`: radix digit widths 8 / 6 / 4 and pass
-// schedules (full lists; first pass n/16 growth 4, n/8 growth 8, n/4 growth
-// 8; the default rule), n = the largest scene.
+// Candidate options around `base`: radix digit widths 8 / 6 (4-bit digits,
+// eight passes, never won on MI355X) and pass schedules (full lists; first
+// pass n/32 .. n/4 with growth 4, 8 or 16; the default rule), n = the
+// largest scene.
 std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint64_t n) {
   std::vector<dl_renderer_opts> out;
-  const uint32_t digits[] = {8, 6, 4};
+  const uint32_t digits[] = {8, 6};
   struct Sched {
     uint32_t div, growth, full;
   };
-  const Sched scheds[] = {{0, 0, 0}, {0, 0, 1}, {16, 4, 0}, {8, 8, 0}, {4, 8, 0}};
+  const Sched scheds[] = {{0, 0, 0},  {0, 0, 1},  {32, 8, 0}, {16, 4, 0}, {16, 8, 0},
+                          {12, 8, 0}, {8, 8, 0},  {8, 16, 0}, {4, 8, 0}};
   for (uint32_t d : digits)
     for (const Sched &sc : scheds) {
       dl_renderer_opts o = base;
