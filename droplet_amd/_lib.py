@@ -13,7 +13,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdroplet_hip.so")
+# the in-tree build; DROPLET_HIP_LIB names another build of the same library
+LIB_PATH = os.environ.get("DROPLET_HIP_LIB") or os.path.join(HERE, "libdroplet_hip.so")
 
 DL_OK = 0
 DL_ERR_INVALID = 1

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(64) void k_backward_raster(const float4 *__restrict
         const float dy = pyf[k] - a.y;
         const float r = a.z;
         if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
-        const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
+        const float power = gauss_power(b.x, b.y, b.z, dx, dy);
         if (power > 0.0f || power < col.w) continue;
         const float e = dl_expf_nonpos(power);
         const float raw = b.w * wmax(0.0f, e - kGaussFloor);

@@ -333,6 +333,15 @@ __device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, cons
   return R | (G << 8) | (B << 16) | (A << 24);
 }
 
+// The Gaussian exponent of rasterize.ts:70, -0.5 (A dx^2 + 2B dx dy + C dy^2),
+// with the two contractions the specification fixes (WGSL lets a compiler
+// fuse a * b + c): fma(C dy, dy, fma(2B dx, dy, A dx dx)). The colour
+// accumulation is the other fused expression. Both the backward's forward
+// recomputation and the oracle use exactly this form.
+__device__ inline float gauss_power(float A, float B, float C, float dx, float dy) {
+  return (-0.5f) * __builtin_fmaf(C * dy, dy, __builtin_fmaf((2.0f * B) * dx, dy, (A * dx) * dx));
+}
+
 // Can a pixel centre of the rectangle [x0, x1] x [y0, y1] pass the
 // reference's r-box test with power >= tau? Evaluated in f64 on the exact
 // quadratic q = A dx^2 + 2B dx dy + C dy^2 (power = -q/2): the minimum of a
@@ -452,14 +461,14 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           const float dy = pyf[k] - a.y;
           const float r = a.z;
           if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
-          const float power = (-0.5f) * (((b.x * dx) * dx + ((2.0f * b.y) * dx) * dy) + (b.z * dy) * dy);
+          const float power = gauss_power(b.x, b.y, b.z, dx, dy);
           if (power > 0.0f || power < col.w) continue;
           const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf_nonpos(power) - kGaussFloor));
           if (alpha < kMinAlpha) continue;
           const float w = T[k] * alpha;
-          cr[k] = cr[k] + w * col.x;
-          cg[k] = cg[k] + w * col.y;
-          cb[k] = cb[k] + w * col.z;
+          cr[k] = __builtin_fmaf(w, col.x, cr[k]);
+          cg[k] = __builtin_fmaf(w, col.y, cg[k]);
+          cb[k] = __builtin_fmaf(w, col.z, cb[k]);
           T[k] = T[k] * (1.0f - alpha);
         }
       }

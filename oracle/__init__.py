@@ -71,6 +71,7 @@ def lib():
         L.orc_expf.restype = C.c_float
         L.orc_expf.argtypes = [C.c_float]
         L.orc_expf_array.argtypes = [f32p, f32p, C.c_size_t]
+        L.orc_fmaf_array.argtypes = [f32p, f32p, f32p, f32p, C.c_size_t]
         L.orc_float_to_sortable.restype = C.c_uint32
         L.orc_float_to_sortable.argtypes = [C.c_float]
         L.orc_u32sat.restype = C.c_uint32
@@ -153,6 +154,17 @@ def expf(x) -> np.ndarray:
     out = np.empty_like(x)
     lib().orc_expf_array(_p(x, f32p), _p(out, f32p), x.size)
     return out
+
+
+def fmaf(a, b, c) -> np.ndarray:
+    """Element-wise correctly rounded f32 fused multiply-add (C fmaf),
+    broadcast over the arguments."""
+    a, b, c = np.broadcast_arrays(np.asarray(a, np.float32), np.asarray(b, np.float32), np.asarray(c, np.float32))
+    shape = a.shape
+    a, b, c = (np.ascontiguousarray(v.reshape(-1)) for v in (a, b, c))
+    out = np.empty_like(a)
+    lib().orc_fmaf_array(_p(a, f32p), _p(b, f32p), _p(c, f32p), _p(out, f32p), a.size)
+    return out.reshape(shape)
 
 
 def focal(cam) -> float:

@@ -375,7 +375,7 @@ static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, 
         {
             float dx = px32 - r[0], dy = py32 - r[1];
             if (!(r[2] <= 0.0f || fabsf(dx) > r[2] || fabsf(dy) > r[2])) {
-                float power = -0.5f * (((r[4] * dx) * dx + ((2.0f * r[5]) * dx) * dy) + (r[6] * dy) * dy);
+                float power = orc_power32(r[4], r[5], r[6], dx, dy);
                 if (power > 0.0f) code = 2;
                 else {
                     float ge = orc_expf(power) - (float)kFloor;
@@ -413,7 +413,7 @@ static double pixel_loss_forced(const orc_gradcheck *g, uint32_t x, uint32_t y, 
         }
         if (k == sub_rank) h = (h ^ (uint64_t)(code64 + 8 * step)) * 1099511628211ull;
         if (code >= 4) {
-            for (int c = 0; c < 3; c++) C32[c] = C32[c] + (T32 * a32) * r[8 + c];
+            for (int c = 0; c < 3; c++) C32[c] = fmaf(T32 * a32, r[8 + c], C32[c]);
             T32 = T32 * (1.0f - a32);
             /* the f32 decision: saturated at the cap, or the unclamped value */
             double a = code == 5 ? kCap : raw64;
@@ -552,13 +552,13 @@ double orc_gc_dalpha(const orc_gradcheck *g, uint32_t id, int f64) {
                     const float *r = g->proj32 + 12 * (size_t)g->tlist[q];
                     float dx = px - r[0], dy = py - r[1];
                     if (r[2] <= 0.0f || fabsf(dx) > r[2] || fabsf(dy) > r[2]) continue;
-                    float power = -0.5f * (((r[4] * dx) * dx + ((2.0f * r[5]) * dx) * dy) + (r[6] * dy) * dy);
+                    float power = orc_power32(r[4], r[5], r[6], dx, dy);
                     if (power > 0.0f) continue;
                     float ge = orc_expf(power) - (float)kFloor;
                     float raw = r[7] * (ge > 0.0f ? ge : 0.0f);
                     float a = raw < (float)kCap ? raw : (float)kCap;
                     if (a < (float)kMinA) continue;
-                    for (int c = 0; c < 3; c++) Cc[c] = Cc[c] + (T * a) * r[8 + c];
+                    for (int c = 0; c < 3; c++) Cc[c] = fmaf(T * a, r[8 + c], Cc[c]);
                     T = T * (1.0f - a);
                 }
                 double G[3];
@@ -578,7 +578,7 @@ double orc_gc_dalpha(const orc_gradcheck *g, uint32_t id, int f64) {
                     const float *r = g->proj32 + 12 * (size_t)k;
                     float dx = px - r[0], dy = py - r[1];
                     if (r[2] <= 0.0f || fabsf(dx) > r[2] || fabsf(dy) > r[2]) continue;
-                    float power = -0.5f * (((r[4] * dx) * dx + ((2.0f * r[5]) * dx) * dy) + (r[6] * dy) * dy);
+                    float power = orc_power32(r[4], r[5], r[6], dx, dy);
                     if (power > 0.0f) continue;
                     float ef = orc_expf(power);
                     float ge = ef - (float)kFloor;

@@ -76,6 +76,10 @@ float orc_expf(float x) {
     return (p * s1) * s2;
 }
 
+void orc_fmaf_array(const float *a, const float *b, const float *c, float *out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = fmaf(a[i], b[i], c[i]);
+}
+
 void orc_expf_array(const float *in, float *out, size_t n) {
     for (size_t i = 0; i < n; i++) out[i] = orc_expf(in[i]);
 }
@@ -522,15 +526,15 @@ static void raster_tile(uint32_t tx, uint32_t ty, const unif_t *u, const float *
                 float r = v[2];
                 if (r <= 0.0f || fabsf(dx) > r || fabsf(dy) > r) continue;
                 st->evals++;
-                float power = (-0.5f) * (((v[4] * dx) * dx + ((2.0f * v[5]) * dx) * dy) + (v[6] * dy) * dy);
+                float power = orc_power32(v[4], v[5], v[6], dx, dy);
                 if (power > 0.0f) continue;
                 float alpha = wmin(K_CAP, v[7] * wmax(0.0f, orc_expf(power) - K_FLOOR));
                 if (alpha < K_AMIN) continue;
                 st->blends++;
                 float w = T * alpha;
-                cr = cr + w * v[8];
-                cg = cg + w * v[9];
-                cb = cb + w * v[10];
+                cr = fmaf(w, v[8], cr);
+                cg = fmaf(w, v[9], cg);
+                cb = fmaf(w, v[10], cb);
                 T = T * (1.0f - alpha);
             }
             st->iters += it;

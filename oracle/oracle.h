@@ -32,6 +32,14 @@
  * (equirect only) are evaluated in f64 and rounded to f32. JS f64 arithmetic
  * (cull, AABB, bounds, BVH) is restated in C double without contraction.
  *
+ * Contraction. C here is compiled without contraction; WGSL lets a compiler
+ * fuse `a * b + c`, and GPU back ends do. The specification fuses exactly two
+ * expressions of the rasterizer's loop body (rasterize.ts:70,77), both on the
+ * device and here: the Gaussian exponent
+ *   power = -0.5 * fma(C*dy, dy, fma((2B)*dx, dy, (A*dx)*dx))
+ * (orc_power32) and the colour accumulation color = fma(T*alpha, c, color).
+ * Everything else is unfused.
+ *
  * Pinning status of each part is listed in the test suite (tests/test_oracle_*.py).
  */
 #ifndef DROPLET_ORACLE_H
@@ -96,7 +104,13 @@ typedef struct {
 } orc_render_taps;
 
 /* --- primitives --- */
+/* the rasterizer's Gaussian exponent (rasterize.ts:70), contracted as above */
+static inline float orc_power32(float A, float B, float C, float dx, float dy) {
+    return (-0.5f) * __builtin_fmaf(C * dy, dy, __builtin_fmaf((2.0f * B) * dx, dy, (A * dx) * dx));
+}
 float orc_expf(float x);
+/* out[i] = fmaf(a[i], b[i], c[i]), for the tests' numpy restatements */
+void orc_fmaf_array(const float *a, const float *b, const float *c, float *out, size_t n);
 void orc_expf_array(const float *in, float *out, size_t n);
 float orc_sigmoidf(float x);
 uint32_t orc_float_to_sortable(float f);

@@ -137,3 +137,37 @@ def test_cull_and_depth_sort(proj):
     sorted_idx = oracle.depth_sort(idx, dep)
     # stable LSD radix on sortable keys == stable sort by f32 depth value
     np.testing.assert_array_equal(sorted_idx, idx[np.argsort(dep, kind="stable")])
+
+
+def _round_f32(q):
+    """Round a rational exactly to the nearest f32 (ties to even)."""
+    from fractions import Fraction
+    f = np.float32(float(q))
+    best = None
+    for cand in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        if not np.isfinite(cand):
+            continue
+        d = abs(Fraction(float(cand)) - q)
+        key = (d, int(np.float32(cand).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, cand)
+    return best[1]
+
+
+def test_fmaf_is_correctly_rounded():
+    """The one fused operation of the rasterizer's specification: the oracle's
+    fmaf equals a*b + c computed exactly and rounded once, including sums
+    whose exponents are far apart (where an unfused or double-rounded
+    evaluation differs)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(17)
+    a = rng.normal(0, 1, 3000).astype(np.float32)
+    b = rng.normal(0, 1, 3000).astype(np.float32)
+    c = (rng.normal(0, 1, 3000) * 10.0 ** rng.integers(-12, 12, 3000)).astype(np.float32)
+    c[:500] = -(a[:500] * b[:500])            # near-cancellation
+    got = oracle.fmaf(a, b, c)
+    want = np.array([_round_f32(Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z)))
+                     for x, y, z in zip(a, b, c)], np.float32)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    unfused = (a * b + c).astype(np.float32)
+    assert (unfused.view(np.uint32) != want.view(np.uint32)).any()

@@ -184,14 +184,15 @@ def _brute_force_accum(proj, W, H):
         dx = px - sx
         dy = py - sy
         inbox = live & ~((np.abs(dx) > r) | (np.abs(dy) > r))
-        power = f32(-0.5) * (((A * dx) * dx + ((f32(2.0) * B) * dx) * dy) + (C * dy) * dy)
+        # rasterize.ts:70 with the specification's two contractions
+        power = f32(-0.5) * oracle.fmaf(C * dy, dy, oracle.fmaf((f32(2.0) * B) * dx, dy, (A * dx) * dx))
         e = oracle.expf(power).reshape(H, W)
         alpha = np.minimum(cap, a * np.maximum(f32(0.0), e - floor))
         upd = inbox & ~(power > 0) & ~(alpha < amin)
         w = T * alpha
-        cr = np.where(upd, cr + w * rec[8], cr)
-        cg = np.where(upd, cg + w * rec[9], cg)
-        cb = np.where(upd, cb + w * rec[10], cb)
+        cr = np.where(upd, oracle.fmaf(w, rec[8], cr), cr)
+        cg = np.where(upd, oracle.fmaf(w, rec[9], cg), cg)
+        cb = np.where(upd, oracle.fmaf(w, rec[10], cb), cb)
         T = np.where(upd, T * (f32(1.0) - alpha), T)
     return np.stack([cr, cg, cb, T], -1)
 
