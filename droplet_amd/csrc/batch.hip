@@ -95,14 +95,13 @@ __global__ __launch_bounds__(1024) void k_sum_pairs(const unsigned long long *__
 // One pass over every tile of the group. resume / finalize / write_accum and
 // the closed flags follow the single-view renderer's passes; accum and the
 // images are indexed by the view's first pixel.
-__global__ __launch_bounds__(64) void k_batch_raster(const BView *__restrict__ views, uint32_t nv, uint32_t cols,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_raster(const BView *__restrict__ views, uint32_t nv, uint32_t cols,
                                                      const float4 *__restrict__ rec,
                                                      const uint32_t *__restrict__ ranges,
                                                      const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
                                                      uint32_t *__restrict__ images, uint8_t *__restrict__ closed,
                                                      uint32_t resume, uint32_t finalize) {
-  __shared__ float4 s_rec[64 * 3];
-  __shared__ uint32_t s_mask[64];
+  __shared__ RastLds lds;
   const BView &V = views[view_of_block(views, nv, blockIdx.x, true)];
   const uint32_t t = blockIdx.x - V.tile_blk0;
   const uint32_t tx = t % V.tiles_x, ty = t / V.tiles_x;
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(64) void k_batch_raster(const BView *__restrict__ v
   p.bg_b = V.bg[2];
   p.bg_a = V.bg[3];
   rasterize_tile<false>(tx, ty, vtile, ranges[vtile], ranges[vtile + 1], rec, pvals, accum ? accum + V.px0 : nullptr,
-                        images + V.px0, p, nullptr, s_rec, s_mask);
+                        images + V.px0, p, nullptr, lds);
 }
 
 template <class T>

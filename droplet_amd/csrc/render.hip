@@ -153,17 +153,19 @@ __global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, co
   }
 }
 
+// Six waves per SIMD: the staging loop keeps the tile's accumulators in LDS,
+// so the kernel fits 80 VGPRs without spilling, and the extra waves hide the
+// latency of the record loads (measured 0.73 -> 0.72 ms at 6M splats / 4K).
 template <bool COUNT>
-__global__ __launch_bounds__(64) void k_rasterize(const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize(const float4 *__restrict__ rec, const uint32_t *__restrict__ off,
                                                   const uint32_t *__restrict__ pvals, float4 *__restrict__ accum,
                                                   uint32_t *__restrict__ out_rgba, RastParams p,
                                                   unsigned long long *__restrict__ d_iters) {
-  __shared__ float4 s_rec[64 * 3];
-  __shared__ uint32_t s_mask[64];
+  __shared__ RastLds lds;
   const uint32_t tile = blockIdx.x;
   const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
   rasterize_tile<COUNT>(tile % p.tiles_x, tile / p.tiles_x, tile, start, end, rec, pvals, accum, out_rgba, p, d_iters,
-                        s_rec, s_mask);
+                        lds);
 }
 
 __global__ void k_zero_u64(unsigned long long *__restrict__ p) {

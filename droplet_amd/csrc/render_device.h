@@ -377,6 +377,16 @@ __device__ inline bool rect_may_hit(double sx, double sy, double r, double A, do
   return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
 }
 
+// LDS of one rasterizer wave: a batch of kStage staged splat records and
+// their strip masks (two halves of 32 lanes, strips 0-1 and 2-3), and the
+// lanes' 16 accumulators while the batch is staged.
+constexpr uint32_t kStage = 32;
+struct RastLds {
+  float4 rec[kStage * 3];
+  uint32_t mask[64];
+  float acc[16 * 64];
+};
+
 // The work of one wave64 on one tile: tile (tx, ty) of the view, whose
 // accum / out_rgba are the view's own row-major pixels, with list entries
 // [start, end) of pvals (which index rec); vtile
@@ -388,8 +398,8 @@ template <bool COUNT>
 __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
                                       const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
                                       float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
-                                      const RastParams &p, unsigned long long *__restrict__ d_iters, float4 *s_rec,
-                                      uint32_t *s_mask) {
+                                      const RastParams &p, unsigned long long *__restrict__ d_iters,
+                                      RastLds &L) {
   if (p.closed && p.resume && !p.finalize && (start == end || p.closed[vtile])) return;
   uint32_t iters = 0;
   const int lane = threadIdx.x;
@@ -417,7 +427,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
   if (start < end) {
     const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
     uint32_t stop = end;
-    for (uint32_t base = start; base < end; base += 64) {
+    for (uint32_t base = start; base < end; base += kStage) {
       uint32_t strips_live = 0;
 #pragma unroll
       for (int k = 0; k < 4; k++) strips_live |= (__ballot(!(T[k] < kMinT)) != 0 ? 1u : 0u) << k;
@@ -425,34 +435,59 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
         stop = base;
         break;
       }
-      const uint32_t i = base + lane;
+      // Staging: the accumulators wait in LDS while the f64 strip tests run,
+      // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
+      // splats, two lanes per splat with two strips each, so that the LDS
+      // of a wave stays under 6 KB. Both bounds together allow 6 waves per
+      // SIMD.
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        L.acc[(4 * k + 0) * 64 + lane] = cr[k];
+        L.acc[(4 * k + 1) * 64 + lane] = cg[k];
+        L.acc[(4 * k + 2) * 64 + lane] = cb[k];
+        L.acc[(4 * k + 3) * 64 + lane] = T[k];
+      }
+      const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
+      const uint32_t i = base + slot;
       if (i < end) {
         const uint32_t j = pvals[i];
         const float4 a = rec[3 * (size_t)j + 0];
         const float4 b = rec[3 * (size_t)j + 1];
         const float4 col = rec[3 * (size_t)j + 2];
-        s_rec[3 * lane + 0] = a;
-        s_rec[3 * lane + 1] = b;
-        s_rec[3 * lane + 2] = col;
+        if (half == 0) {
+          L.rec[3 * slot + 0] = a;
+          L.rec[3 * slot + 1] = b;
+          L.rec[3 * slot + 2] = col;
+        }
         uint32_t m = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
+        // one strip at a time (unrolled, the f64 temporaries of several
+        // strips take more VGPRs)
+#pragma unroll 1
+        for (uint32_t q = 0; q < 2; q++) {
+          const uint32_t k = 2 * half + q;
           const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
           if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
         }
-        s_mask[lane] = m;
+        L.mask[lane] = m;
       }
       __syncthreads();
-      const uint32_t cnt = min(64u, end - base);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        cr[k] = L.acc[(4 * k + 0) * 64 + lane];
+        cg[k] = L.acc[(4 * k + 1) * 64 + lane];
+        cb[k] = L.acc[(4 * k + 2) * 64 + lane];
+        T[k] = L.acc[(4 * k + 3) * 64 + lane];
+      }
+      const uint32_t cnt = min(kStage, end - base);
       for (uint32_t c = 0; c < cnt; c++) {
         if (COUNT)
           iters += (uint32_t)!(T[0] < kMinT) + (uint32_t)!(T[1] < kMinT) + (uint32_t)!(T[2] < kMinT) +
                    (uint32_t)!(T[3] < kMinT);
-        const uint32_t m = __builtin_amdgcn_readfirstlane(s_mask[c]) & strips_live;
+        const uint32_t m = __builtin_amdgcn_readfirstlane(L.mask[c] | L.mask[c + kStage]) & strips_live;
         if (m == 0) continue;
-        const float4 a = s_rec[3 * c + 0];
-        const float4 b = s_rec[3 * c + 1];
-        const float4 col = s_rec[3 * c + 2];
+        const float4 a = L.rec[3 * c + 0];
+        const float4 b = L.rec[3 * c + 1];
+        const float4 col = L.rec[3 * c + 2];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           if (!((m >> k) & 1u)) continue;
