@@ -88,6 +88,14 @@ class DlVoxelOpts(C.Structure):
                 ("bounds_min", C.c_double * 3), ("bounds_max", C.c_double * 3)]
 
 
+class DlVoxelDeviceView(C.Structure):
+    _fields_ = [("masks", C.c_void_p), ("num_blocks", C.c_uint64), ("nb", C.c_int32 * 3),
+                ("bounds_min", C.c_double * 3), ("voxel_resolution", C.c_double), ("block_idx", C.c_void_p),
+                ("mask_lo", C.c_void_p), ("mask_hi", C.c_void_p), ("count", C.c_uint64), ("device", C.c_int)]
+
+
+f64p = C.POINTER(C.c_double)
+
 # (name, restype, argtypes) of every entry point declared in include/droplet_hip.h
 SIGNATURES = [
     ("dl_context_create", C.c_int, [C.POINTER(DlDeviceOpts), C.POINTER(C.c_void_p)]),
@@ -129,6 +137,17 @@ SIGNATURES = [
     ("dl_voxel_result_bounds", C.c_int, [C.c_void_p, C.c_double * 3, C.c_double * 3, C.c_double * 3]),
     ("dl_voxel_result_device_ms", C.c_float, [C.c_void_p]),
     ("dl_voxel_result_free", None, [C.c_void_p]),
+    ("dl_voxel_result_device", C.c_int, [C.c_void_p, C.POINTER(DlVoxelDeviceView)]),
+    ("dl_voxel_result_create", C.c_int, [C.c_void_p, C.c_double * 3, C.c_double * 3, C.c_double * 3, C.c_double,
+                                         u32p, u32p, u32p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("dl_voxel_fill", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("dl_voxel_carve", C.c_int, [C.c_void_p, C.c_void_p, f64p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("dl_voxel_query", C.c_int, [C.c_void_p, C.c_void_p, f64p, C.c_uint64, u8p]),
+    ("dl_voxel_mesh_build", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("dl_voxel_mesh_quads", C.c_uint64, [C.c_void_p]),
+    ("dl_voxel_mesh_read", C.c_int, [C.c_void_p, f32p, u32p]),
+    ("dl_voxel_mesh_device_ms", C.c_float, [C.c_void_p]),
+    ("dl_voxel_mesh_free", None, [C.c_void_p]),
     ("dl_debug_math", C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, C.c_uint32]),
 ]
 

@@ -31,6 +31,7 @@
 
 #include "dl_common.h"
 #include "dl_internal.h"
+#include "voxel_grid.h"
 
 namespace dl {
 
@@ -290,9 +291,13 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
   if (lane == 0) masks[blk] = make_uint2((uint32_t)solid, (uint32_t)(solid >> 32));
 }
 
+struct GridDims {
+  int32_t nb[3];
+};
+
 // Insertion order of BlockMaskBuffer (voxelize.ts:186-188, 162-177): batches
 // in (bz, by, bx) loop order, blocks inside a batch by lx + ly*cx + lz*cx*cy.
-__device__ inline uint32_t ref_order(uint32_t ax, uint32_t ay, uint32_t az, const VoxGrid &G) {
+__device__ inline uint32_t ref_order(uint32_t ax, uint32_t ay, uint32_t az, const GridDims &G) {
   const uint32_t nbx = (uint32_t)G.nb[0], nby = (uint32_t)G.nb[1], nbz = (uint32_t)G.nb[2];
   const uint32_t bx0 = (ax / 16u) * 16u, by0 = (ay / 16u) * 16u, bz0 = (az / 16u) * 16u;
   const uint32_t cx = min(16u, nbx - bx0), cy = min(16u, nby - by0), cz = min(16u, nbz - bz0);
@@ -300,7 +305,7 @@ __device__ inline uint32_t ref_order(uint32_t ax, uint32_t ay, uint32_t az, cons
   return bz0 * nby * nbx + cz * by0 * nbx + cz * cy * bx0 + local;
 }
 
-__global__ __launch_bounds__(256) void k_vox_order(const uint2 *__restrict__ masks, uint32_t total, VoxGrid G,
+__global__ __launch_bounds__(256) void k_vox_order(const uint2 *__restrict__ masks, uint32_t total, GridDims G,
                                                    uint32_t *__restrict__ flags, uint2 *__restrict__ omask,
                                                    uint32_t *__restrict__ oidx) {
   const uint32_t blk = blockIdx.x * 256 + threadIdx.x;
@@ -335,11 +340,56 @@ double js_round(double x) {
 
 }  // namespace
 
-struct VoxelResult {
-  std::vector<uint32_t> idx, lo, hi;
-  double bmin[3], bmax[3], nblocks[3];
-  float device_ms = 0.0f;
-};
+void voxel_compact(hipStream_t s, VoxelResult &r, hipEvent_t done) {
+  r.count = 0;
+  r.host_ready = false;
+  if (r.total == 0) {
+    if (done) DL_HIP(hipEventRecord(done, s));
+    DL_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  if (r.total >= 0xFFFFFFFFull) fail(DL_ERR_CAPACITY, "voxel grid has more than 2^32-1 blocks");
+  const uint32_t total = (uint32_t)r.total;
+  GridDims G = {{r.nb[0], r.nb[1], r.nb[2]}};
+  DeviceBuffer flags, fpos, omask, oidx, scan_ws, small;
+  flags.reserve((size_t)total * 4);
+  fpos.reserve((size_t)total * 4);
+  omask.reserve((size_t)total * 8);
+  oidx.reserve((size_t)total * 4);
+  scan_ws.reserve(scan_workspace_bytes(total));
+  small.reserve(256);
+  k_vox_order<<<div_up(total, 256), 256, 0, s>>>(r.masks.as<uint2>(), total, G, flags.as<uint32_t>(),
+                                                 omask.as<uint2>(), oidx.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+  exclusive_scan_u32(s, flags.as<uint32_t>(), fpos.as<uint32_t>(), total, small.as<uint32_t>(), scan_ws.get());
+  uint32_t nonempty = 0;
+  DL_HIP(hipMemcpyAsync(&nonempty, small.get(), 4, hipMemcpyDeviceToHost, s));
+  DL_HIP(hipStreamSynchronize(s));
+  r.idx.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
+  r.lo.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
+  r.hi.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
+  k_vox_compact<<<div_up(total, 256), 256, 0, s>>>(flags.as<uint32_t>(), fpos.as<uint32_t>(), omask.as<uint2>(),
+                                                   oidx.as<uint32_t>(), total, r.idx.as<uint32_t>(),
+                                                   r.lo.as<uint32_t>(), r.hi.as<uint32_t>());
+  DL_HIP(hipGetLastError());
+  if (done) DL_HIP(hipEventRecord(done, s));
+  DL_HIP(hipStreamSynchronize(s));
+  r.count = nonempty;
+}
+
+void voxel_lists_to_host(const VoxelResult &r) {
+  if (r.host_ready) return;
+  r.h_idx.resize(r.count);
+  r.h_lo.resize(r.count);
+  r.h_hi.resize(r.count);
+  if (r.count) {
+    DL_HIP(hipSetDevice(r.device));
+    DL_HIP(hipMemcpy(r.h_idx.data(), r.idx.get(), (size_t)r.count * 4, hipMemcpyDeviceToHost));
+    DL_HIP(hipMemcpy(r.h_lo.data(), r.lo.get(), (size_t)r.count * 4, hipMemcpyDeviceToHost));
+    DL_HIP(hipMemcpy(r.h_hi.data(), r.hi.get(), (size_t)r.count * 4, hipMemcpyDeviceToHost));
+  }
+  r.host_ready = true;
+}
 
 static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts *opts, VoxelResult &out) {
   const double vres = (opts && opts->voxel_resolution > 0) ? opts->voxel_resolution : 0.05;
@@ -347,6 +397,8 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   const double bs = 4.0 * vres;
   const uint32_t n = sp.count;
   ctx.make_current();
+  out.device = ctx.device;
+  out.vres = vres;
   hipStream_t s = ctx.stream;
   hipEvent_t e0, e1;
   DL_HIP(hipEventCreate(&e0));
@@ -405,9 +457,9 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     out.bmax[k] = bmx[k];
     out.nblocks[k] = nbd[k];
   }
-  out.idx.clear();
-  out.lo.clear();
-  out.hi.clear();
+  out.count = 0;
+  out.total = 0;
+  out.host_ready = false;
   int64_t nb[3];
   for (int k = 0; k < 3; k++) nb[k] = (std::isfinite(nbd[k]) && nbd[k] > 0) ? (int64_t)nbd[k] : 0;
   const uint64_t total64 = (uint64_t)nb[0] * (uint64_t)nb[1] * (uint64_t)nb[2];
@@ -422,6 +474,8 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   if (total64 >= 0xFFFFFFFFull || nb[0] > 0x7FFFFFFF || nb[1] > 0x7FFFFFFF || nb[2] > 0x7FFFFFFF)
     fail(DL_ERR_CAPACITY, "voxel grid has more than 2^32-1 blocks");
   const uint32_t total = (uint32_t)total64;
+  for (int k = 0; k < 3; k++) out.nb[k] = (int32_t)nb[k];
+  out.total = total;
 
   VoxGrid G;
   for (int k = 0; k < 3; k++) {
@@ -453,7 +507,8 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   DL_HIP(hipMemcpyAsync(&npairs, small.get(), 4, hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
 
-  DeviceBuffer keys, vals, tkeys, tvals, sort_ws, ranges, masks;
+  DeviceBuffer keys, vals, tkeys, tvals, sort_ws, ranges;
+  DeviceBuffer &masks = out.masks;
   const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(total - 1));
   keys.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
   vals.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
@@ -479,46 +534,13 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   DL_HIP(hipGetLastError());
 
   // reference insertion order + compaction of non-empty blocks
-  DeviceBuffer flags, fpos, omask, oidx, o_idx, o_lo, o_hi;
-  flags.reserve((size_t)total * 4);
-  fpos.reserve((size_t)total * 4);
-  omask.reserve((size_t)total * 8);
-  oidx.reserve((size_t)total * 4);
-  scan_ws.reserve(scan_workspace_bytes(total));
-  k_vox_order<<<div_up(total, 256), 256, 0, s>>>(masks.as<uint2>(), total, G, flags.as<uint32_t>(), omask.as<uint2>(),
-                                                 oidx.as<uint32_t>());
-  DL_HIP(hipGetLastError());
-  exclusive_scan_u32(s, flags.as<uint32_t>(), fpos.as<uint32_t>(), total, small.as<uint32_t>(), scan_ws.get());
-  uint32_t nonempty = 0;
-  DL_HIP(hipMemcpyAsync(&nonempty, small.get(), 4, hipMemcpyDeviceToHost, s));
-  DL_HIP(hipStreamSynchronize(s));
-  o_idx.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
-  o_lo.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
-  o_hi.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
-  k_vox_compact<<<div_up(total, 256), 256, 0, s>>>(flags.as<uint32_t>(), fpos.as<uint32_t>(), omask.as<uint2>(),
-                                                   oidx.as<uint32_t>(), total, o_idx.as<uint32_t>(),
-                                                   o_lo.as<uint32_t>(), o_hi.as<uint32_t>());
-  DL_HIP(hipGetLastError());
-  DL_HIP(hipEventRecord(e1, s));
-  out.idx.resize(nonempty);
-  out.lo.resize(nonempty);
-  out.hi.resize(nonempty);
-  if (nonempty) {
-    DL_HIP(hipMemcpyAsync(out.idx.data(), o_idx.get(), (size_t)nonempty * 4, hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(out.lo.data(), o_lo.get(), (size_t)nonempty * 4, hipMemcpyDeviceToHost, s));
-    DL_HIP(hipMemcpyAsync(out.hi.data(), o_hi.get(), (size_t)nonempty * 4, hipMemcpyDeviceToHost, s));
-  }
-  DL_HIP(hipStreamSynchronize(s));
+  voxel_compact(s, out, e1);
   DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }
 
 }  // namespace dl
-
-struct dl_voxel_result {
-  dl::VoxelResult r;
-};
 
 using namespace dl;
 
@@ -534,16 +556,17 @@ int dl_voxelize(dl_context *ctx, const dl_splats *splats, const dl_voxel_opts *o
   });
 }
 
-uint64_t dl_voxel_result_count(const dl_voxel_result *res) { return res ? res->r.idx.size() : 0; }
+uint64_t dl_voxel_result_count(const dl_voxel_result *res) { return res ? res->r.count : 0; }
 
 int dl_voxel_result_blocks(const dl_voxel_result *res, uint32_t *block_idx, uint32_t *mask_lo, uint32_t *mask_hi) {
   return guard([&] {
-    if (!res || (!res->r.idx.empty() && (!block_idx || !mask_lo || !mask_hi))) fail(DL_ERR_INVALID, "null argument");
-    const size_t n = res->r.idx.size();
+    if (!res || (res->r.count && (!block_idx || !mask_lo || !mask_hi))) fail(DL_ERR_INVALID, "null argument");
+    const size_t n = res->r.count;
     if (!n) return;
-    std::memcpy(block_idx, res->r.idx.data(), n * 4);
-    std::memcpy(mask_lo, res->r.lo.data(), n * 4);
-    std::memcpy(mask_hi, res->r.hi.data(), n * 4);
+    voxel_lists_to_host(res->r);
+    std::memcpy(block_idx, res->r.h_idx.data(), n * 4);
+    std::memcpy(mask_lo, res->r.h_lo.data(), n * 4);
+    std::memcpy(mask_hi, res->r.h_hi.data(), n * 4);
   });
 }
 

@@ -298,6 +298,74 @@ int dl_voxel_result_bounds(const dl_voxel_result *res, double bounds_min[3], dou
 float dl_voxel_result_device_ms(const dl_voxel_result *res);
 void dl_voxel_result_free(dl_voxel_result *res);
 
+/* ---- the voxel grid in HBM and the collision stages after it.
+ * The reference keeps BlockMaskBuffer on the host
+ * (src/voxel/block-mask-buffer.ts:23-58) and has no code for the stages
+ * after it: README.md:108 and block-mask-buffer.ts:10-11 name fill, carve
+ * and mesh as grid arithmetic on the buffer. A dl_voxel_result keeps the
+ * dense grid of 64-bit block masks on the device; the stages below read it
+ * there and return new device-resident results. Definitions:
+ *   fill:  every empty voxel that 6-connected empty voxels do not join to
+ *          the grid's boundary becomes solid (enclosed cavities close);
+ *   carve: every voxel that 6-connected empty voxels do not join to a voxel
+ *          holding one of the points becomes solid. A point seeds the voxel
+ *          floor((p - bounds_min) / voxel_resolution) (f64) when that voxel
+ *          is inside the grid and empty;
+ *   mesh:  one quad per maximal chain of identical maximal face runs (faces
+ *          = solid voxels whose neighbour along the face's direction is
+ *          empty or outside the grid); see dl_voxel_mesh_build.
+ * Results of fill, carve and dl_voxel_result_create list their non-empty
+ * blocks in the voxelizer's insertion order. */
+typedef struct {
+    const uint32_t *masks;      /* device: (lo, hi) per block at the flat block index
+                                   bx + by*nb[0] + bz*nb[0]*nb[1]; voxel bit x + 4y + 16z */
+    uint64_t num_blocks;        /* nb[0]*nb[1]*nb[2]; 0 for an empty grid */
+    int32_t nb[3];
+    double bounds_min[3];
+    double voxel_resolution;
+    const uint32_t *block_idx;  /* device: the non-empty blocks in insertion order */
+    const uint32_t *mask_lo;
+    const uint32_t *mask_hi;
+    uint64_t count;
+    int device;                 /* HIP device that holds the arrays */
+} dl_voxel_device_view;
+
+/* Device pointers of a result; valid until dl_voxel_result_free. */
+int dl_voxel_result_device(const dl_voxel_result *res, dl_voxel_device_view *out);
+/* Uploads a host BlockMaskBuffer (the fields of VoxelizeResult,
+ * src/voxel/voxelize.ts:38-45): num_blocks must be integers with a product
+ * below 2^32-1, block_idx below that product; repeated blocks OR-merge as
+ * addBlock does (block-mask-buffer.ts:30-41). */
+int dl_voxel_result_create(dl_context *ctx, const double bounds_min[3], const double bounds_max[3],
+                           const double num_blocks[3], double voxel_resolution, const uint32_t *block_idx,
+                           const uint32_t *mask_lo, const uint32_t *mask_hi, uint64_t count,
+                           dl_voxel_result **out);
+int dl_voxel_fill(dl_context *ctx, const dl_voxel_result *in, dl_voxel_result **out);
+/* points: num_points (x, y, z) f64 triples, host memory */
+int dl_voxel_carve(dl_context *ctx, const dl_voxel_result *in, const double *points, uint64_t num_points,
+                   dl_voxel_result **out);
+/* out[i] = 1 when the voxel holding points[i] is solid, else 0 (also
+ * outside the grid); BlockMaskBuffer.get plus the bit test, for a batch. */
+int dl_voxel_query(dl_context *ctx, const dl_voxel_result *res, const double *points, uint64_t num_points,
+                   uint8_t *out);
+
+/* Quad mesh of the solid voxels' boundary. For the normal axis a the
+ * in-plane axes (u, v) are (y, z), (x, z), (x, y) for a = x, y, z. The faces
+ * of one slice and row v form maximal runs [u0, u1) along u; a run starts a
+ * quad unless row v-1 holds exactly the same run, and the quad covers the
+ * following rows that hold exactly that run. Quads are ordered by direction
+ * (+x, -x, +y, -y, +z, -z), slice, v0, u0. Quad q has vertices 4q..4q+3 at
+ * (u0,v0) (u1,v0) (u1,v1) (u0,v1) on its plane, each coordinate
+ * (float)(bounds_min + (double)i * voxel_resolution), and the triangles
+ * 6q..6q+5 wound counter-clockwise seen from outside the solid. */
+typedef struct dl_voxel_mesh dl_voxel_mesh;
+int dl_voxel_mesh_build(dl_context *ctx, const dl_voxel_result *res, dl_voxel_mesh **out);
+uint64_t dl_voxel_mesh_quads(const dl_voxel_mesh *m);
+/* positions: 12 floats per quad; indices: 6 per quad */
+int dl_voxel_mesh_read(const dl_voxel_mesh *m, float *positions, uint32_t *indices);
+float dl_voxel_mesh_device_ms(const dl_voxel_mesh *m);
+void dl_voxel_mesh_free(dl_voxel_mesh *m);
+
 /* ---- numerics self-check: evaluates the device's transcendental and IEEE
  * primitives on n inputs (used by the parity tests).
  * op: 0 exp, 1 sqrt, 2 div (a/b), 3 f64->f32 depth key, 4 sigmoid,

@@ -98,6 +98,14 @@ def lib():
         L.orc_morton_encode3.restype = C.c_uint32
         L.orc_morton_encode3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_morton_decode3.argtypes = [C.c_uint32, u32p]
+        u64p, i64p, f64p = C.POINTER(C.c_uint64), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        L.orc_vox_fill.restype = C.c_int
+        L.orc_vox_fill.argtypes = [u64p, i64p, u64p]
+        L.orc_vox_carve.restype = C.c_int
+        L.orc_vox_carve.argtypes = [u64p, i64p, f64p, C.c_double, f64p, C.c_uint64, u64p]
+        L.orc_vox_query.argtypes = [u64p, i64p, f64p, C.c_double, f64p, C.c_uint64, u8p]
+        L.orc_vox_mesh.restype = C.c_uint64
+        L.orc_vox_mesh.argtypes = [u64p, i64p, f64p, C.c_double, f32p, u32p, C.c_uint64]
         L.orc_gc_create.restype = C.c_void_p
         L.orc_gc_create.argtypes = [C.POINTER(OrcSplats), C.POINTER(OrcCamera), f32p, f32p, C.c_uint32, C.c_int]
         L.orc_gc_free.argtypes = [C.c_void_p]
@@ -333,6 +341,100 @@ def voxelize(splats, voxel_resolution: float = 0.05, opacity_cutoff: float = 0.1
         "near_bit": arrs["near_bit"][:min(nn, near_cap)].copy(),
         "near_alpha": arrs["near_alpha"][:min(nn, near_cap)].copy(),
     }
+
+
+def block_insertion_order(nb) -> np.ndarray:
+    """Flat indices of every block of an nb[0] x nb[1] x nb[2] grid in the order
+    voxelize adds them to its BlockMaskBuffer: 16^3-block batches in (z, y, x)
+    loop order (voxelize.ts:186-195), and inside a batch bi = lx + ly*cx +
+    lz*cx*cy (voxelize.ts:166-176). Empty blocks are skipped by addBlock, so
+    the non-empty blocks of any grid appear in this order."""
+    nbx, nby, nbz = (int(v) for v in nb)
+    out = []
+    for bz in range(0, nbz, 16):
+        for by in range(0, nby, 16):
+            for bx in range(0, nbx, 16):
+                cx, cy, cz = min(16, nbx - bx), min(16, nby - by), min(16, nbz - bz)
+                lz, ly, lx = np.meshgrid(np.arange(cz), np.arange(cy), np.arange(cx), indexing="ij")
+                flat = (bx + lx) + (by + ly) * nbx + (bz + lz) * nbx * nby
+                out.append(flat.ravel())
+    return np.concatenate(out).astype(np.int64) if out else np.zeros(0, np.int64)
+
+
+def sparse_blocks(dense: np.ndarray, nb):
+    """(block_idx, mask_lo, mask_hi) of the non-empty blocks of a dense grid in
+    voxelize's insertion order."""
+    order = block_insertion_order(nb)
+    m = np.asarray(dense, np.uint64)[order]
+    keep = m != 0
+    return (order[keep].astype(np.uint32), (m[keep] & np.uint64(0xFFFFFFFF)).astype(np.uint32),
+            (m[keep] >> np.uint64(32)).astype(np.uint32))
+
+
+def dense_masks(block_idx, mask_lo, mask_hi, nb) -> np.ndarray:
+    """Dense u64 grid (flat block index) of a BlockMaskBuffer's blocks; repeated
+    blocks OR-merge as BlockMaskBuffer.addBlock does."""
+    total = int(nb[0]) * int(nb[1]) * int(nb[2])
+    out = np.zeros(total, np.uint64)
+    m = np.asarray(mask_lo, np.uint64) | (np.asarray(mask_hi, np.uint64) << np.uint64(32))
+    np.bitwise_or.at(out, np.asarray(block_idx, np.int64), m)
+    return out
+
+
+def _grid_args(mask, nb):
+    mask = np.ascontiguousarray(mask, np.uint64)
+    nbv = np.ascontiguousarray(nb, np.int64)
+    assert nbv.size == 3 and mask.size == int(nbv.prod())
+    return mask, nbv
+
+
+def vox_fill(mask, nb) -> np.ndarray:
+    """Grid with every cavity enclosed by solid voxels closed (collision.c)."""
+    mask, nbv = _grid_args(mask, nb)
+    out = np.empty_like(mask)
+    rc = lib().orc_vox_fill(_p(mask, C.POINTER(C.c_uint64)), _p(nbv, C.POINTER(C.c_int64)),
+                            _p(out, C.POINTER(C.c_uint64)))
+    assert rc == 0
+    return out
+
+
+def vox_carve(mask, nb, bmin, res: float, points) -> np.ndarray:
+    """Grid where every voxel not reachable from the points is solid (collision.c)."""
+    mask, nbv = _grid_args(mask, nb)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    bm = np.ascontiguousarray(bmin, np.float64)
+    out = np.empty_like(mask)
+    rc = lib().orc_vox_carve(_p(mask, C.POINTER(C.c_uint64)), _p(nbv, C.POINTER(C.c_int64)),
+                             _p(bm, C.POINTER(C.c_double)), float(res), _p(pts, C.POINTER(C.c_double)),
+                             pts.shape[0], _p(out, C.POINTER(C.c_uint64)))
+    assert rc == 0
+    return out
+
+
+def vox_query(mask, nb, bmin, res: float, points) -> np.ndarray:
+    mask, nbv = _grid_args(mask, nb)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    bm = np.ascontiguousarray(bmin, np.float64)
+    out = np.empty(pts.shape[0], np.uint8)
+    lib().orc_vox_query(_p(mask, C.POINTER(C.c_uint64)), _p(nbv, C.POINTER(C.c_int64)),
+                        _p(bm, C.POINTER(C.c_double)), float(res), _p(pts, C.POINTER(C.c_double)),
+                        pts.shape[0], _p(out, u8p))
+    return out
+
+
+def vox_mesh(mask, nb, bmin, res: float):
+    """(positions float32 [Q*4, 3], indices uint32 [Q*2, 3]) of the quad mesh."""
+    mask, nbv = _grid_args(mask, nb)
+    bm = np.ascontiguousarray(bmin, np.float64)
+    L = lib()
+    args = (_p(mask, C.POINTER(C.c_uint64)), _p(nbv, C.POINTER(C.c_int64)), _p(bm, C.POINTER(C.c_double)),
+            float(res))
+    q = int(L.orc_vox_mesh(*args, None, None, 0))
+    pos = np.empty(q * 12, np.float32)
+    idx = np.empty(q * 6, np.uint32)
+    q2 = int(L.orc_vox_mesh(*args, _p(pos, f32p), _p(idx, u32p), q))
+    assert q2 == q
+    return pos.reshape(-1, 3), idx.reshape(-1, 3)
 
 
 def morton_encode3(x: int, y: int, z: int) -> int:

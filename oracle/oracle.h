@@ -207,6 +207,18 @@ int orc_bvh_order(const float *positions, const float *extents, uint32_t n, uint
 uint32_t orc_morton_encode3(uint32_t x, uint32_t y, uint32_t z);
 void orc_morton_decode3(uint32_t m, uint32_t out[3]);
 
+/* Collision stages (collision.c; no reference counterpart, definitions in
+ * that file). `mask` is the dense grid: one 64-bit mask per block, flat
+ * block index bx + by*nb0 + bz*nb0*nb1. Return 0, or -1 when out of memory. */
+int orc_vox_fill(const uint64_t *mask, const int64_t nb[3], uint64_t *out);
+int orc_vox_carve(const uint64_t *mask, const int64_t nb[3], const double bmin[3], double res, const double *points,
+                  uint64_t npoints, uint64_t *out);
+void orc_vox_query(const uint64_t *mask, const int64_t nb[3], const double bmin[3], double res,
+                   const double *points, uint64_t npoints, uint8_t *out);
+/* number of quads; the first cap_quads are written (12 floats, 6 indices each) */
+uint64_t orc_vox_mesh(const uint64_t *mask, const int64_t nb[3], const double bmin[3], double res, float *pos,
+                      uint32_t *idx, uint64_t cap_quads);
+
 #ifdef __cplusplus
 }
 #endif

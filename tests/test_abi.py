@@ -72,3 +72,11 @@ def test_generators_are_deterministic():
     q = a.rotations.reshape(-1, 4).astype(np.float64)
     np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-6)
     assert np.abs(a.positions).max() <= 2.0
+
+
+def test_voxel_device_view_layout():
+    # masks, num_blocks, nb[3] (+4 pad), bounds_min[3], voxel_resolution,
+    # three list pointers, count, device (+4 pad)
+    assert ctypes.sizeof(L.DlVoxelDeviceView) == 8 + 8 + 16 + 24 + 8 + 24 + 8 + 8
+    assert L.DlVoxelDeviceView.bounds_min.offset == 32
+    assert L.DlVoxelDeviceView.count.offset == 88
