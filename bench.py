@@ -161,7 +161,37 @@ def voxelize_leg(dev, note):
     return {"workload": "1M Gaussians in a 10x3x10 m box, voxelResolution 0.02, opacityCutoff 0.1",
             "device_ms": round(min(devs), 3), "wall_ms_incl_pcie": round(min(walls), 3),
             "blocks": int(res.buffer.count), "solid_voxels": int(res.buffer.popcount()),
-            "gaussians_per_s": round(data.count / (min(devs) * 1e-3), 1)}
+            "gaussians_per_s": round(data.count / (min(devs) * 1e-3), 1),
+            "collision": collision_leg(dev, data, opts, note)}
+
+
+def collision_leg(dev, data, opts, note):
+    """The stages after the voxelizer on its resident grid: fill, carve from
+    the grid's corner voxel, and the quad mesh of the filled grid (stage
+    times from hipEvents around each call, best of 3)."""
+    from droplet_amd.voxel import voxelize_resident
+
+    out = {}
+    with voxelize_resident(dev, data, opts) as g:
+        bmin = g.result().bounds.min
+        seed = [[bmin[0] + 0.5 * opts.voxel_resolution, bmin[1] + 0.5 * opts.voxel_resolution,
+                 bmin[2] + 0.5 * opts.voxel_resolution]]
+        fills, carves, meshes = [], [], []
+        for _ in range(3):
+            with g.fill() as f:
+                fills.append(f.device_ms)
+                m = f.mesh()
+                meshes.append(m.device_ms)
+                filled = f.result().buffer.popcount()
+            with g.carve(seed) as c:
+                carves.append(c.device_ms)
+                carved = c.result().buffer.popcount()
+        out = {"fill_ms": round(min(fills), 3), "carve_ms": round(min(carves), 3),
+               "mesh_ms": round(min(meshes), 3), "mesh_quads": int(m.quads),
+               "filled_solid_voxels": int(filled), "carved_solid_voxels": int(carved)}
+    note(f"collision: fill {out['fill_ms']} ms, carve {out['carve_ms']} ms, mesh {out['mesh_ms']} ms "
+         f"({out['mesh_quads']} quads)")
+    return out
 
 
 def main() -> int:
