@@ -316,10 +316,12 @@ __global__ __launch_bounds__(256) void k_splat_colors(SceneView sc, const uint32
 //
 // One wave64 per 16x16 tile. Lane l owns pixel column l%16 and the four rows
 // 4*(l/16) .. 4*(l/16)+3, so the wave covers the whole tile and needs no
-// workgroup barrier. The tile's splats are staged 64 at a time in LDS (one
-// record per lane) and read back as broadcasts. Every pixel runs exactly the
-// reference's loop body in the reference's order; a pixel stops when T < 1e-4
-// and the wave stops when all its pixels have.
+// workgroup barrier. The tile's splats are staged 32 at a time in LDS and read
+// back as broadcasts; the strips each staged splat can reach are four 32-bit
+// ballots held in scalar registers, and the wave visits only the splats with
+// a bit set. Every pixel runs exactly the reference's loop body in the
+// reference's order; a pixel stops when T < 1e-4 and the wave stops when all
+// its pixels have.
 // ---------------------------------------------------------------------------
 __device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, const RastParams &p) {
   const float r = wclamp(cr + T * p.bg_r, 0.0f, 1.0f);
@@ -377,13 +379,11 @@ __device__ inline bool rect_may_hit(double sx, double sy, double r, double A, do
   return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
 }
 
-// LDS of one rasterizer wave: a batch of kStage staged splat records and
-// their strip masks (two halves of 32 lanes, strips 0-1 and 2-3), and the
+// LDS of one rasterizer wave: a batch of kStage staged splat records and the
 // lanes' 16 accumulators while the batch is staged.
 constexpr uint32_t kStage = 32;
 struct RastLds {
   float4 rec[kStage * 3];
-  uint32_t mask[64];
   float acc[16 * 64];
 };
 
@@ -449,6 +449,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
       }
       const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
       const uint32_t i = base + slot;
+      uint32_t m = 0;
       if (i < end) {
         const uint32_t j = pvals[i];
         const float4 a = rec[3 * (size_t)j + 0];
@@ -459,7 +460,6 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           L.rec[3 * slot + 1] = b;
           L.rec[3 * slot + 2] = col;
         }
-        uint32_t m = 0;
         // one strip at a time (unrolled, the f64 temporaries of several
         // strips take more VGPRs)
 #pragma unroll 1
@@ -468,8 +468,14 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
           if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
         }
-        L.mask[lane] = m;
       }
+      // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
+      // strips 0 and 1, lanes 32-63 strips 2 and 3)
+      const uint64_t b02 = __ballot((m & 0x5u) != 0u), b13 = __ballot((m & 0xAu) != 0u);
+      const uint32_t s0 = (strips_live & 1u) ? (uint32_t)b02 : 0u;
+      const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
+      const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
+      const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -479,11 +485,17 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
         T[k] = L.acc[(4 * k + 3) * 64 + lane];
       }
       const uint32_t cnt = min(kStage, end - base);
-      for (uint32_t c = 0; c < cnt; c++) {
+      // COUNT visits every staged splat to count the reference's loop
+      // iterations; otherwise only the splats with a strip bit
+      uint32_t todo = COUNT ? (cnt == 32u ? ~0u : (1u << cnt) - 1u) : (s0 | s1 | s2 | s3);
+      while (todo) {
+        const uint32_t c = (uint32_t)__builtin_ctz(todo);
+        todo &= todo - 1u;
         if (COUNT)
           iters += (uint32_t)!(T[0] < kMinT) + (uint32_t)!(T[1] < kMinT) + (uint32_t)!(T[2] < kMinT) +
                    (uint32_t)!(T[3] < kMinT);
-        const uint32_t m = __builtin_amdgcn_readfirstlane(L.mask[c] | L.mask[c + kStage]) & strips_live;
+        const uint32_t m = ((s0 >> c) & 1u) | (((s1 >> c) & 1u) << 1) | (((s2 >> c) & 1u) << 2) |
+                           (((s3 >> c) & 1u) << 3);
         if (m == 0) continue;
         const float4 a = L.rec[3 * c + 0];
         const float4 b = L.rec[3 * c + 1];
