@@ -21,6 +21,10 @@ constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTileN = kBlock * kItems;  // 4096
 constexpr int kWaves = kBlock / 64;
+// Up to this many tiles, a scan's down-sweep sums the earlier tiles' totals
+// itself (at most 16 loads per thread from L2) instead of a separate scan of
+// the totals.
+constexpr int kSumPartialsMax = 4096;
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = __lane_id();
@@ -90,8 +94,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restri
   }
 }
 
-// Exclusive scan of one 4096-element tile plus the tile's prefix (from the
-// scanned partials). The last tile also writes the grand total.
+// Exclusive scan of one 4096-element tile plus the tile's prefix. With
+// SUM_PARTIALS the prefix is the sum of the per-tile totals of all earlier
+// tiles (the unscanned partials, read from L2 by every tile); otherwise it is
+// prefixes[tile] (already scanned) or 0. The last tile also writes the grand
+// total.
+template <bool SUM_PARTIALS>
 __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
                                                       size_t n_cap, const uint32_t *__restrict__ prefixes,
                                                       uint32_t *__restrict__ d_total,
@@ -105,6 +113,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict
     return;
   }
   if (base >= n) return;
+  uint32_t prefix = 0;
+  if (SUM_PARTIALS) {
+    uint32_t acc = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kBlock) acc += prefixes[j];
+    acc = wave_sum(acc);
+    if (__lane_id() == 0) s_wave[threadIdx.x >> 6] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) prefix += s_wave[w];
+    __syncthreads();
+  } else if (prefixes) {
+    prefix = prefixes[blockIdx.x];
+  }
   const bool last = base + kTileN >= n;
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
@@ -122,7 +143,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict
   }
   uint32_t total;
   uint32_t run = block_exclusive_scan(sum, s_wave, &total);
-  const uint32_t prefix = prefixes ? prefixes[blockIdx.x] : 0u;
   run += prefix;
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
@@ -369,7 +389,7 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
     return;
   }
   if (n <= (size_t)kTileN) {
-    k_scan_down<<<1, kBlock, 0, s>>>(in, out, n, nullptr, d_total, d_n, mult);
+    k_scan_down<false><<<1, kBlock, 0, s>>>(in, out, n, nullptr, d_total, d_n, mult);
     DL_HIP(hipGetLastError());
     return;
   }
@@ -380,8 +400,13 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
   uint8_t *next = p + 256;
   k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, partials, d_n, mult, d_nb);
   DL_HIP(hipGetLastError());
-  scan_rec(s, partials, partials, nb, nullptr, next, d_nb, 1);
-  k_scan_down<<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total, d_n, mult);
+  if (nb <= (uint32_t)kSumPartialsMax) {
+    // two launches: every tile sums the totals of the tiles before it
+    k_scan_down<true><<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total, d_n, mult);
+  } else {
+    scan_rec(s, partials, partials, nb, nullptr, next, d_nb, 1);
+    k_scan_down<false><<<nb, kBlock, 0, s>>>(in, out, n, partials, d_total, d_n, mult);
+  }
   DL_HIP(hipGetLastError());
 }
 

@@ -102,11 +102,12 @@ def test_sort_pairs_stable(device):
 
 def test_scan_u32(device):
     """The device-wide scan against a wrapping u32 cumulative sum: empty, one
-    tile, tile edges, several levels of partials and 20M values; three scans
-    back to back on one workspace."""
+    tile, tile edges, the largest scan whose tiles sum the earlier tiles'
+    totals themselves (4096 tiles), a recursive one (20M values); three
+    scans back to back on one workspace."""
     rng = np.random.default_rng(3)
     for n, hi in [(0, 10), (1, 10), (4095, 2 ** 32), (4096, 1000), (4097, 2 ** 32), (64 * 4096 + 5, 2 ** 32),
-                  (300_001, 3), (20_000_000, 2 ** 32)]:
+                  (300_001, 3), (4096 * 4096, 2 ** 32), (20_000_000, 2 ** 32)]:
         print(f"case: scan n={n}", flush=True)
         a = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
         out, total = scan_u32(device, a, repeat=3)
