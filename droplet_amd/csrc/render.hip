@@ -219,10 +219,13 @@ void launch_colors(hipStream_t s, const SceneView &sc, const uint32_t *ids, cons
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Stage events only timestamp the stream: without the system-scope fence a
+// record costs no cache writeback and the GPU does not idle at stage
+// boundaries (timestamps are read after the frame's own synchronisation).
 void Renderer::ensure_events(size_t n) {
   while (events.size() < n) {
     hipEvent_t e;
-    DL_HIP(hipEventCreate(&e));
+    DL_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     events.push_back(e);
   }
 }
