@@ -115,6 +115,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   p.finalize = finalize;
   p.write_accum = !finalize;
   p.closed = closed;
+  p.final_on_close = closed != nullptr;
   p.bg_r = V.bg[0];
   p.bg_g = V.bg[1];
   p.bg_b = V.bg[2];

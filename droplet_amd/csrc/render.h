@@ -45,6 +45,9 @@ constexpr uint64_t kPassGrowth = 8;
 struct RastParams {
   uint32_t tiles_x, width, height;
   uint32_t has_pairs, resume, finalize, write_accum;
+  // with closed tracking: a tile that closes writes its RGBA8 at once and no
+  // accumulator, and later passes (the final one too) skip it
+  uint32_t final_on_close;
   float bg_r, bg_g, bg_b, bg_a;
   uint8_t *closed;  // per (virtual) tile: 1 once every pixel has T < 1e-4 (null: not tracked)
 };

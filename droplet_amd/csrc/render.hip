@@ -473,6 +473,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       rp.finalize = last;
       rp.write_accum = !last || need_accum;
       rp.closed = track_closed ? closed.as<uint8_t>() : nullptr;
+      rp.final_on_close = track_closed && !need_accum;
       if (npairs == 0) {
         // pipeline.ts:306: nothing to rasterize in this chunk
         if (keep_taps) {

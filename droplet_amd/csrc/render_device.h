@@ -391,16 +391,22 @@ struct RastLds {
 // accum / out_rgba are the view's own row-major pixels, with list entries
 // [start, end) of pvals (which index rec); vtile
 // indexes the open/closed flags (p.closed). A tile is closed once all its
-// pixels have T < 1e-4: later passes give it no entries, and until the
-// final pass it is skipped, like a tile without entries in this pass, as
-// long as the accumulator already holds its state.
+// pixels have T < 1e-4: later passes give it no entries, so its colour is
+// final. With p.final_on_close it writes its RGBA8 when it closes (and no
+// accumulator) and every later pass skips it; otherwise later passes skip it
+// until the final one, which finalizes it from the accumulator. A tile
+// without entries in a non-final pass is skipped too, as long as the
+// accumulator already holds its state.
 template <bool COUNT>
 __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
                                       const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
                                       float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
                                       const RastParams &p, unsigned long long *__restrict__ d_iters,
                                       RastLds &L) {
-  if (p.closed && p.resume && !p.finalize && (start == end || p.closed[vtile])) return;
+  if (p.closed && p.resume) {
+    if (p.closed[vtile] && (p.final_on_close || !p.finalize)) return;
+    if (!p.finalize && start == end) return;
+  }
   uint32_t iters = 0;
   const int lane = threadIdx.x;
   // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
@@ -523,16 +529,19 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
     }
     consumed = stop - start;
   }
+  bool closing = false;
+  if (p.closed && !p.finalize) {
+    const bool open = __ballot(!(T[0] < kMinT) || !(T[1] < kMinT) || !(T[2] < kMinT) || !(T[3] < kMinT)) != 0;
+    if (lane == 0) p.closed[vtile] = open ? 0 : 1;
+    closing = !open && p.final_on_close;
+  }
+  const bool write_accum = p.write_accum && !closing, finalize = p.finalize || closing;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     if (!inb[k]) continue;
     const size_t pix = (size_t)(py0 + 4 * k) * p.width + px;
-    if (p.write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
-    if (p.finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
-  }
-  if (p.closed && !p.finalize) {
-    const bool open = __ballot(!(T[0] < kMinT) || !(T[1] < kMinT) || !(T[2] < kMinT) || !(T[3] < kMinT)) != 0;
-    if (lane == 0) p.closed[vtile] = open ? 0 : 1;
+    if (write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
+    if (finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }
   if (COUNT) {
     uint32_t s = iters;
