@@ -305,9 +305,13 @@ def main() -> int:
         "roofline": {
             "kernel": "k_rasterize", "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            # one unit = the frame's rasterize stage: st["num_chunks"] launches
+            # (front-to-back passes); work, time and traffic all cover the
+            # whole stage
+            "traffic": traffic.get("bytes_per_frame") if traffic else None,
             "work": f"{iters} pixel loop iterations x {RAST_OPS_PER_ITER} ops",
-            "launch_ms": rast_ms,
+            "launches_per_frame": st["num_chunks"],
+            "stage_ms": rast_ms,
         },
         "autotune": tune,
         "stage_ms": stage_ms,
