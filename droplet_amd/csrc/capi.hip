@@ -92,15 +92,46 @@ void Scene::upload(const dl_splats &sp) {
   upload_array(op, sp.opacity_logits, n, "opacityLogits", s);
   upload_array(dc, sp.colors_dc, 3 * n, "colorsDC", s);
   upload_array(sh, sp.colors_sh, 3 * (size_t)M * n, "colorsSH", s);
-  DL_HIP(hipStreamSynchronize(s));
   count = sp.count;
   bands = sp.sh_bands;
   sh_coeffs = M;
+  build_color_records();
+}
+
+// Splat i's record: DC, then the 3 * M SH values the renderer reads at
+// colorsSH[3 M i ..] (for a PLY scene with more coefficients per channel
+// than M(bands) that is the reference's own stride-3M read), then zeros.
+__global__ __launch_bounds__(256) void k_color_records(const float *__restrict__ dc, const float *__restrict__ sh,
+                                                       uint32_t n, uint32_t m3, uint32_t stride,
+                                                       float *__restrict__ col) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;  // one float of the output
+  if (e >= (size_t)n * stride) return;
+  const size_t i = e / stride;
+  const uint32_t j = (uint32_t)(e - i * stride);
+  float v = 0.0f;
+  if (j < 3) v = dc[3 * i + j];
+  else if (j < 3 + m3) v = sh[i * m3 + (j - 3)];
+  col[e] = v;
+}
+
+void Scene::build_color_records() {
+  ctx->make_current();
+  hipStream_t s = ctx->stream;
+  const uint32_t stride = color_record_floats(bands);
+  const uint32_t M = bands == 0 ? 0 : bands == 1 ? 3 : bands == 2 ? 8 : 15;
+  const size_t total = (size_t)count * stride;
+  col.reserve(std::max<size_t>(total * sizeof(float), 256));
+  if (total) {
+    k_color_records<<<(uint32_t)((total + 255) / 256), 256, 0, s>>>(dc.as<const float>(), sh.as<const float>(),
+                                                                   count, 3 * M, stride, col.as<float>());
+    DL_HIP(hipGetLastError());
+  }
+  DL_HIP(hipStreamSynchronize(s));
 }
 
 SceneView Scene::view() const {
   return SceneView{pos.as<const float>(), rot.as<const float>(), ls.as<const float>(), op.as<const float>(),
-                   dc.as<const float>(), sh.as<const float>(), count, bands};
+                   dc.as<const float>(), sh.as<const float>(), col.as<const float>(), count, bands};
 }
 
 // ---------------------------------------------------------------------------

@@ -475,6 +475,7 @@ void Scene::load_ply(const uint8_t *bytes, size_t len, bool flip_y) {
   count = n;
   bands = bands_;
   sh_coeffs = bands_ > 0 ? cpc : 0;
+  build_color_records();
 }
 
 void Scene::bounds(double lo[3], double hi[3]) {

@@ -10,7 +10,16 @@
 
 namespace dl {
 
-// Device-resident SplatData (src/types.ts:23-32), same array layouts.
+// Floats per splat of a scene's colour record: colorsDC (3) then the 3 * M
+// SH coefficients the renderer reads (channel-major, as colorsSH holds them),
+// zero-padded to 16 B, 64 B, 128 B and 192 B for bands 0-3, so that the
+// gather of one splat's colour data touches at most two 128-byte lines.
+__host__ __device__ constexpr uint32_t color_record_floats(uint32_t bands) {
+  return bands == 0 ? 4u : bands == 1 ? 16u : bands == 2 ? 32u : 48u;
+}
+
+// Device-resident SplatData (src/types.ts:23-32), same array layouts, plus
+// the colour records built from colorsDC / colorsSH (color_record_floats).
 struct SceneView {
   const float *pos;
   const float *rot;
@@ -18,6 +27,7 @@ struct SceneView {
   const float *op;
   const float *dc;
   const float *sh;
+  const float *col;
   uint32_t count;
   uint32_t bands;
 };
@@ -60,7 +70,10 @@ struct Scene {
   // colorsSH array the renderer reads with stride 3 * M like the reference
   uint32_t sh_coeffs = 0;
   DeviceBuffer pos, rot, ls, op, dc, sh;
+  DeviceBuffer col;  // colour records (color_record_floats(bands) per splat)
   void upload(const dl_splats &s);
+  // builds col from dc and sh (after upload / load_ply); waits for the stream
+  void build_color_records();
   // loadPlySplats (ply.ts:120-239) into this scene (ply.hip)
   void load_ply(const uint8_t *bytes, size_t len, bool flip_y);
   // splatBounds (ply.ts:242-255) of the resident positions

@@ -105,8 +105,9 @@ __device__ inline void color_from(const float *dc, const float *shc, float pcx, 
 
 template <int BANDS>
 __device__ inline void splat_color(const SceneView &sc, uint32_t id, float pcx, float pcy, float pcz, float col[3]) {
-  constexpr int M = BANDS == 0 ? 0 : BANDS == 1 ? 3 : BANDS == 2 ? 8 : 15;
-  color_from<BANDS>(sc.dc + 3 * (size_t)id, sc.sh + (size_t)id * 3 * M, pcx, pcy, pcz, col);
+  const float *r = static_cast<const float *>(
+      __builtin_assume_aligned(sc.col + (size_t)id * color_record_floats(BANDS), 16));
+  color_from<BANDS>(r, r + 3, pcx, pcy, pcz, col);
 }
 
 // One splat through project.ts:111-285. Returns the tile count; a culled
