@@ -517,7 +517,11 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
           const float power = gauss_power(b.x, b.y, b.z, dx, dy);
           if (power > 0.0f || power < col.w) continue;
-          const float alpha = wmin(kOpacityCap, b.w * wmax(0.0f, dl_expf_nonpos(power) - kGaussFloor));
+          // min(cap, opacity * max(0, exp(power) - floor)) (rasterize.ts:73)
+          // without the max: the record's opacity is sigmoid * fade * aa, so
+          // >= 0 or NaN, and for exp - floor <= 0 the product is <= 0 and
+          // fails the 1/255 test below exactly like 0 does (NaN stays NaN)
+          const float alpha = wmin(kOpacityCap, b.w * (dl_expf_nonpos(power) - kGaussFloor));
           if (alpha < kMinAlpha) continue;
           const float w = T[k] * alpha;
           cr[k] = __builtin_fmaf(w, col.x, cr[k]);
