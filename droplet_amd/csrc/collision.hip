@@ -32,7 +32,9 @@
 //   it with a one-block halo in LDS; each thread grows its block's reached
 //   set with shifts and masks (inside the block) and the neighbours' facing
 //   bit planes (across blocks) until the whole tile stops changing, then
-//   writes back what changed. Launches repeat until one changes nothing.
+//   writes back what changed. Launches repeat until one changes nothing;
+//   a tile runs only when it or a face neighbour changed since the launch
+//   before, so after the first launch only the moving front costs work.
 //   Reached sets only grow, so reading a neighbour's old or new word is
 //   always a subset of the answer and the fixed point is unique.
 //   Mesh: face bits are gathered into rows of 64-bit words along u (one
@@ -153,17 +155,37 @@ __global__ __launch_bounds__(256) void k_complement(const uint2 *__restrict__ in
   }
 }
 
-// One flood step over an 8x8x4-block tile; sets *changed when the tile's
-// reached set grew. `outside` is the reached set of blocks beyond the grid.
+// One flood step over an 8x8x4-block tile (launch number `launch`, from 1);
+// sets *changed and last_change[tile] = launch when the tile's reached set
+// grew. A tile runs only when it or a face neighbour changed in the previous
+// launch or later (every tile in launch 1): otherwise its inputs are what it
+// already saturated against. `outside` is the reached set of blocks beyond
+// the grid.
 __global__ __launch_bounds__(256) void k_flood(const uint2 *__restrict__ solid, uint2 *__restrict__ reach,
-                                               GridGeom g, uint32_t tiles_x, uint32_t tiles_y, uint64_t outside,
-                                               uint32_t *__restrict__ changed) {
+                                               GridGeom g, uint32_t tiles_x, uint32_t tiles_y, uint32_t tiles_z,
+                                               uint64_t outside, uint32_t launch,
+                                               uint32_t *__restrict__ last_change, uint32_t *__restrict__ changed) {
   __shared__ uint64_t R[kHalo];
+  __shared__ uint32_t s_active;
   const int32_t nbx = g.nb[0], nby = g.nb[1], nbz = g.nb[2];
   const uint32_t tile = blockIdx.x;
-  const int32_t ox = (int32_t)(tile % tiles_x) * kTX - 1;
-  const int32_t oy = (int32_t)((tile / tiles_x) % tiles_y) * kTY - 1;
-  const int32_t oz = (int32_t)(tile / (tiles_x * tiles_y)) * kTZ - 1;
+  const uint32_t tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, tz = tile / (tiles_x * tiles_y);
+  if (threadIdx.x == 0) {
+    // one thread decides, so that the whole workgroup returns or none does
+    const uint32_t need = launch - 1u;
+    const uint32_t sxy = tiles_x * tiles_y;
+    bool act = launch <= 1u || last_change[tile] >= need;
+    act = act || (tx > 0 && last_change[tile - 1] >= need) || (tx + 1 < tiles_x && last_change[tile + 1] >= need);
+    act = act || (ty > 0 && last_change[tile - tiles_x] >= need) ||
+          (ty + 1 < tiles_y && last_change[tile + tiles_x] >= need);
+    act = act || (tz > 0 && last_change[tile - sxy] >= need) || (tz + 1 < tiles_z && last_change[tile + sxy] >= need);
+    s_active = act ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!s_active) return;
+  const int32_t ox = (int32_t)tx * kTX - 1;
+  const int32_t oy = (int32_t)ty * kTY - 1;
+  const int32_t oz = (int32_t)tz * kTZ - 1;
   for (int i = threadIdx.x; i < kHalo; i += kFloodThreads) {
     const int32_t bx = ox + i % kHX, by = oy + (i / kHX) % kHY, bz = oz + i / (kHX * kHY);
     uint64_t w = outside;
@@ -203,8 +225,10 @@ __global__ __launch_bounds__(256) void k_flood(const uint2 *__restrict__ solid, 
     }
     if (!__syncthreads_or(grew)) break;
   }
-  if (mine && r != r0) {
-    reach[blk] = as_uint2(r);
+  const bool any = __syncthreads_or(mine && r != r0);
+  if (mine && r != r0) reach[blk] = as_uint2(r);
+  if (any && threadIdx.x == 0) {
+    last_change[tile] = launch;
     *changed = 1u;
   }
 }
@@ -416,23 +440,31 @@ void flood_impl(Context &ctx, const VoxelResult &in, bool fill, const double *po
   const uint32_t tz = (uint32_t)div_up((uint32_t)g.nb[2], kTZ);
   const uint64_t tiles = (uint64_t)tx * ty * tz;
   if (tiles > 0x7FFFFFFFull) fail(DL_ERR_CAPACITY, "voxel grid too large for the flood");
-  constexpr uint32_t kBatch = 8;
+  // launches between reads of the flags: 8, 16, 32, then 64 (a launch that
+  // finds no active tile costs a few microseconds)
+  constexpr uint32_t kMaxBatch = 64;
   constexpr uint64_t kMaxLaunches = 1000000;
-  flags.reserve(kBatch * 4);
+  DeviceBuffer last_change;
+  flags.reserve(kMaxBatch * 4);
+  last_change.reserve(tiles * 4);
+  k_fill_u32<<<(uint32_t)((tiles + 63) / 64), 64, 0, s>>>(last_change.as<uint32_t>(), (uint32_t)tiles, 0u);
+  DL_HIP(hipGetLastError());
   const uint64_t outside = fill ? ~0ull : 0ull;
-  uint32_t h_flags[kBatch];
-  for (uint64_t launched = 0;; launched += kBatch) {
-    if (launched >= kMaxLaunches) fail(DL_ERR_INTERNAL, "flood did not converge");
-    k_fill_u32<<<1, 64, 0, s>>>(flags.as<uint32_t>(), kBatch, 0u);
+  uint32_t h_flags[kMaxBatch];
+  uint32_t launch = 0, batch = 8;
+  for (;; batch = std::min(2 * batch, kMaxBatch)) {
+    if (launch >= kMaxLaunches) fail(DL_ERR_INTERNAL, "flood did not converge");
+    k_fill_u32<<<1, 64, 0, s>>>(flags.as<uint32_t>(), batch, 0u);
     DL_HIP(hipGetLastError());
-    for (uint32_t i = 0; i < kBatch; i++) {
-      k_flood<<<(uint32_t)tiles, kFloodThreads, 0, s>>>(in.masks.as<uint2>(), reach.as<uint2>(), g, tx, ty, outside,
+    for (uint32_t i = 0; i < batch; i++) {
+      k_flood<<<(uint32_t)tiles, kFloodThreads, 0, s>>>(in.masks.as<uint2>(), reach.as<uint2>(), g, tx, ty, tz,
+                                                        outside, ++launch, last_change.as<uint32_t>(),
                                                         flags.as<uint32_t>() + i);
       DL_HIP(hipGetLastError());
     }
-    DL_HIP(hipMemcpyAsync(h_flags, flags.get(), sizeof(h_flags), hipMemcpyDeviceToHost, s));
+    DL_HIP(hipMemcpyAsync(h_flags, flags.get(), (size_t)batch * 4, hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
-    if (!h_flags[kBatch - 1]) break;
+    if (!h_flags[batch - 1]) break;
   }
   out.masks.reserve(total * 8);
   k_complement<<<fill_blocks, 256, 0, s>>>(reach.as<uint2>(), out.masks.as<uint2>(), total);
