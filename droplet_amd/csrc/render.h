@@ -111,6 +111,12 @@ struct Renderer {
   // with the exact totals. spec_cap is 0 until a frame has run.
   uint64_t spec_cap = 0;
   uint64_t last_pairs_all = 0;
+  // Bits of the visible depth keys' span (max - min + 1, plus headroom)
+  // seen in the last frame: a speculative frame sorts on the keys less their
+  // minimum with that many bits, in fewer passes, and is rendered again with
+  // the full 32-bit sort when its own span did not fit. 0 until a frame ran.
+  uint32_t spec_key_bits = 0;
+  uint32_t key_bits_used = 0;  // bits of the current frame's mapped sort (0: raw 32-bit keys)
   bool force_exact = false;
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)

@@ -37,21 +37,29 @@ constexpr int kProjBlock = 256;
 // One thread per splat in id order, so every scene array is read coalesced.
 // Per-block partial sums of visible splats and tile counts replace any
 // same-address atomics.
+// Per-block partials: visible splats, tile pairs, and the least and greatest
+// depth key of the visible splats (0xFFFFFFFF / 0 when none).
+struct CullParts {
+  uint32_t *vis;
+  unsigned long long *pairs;
+  uint32_t *kmin, *kmax;
+};
+
 template <int BANDS, bool COLOR>
 __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjUniforms u, double near_plane,
                                                              double far_plane, uint32_t *__restrict__ keys,
                                                              uint32_t *__restrict__ ids, float4 *__restrict__ rec,
                                                              uint32_t *__restrict__ counts,
-                                                             uint2 *__restrict__ rect8,
-                                                             uint32_t *__restrict__ part_vis,
-                                                             unsigned long long *__restrict__ part_pairs) {
-  __shared__ uint32_t s_vis[kProjBlock / 64];
+                                                             uint2 *__restrict__ rect8, CullParts parts) {
+  __shared__ uint32_t s_vis[kProjBlock / 64], s_kmin[kProjBlock / 64], s_kmax[kProjBlock / 64];
   __shared__ unsigned long long s_pairs[kProjBlock / 64];
   const uint32_t i = blockIdx.x * kProjBlock + threadIdx.x;
   bool vis = false;
   uint32_t count = 0;
+  uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
   if (i < sc.count) {
     cull_project_one<BANDS, COLOR>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
+    if (vis) kmin = kmax = keys[i];
     // packed tile rect for the live-splat compaction of span binning
     if (rect8) rect8[i] = pack_rect8(splat_rect(rec[3 * (size_t)i], u.tiles_x, u.tiles_y, u.max_cov));
   }
@@ -61,55 +69,75 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
   for (int d = 32; d >= 1; d >>= 1) {
     v += __shfl_xor(v, d, 64);
     pc += __shfl_xor(pc, d, 64);
+    kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, d, 64));
+    kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, d, 64));
   }
   if (__lane_id() == 0) {
     s_vis[threadIdx.x >> 6] = v;
     s_pairs[threadIdx.x >> 6] = pc;
+    s_kmin[threadIdx.x >> 6] = kmin;
+    s_kmax[threadIdx.x >> 6] = kmax;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t tv = 0;
+    uint32_t tv = 0, tmin = 0xFFFFFFFFu, tmax = 0u;
     unsigned long long tp = 0;
 #pragma unroll
     for (int w = 0; w < kProjBlock / 64; w++) {
       tv += s_vis[w];
       tp += s_pairs[w];
+      tmin = min(tmin, s_kmin[w]);
+      tmax = max(tmax, s_kmax[w]);
     }
-    part_vis[blockIdx.x] = tv;
-    part_pairs[blockIdx.x] = tp;
+    parts.vis[blockIdx.x] = tv;
+    parts.pairs[blockIdx.x] = tp;
+    parts.kmin[blockIdx.x] = tmin;
+    parts.kmax[blockIdx.x] = tmax;
   }
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_totals(const uint32_t *__restrict__ part_vis,
-                                                        const unsigned long long *__restrict__ part_pairs,
-                                                        uint32_t nparts, uint32_t *__restrict__ d_nvis,
+// Frame totals from the cull's partials: d_nvis[0] visible splats,
+// d_nvis[2] / d_nvis[3] the least / greatest visible depth key, *d_pairs the
+// tile pairs.
+__global__ __launch_bounds__(1024) void k_reduce_totals(CullParts parts, uint32_t nparts,
+                                                        uint32_t *__restrict__ d_nvis,
                                                         unsigned long long *__restrict__ d_pairs) {
-  __shared__ uint32_t s_v[16];
+  __shared__ uint32_t s_v[16], s_lo[16], s_hi[16];
   __shared__ unsigned long long s_p[16];
-  uint32_t v = 0;
+  uint32_t v = 0, lo = 0xFFFFFFFFu, hi = 0u;
   unsigned long long p = 0;
   for (uint32_t k = threadIdx.x; k < nparts; k += 1024) {
-    v += part_vis[k];
-    p += part_pairs[k];
+    v += parts.vis[k];
+    p += parts.pairs[k];
+    lo = min(lo, parts.kmin[k]);
+    hi = max(hi, parts.kmax[k]);
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     v += __shfl_xor(v, d, 64);
     p += __shfl_xor(p, d, 64);
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
   }
   if (__lane_id() == 0) {
     s_v[threadIdx.x >> 6] = v;
     s_p[threadIdx.x >> 6] = p;
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t tv = 0;
+    uint32_t tv = 0, tlo = 0xFFFFFFFFu, thi = 0u;
     unsigned long long tp = 0;
     for (int w = 0; w < 16; w++) {
       tv += s_v[w];
       tp += s_p[w];
+      tlo = min(tlo, s_lo[w]);
+      thi = max(thi, s_hi[w]);
     }
-    *d_nvis = tv;
+    d_nvis[0] = tv;
+    d_nvis[2] = tlo;
+    d_nvis[3] = thi;
     *d_pairs = tp;
   }
 }
@@ -192,7 +220,7 @@ void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float
 }
 
 using CullProjFn = void (*)(SceneView, ProjUniforms, double, double, uint32_t *, uint32_t *, float4 *, uint32_t *,
-                            uint2 *, uint32_t *, unsigned long long *);
+                            uint2 *, CullParts);
 
 CullProjFn cull_project_for(uint32_t bands, bool color) {
   switch (bands) {
@@ -347,23 +375,40 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     rec.reserve((size_t)n * 48);
     if (want_rect8) rect8.reserve((size_t)n * 8);
     counts.reserve((size_t)n * 4);
-    parts.reserve((size_t)nparts * 12 + 256);
+    const size_t pstride = ((size_t)nparts * 8 + 255) & ~(size_t)255;
+    parts.reserve(4 * pstride);
     sort_ws.reserve(radix_workspace_bytes(n, 32));
-    uint32_t *part_vis = parts.as<uint32_t>();
-    unsigned long long *part_pairs =
-        reinterpret_cast<unsigned long long *>(parts.as<uint8_t>() + (((size_t)nparts * 4 + 255) & ~(size_t)255));
+    CullParts cp;
+    cp.pairs = parts.as<unsigned long long>();
+    cp.vis = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + pstride);
+    cp.kmin = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 2 * pstride);
+    cp.kmax = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 3 * pstride);
     mark(DL_STAGE_CULL);
     cull_project_for(sc.bands, !defer_color)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
                                                              keys.as<uint32_t>(), ids.as<uint32_t>(), rec.as<float4>(),
                                                              counts.as<uint32_t>(),
-                                                             want_rect8 ? rect8.as<uint2>() : nullptr, part_vis,
-                                                             part_pairs);
+                                                             want_rect8 ? rect8.as<uint2>() : nullptr, cp);
     DL_HIP(hipGetLastError());
-    k_reduce_totals<<<1, 1024, 0, s>>>(part_vis, part_pairs, nparts, d_nvis, d_pairs_all);
+    k_reduce_totals<<<1, 1024, 0, s>>>(cp, nparts, d_nvis, d_pairs_all);
     DL_HIP(hipGetLastError());
     mark(DL_STAGE_DEPTH_SORT);
+    // A speculative frame sorts on key - (least visible key) over the span
+    // bits of the last frame, with up to 9-bit digits, when that takes fewer
+    // passes; invisible splats map past every visible key, so the order is
+    // the 32-bit sort's whenever the span fits (checked after the frame).
+    int sort_bits = 32, sort_digits = digit_bits();
+    RadixKeyMap km{};
+    if (speculative && spec_key_bits > 0 && spec_key_bits < 32) {
+      const int md = digit_bits() >= 8 ? 9 : digit_bits();
+      if (radix_passes((int)spec_key_bits, md) < radix_passes(32, digit_bits())) {
+        sort_bits = (int)spec_key_bits;
+        sort_digits = md;
+        km = RadixKeyMap{d_nvis + 2, kInvisibleKey, (uint32_t)((1ull << spec_key_bits) - 1ull)};
+      }
+    }
+    key_bits_used = km.kmin ? (uint32_t)sort_bits : 0u;
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
-                                         tids.as<uint32_t>(), n, 32, sort_ws.get(), digit_bits());
+                                         tids.as<uint32_t>(), n, sort_bits, sort_ws.get(), sort_digits, km);
     sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
     const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
     if (speculative) {
@@ -568,11 +613,22 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       last_pairs_all = all;
       if (all <= 0xFFFFFFF0ull)
         spec_cap = std::max<uint64_t>(spec_cap, std::min<uint64_t>(all + all / 8 + 4096, 0xFFFFFFF0ull));
+      // span of the visible depth keys: (kmax - kmin + 1) and the sentinel
+      // must fit the sort's bits
+      bool keys_fit = true;
+      if (sm[0] > 0 && sm[2] <= sm[3]) {
+        const uint64_t span = (uint64_t)sm[3] - (uint64_t)sm[2] + 1ull;
+        uint32_t need = 0;
+        while (need < 33 && (span >> need) != 0) need++;
+        keys_fit = key_bits_used == 0 || need <= key_bits_used;
+        spec_key_bits = std::min<uint32_t>(32, need + 1);
+      }
       if (speculative) {
         stats.num_visible = sm[0];
         stats.num_pairs = all;
-        if (all > pairs_all) {
-          // the lists were cut at the capacity: render again with exact totals
+        if (all > pairs_all || !keys_fit) {
+          // the lists were cut at the capacity, or the depth keys spanned
+          // more bits than the sort used: render again with exact totals
           struct Reset {
             bool &f;
             ~Reset() { f = false; }

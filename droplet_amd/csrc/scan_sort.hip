@@ -21,6 +21,9 @@ constexpr int kBlock = 256;
 constexpr int kItems = 16;
 constexpr int kTileN = kBlock * kItems;  // 4096
 constexpr int kWaves = kBlock / 64;
+// digits of one radix pass: at most 9 bits
+constexpr int kMaxDigitBits = 9;
+constexpr uint32_t kMaxDigits = 1u << kMaxDigitBits;
 // Up to this many tiles, a scan's down-sweep sums the earlier tiles' totals
 // itself (at most 16 loads per thread from L2) instead of a separate scan of
 // the totals.
@@ -198,25 +201,31 @@ __device__ inline TileSpan tile_span(const SegGeom &g, size_t n, uint32_t nblock
   return t;
 }
 
+// The key whose digits a pass sorts on (RadixKeyMap)
+__device__ inline uint32_t mapped_key(uint32_t k, const RadixKeyMap &m, uint32_t kmin) {
+  return m.kmin ? (k == m.invisible ? m.sentinel : k - kmin) : k;
+}
+
 // Tile layout: wave w owns elements [w*1024, (w+1)*1024) of the tile; item k
 // of lane l is element w*1024 + k*64 + l. Loads are coalesced and the
 // (k, lane) processing order is the input order, which makes ranks stable.
 __global__ __launch_bounds__(kBlock) void k_radix_upsweep(const uint32_t *__restrict__ keys, size_t n, int shift,
                                                           int bits, uint32_t *__restrict__ counts,
-                                                          uint32_t nblocks, SegGeom g) {
-  __shared__ uint32_t s_hist[256];
+                                                          uint32_t nblocks, SegGeom g, RadixKeyMap km) {
+  __shared__ uint32_t s_hist[kMaxDigits];
   const uint32_t mask = (1u << bits) - 1u;
   const TileSpan ts = tile_span(g, n, nblocks, blockIdx.x, mask + 1u);
   if (ts.n == 0 && g.seg_start) return;  // unused capacity block
-  s_hist[threadIdx.x] = 0;
+  const uint32_t kmin = km.kmin ? *km.kmin : 0u;
+  for (uint32_t d = threadIdx.x; d < kMaxDigits; d += kBlock) s_hist[d] = 0;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
     const uint32_t li = (uint32_t)k * kBlock + threadIdx.x;
-    if (li < ts.n) atomicAdd(&s_hist[(keys[ts.i0 + li] >> shift) & mask], 1u);
+    if (li < ts.n) atomicAdd(&s_hist[(mapped_key(keys[ts.i0 + li], km, kmin) >> shift) & mask], 1u);
   }
   __syncthreads();
-  if (threadIdx.x <= mask) counts[ts.cbase + (size_t)threadIdx.x * ts.cstride] = s_hist[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d <= mask; d += kBlock) counts[ts.cbase + (size_t)d * ts.cstride] = s_hist[d];
 }
 
 template <int BITS>
@@ -225,14 +234,17 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
                                                             uint32_t *__restrict__ keys_out,
                                                             uint32_t *__restrict__ vals_out, size_t n, int shift,
                                                             const uint32_t *__restrict__ offsets,
-                                                            uint32_t nblocks, SegGeom g) {
+                                                            uint32_t nblocks, SegGeom g, RadixKeyMap km) {
   constexpr uint32_t kDigits = 1u << BITS;
   constexpr uint32_t kMask = kDigits - 1u;
-  __shared__ uint32_t s_whist[kWaves][256];
+  // digit slots: one per thread, two per thread for 9-bit digits
+  constexpr uint32_t kSlots = kDigits > (uint32_t)kBlock ? kDigits : (uint32_t)kBlock;
+  constexpr uint32_t kPer = kSlots / kBlock;
+  __shared__ uint32_t s_whist[kWaves][kSlots];
   __shared__ uint32_t s_keys[kTileN];
   __shared__ uint32_t s_vals[kTileN];
-  __shared__ uint32_t s_start[256];
-  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_start[kSlots];
+  __shared__ uint32_t s_gbase[kSlots];
   __shared__ uint32_t s_wave[kWaves];
 
   const int lane = __lane_id();
@@ -241,8 +253,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   if (ts.n == 0) return;
   const size_t tile0 = ts.i0;
   const uint32_t tile_n = ts.n;
+  const uint32_t kmin = km.kmin ? *km.kmin : 0u;
 
-  for (int d = threadIdx.x; d < kWaves * 256; d += kBlock) (&s_whist[0][0])[d] = 0;
+  for (uint32_t d = threadIdx.x; d < kWaves * kSlots; d += kBlock) (&s_whist[0][0])[d] = 0;
   __syncthreads();
 
   uint32_t key[kItems], val[kItems], rank[kItems];
@@ -253,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
     const bool valid = li < tile_n;
     key[k] = valid ? keys_in[tile0 + li] : 0u;
     val[k] = valid ? vals_in[tile0 + li] : 0u;
-    const uint32_t d = (key[k] >> shift) & kMask;
+    const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
     uint64_t peers = __ballot(valid);
     if (!valid) peers = 0;
 #pragma unroll
@@ -273,27 +286,39 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   }
   __syncthreads();
   // per digit: wave offsets and block count, then block-wide digit starts
+  // (thread t owns digits t*kPer .. t*kPer + kPer - 1, in digit order)
   {
-    const uint32_t d = threadIdx.x;
-    uint32_t cnt = 0;
+    uint32_t cnt[kPer];
+    uint32_t tsum = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) {
-      uint32_t c = s_whist[w][d];
-      s_whist[w][d] = cnt;
-      cnt += c;
+    for (uint32_t j = 0; j < kPer; j++) {
+      const uint32_t d = threadIdx.x * kPer + j;
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; w++) {
+        const uint32_t x = s_whist[w][d];
+        s_whist[w][d] = c;
+        c += x;
+      }
+      cnt[j] = d < kDigits ? c : 0u;
+      tsum += cnt[j];
     }
-    if (d >= kDigits) cnt = 0;
     uint32_t total;
-    uint32_t start = block_exclusive_scan(cnt, s_wave, &total);
-    s_start[d] = start;
-    if (d < kDigits) s_gbase[d] = offsets[ts.cbase + (size_t)d * ts.cstride];
+    uint32_t start = block_exclusive_scan(tsum, s_wave, &total);
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) {
+      const uint32_t d = threadIdx.x * kPer + j;
+      s_start[d] = start;
+      start += cnt[j];
+      if (d < kDigits) s_gbase[d] = offsets[ts.cbase + (size_t)d * ts.cstride];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
     const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
     if (li < tile_n) {
-      const uint32_t d = (key[k] >> shift) & kMask;
+      const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
       const uint32_t pos = s_start[d] + s_whist[wave][d] + rank[k];
       s_keys[pos] = key[k];
       s_vals[pos] = val[k];
@@ -305,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
     const uint32_t p = (uint32_t)k * kBlock + threadIdx.x;
     if (p < tile_n) {
       const uint32_t kk = s_keys[p];
-      const uint32_t d = (kk >> shift) & kMask;
+      const uint32_t d = (mapped_key(kk, km, kmin) >> shift) & kMask;
       const size_t dst = (size_t)s_gbase[d] + (p - s_start[d]);
       keys_out[dst] = kk;
       vals_out[dst] = s_vals[p];
@@ -351,7 +376,7 @@ __global__ __launch_bounds__(256) void k_unit_owner(const uint32_t *__restrict__
 }
 
 using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
-                        const uint32_t *, uint32_t, SegGeom);
+                        const uint32_t *, uint32_t, SegGeom, RadixKeyMap);
 
 DownFn downsweep_for(int bits) {
   switch (bits) {
@@ -362,6 +387,7 @@ DownFn downsweep_for(int bits) {
     case 5: return k_radix_downsweep<5>;
     case 6: return k_radix_downsweep<6>;
     case 7: return k_radix_downsweep<7>;
+    case 9: return k_radix_downsweep<9>;
     default: return k_radix_downsweep<8>;
   }
 }
@@ -411,7 +437,7 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
 }
 
 void pass_layout(int key_bits, int max_bits, int *passes, int bits[32]) {
-  max_bits = std::min(8, std::max(1, max_bits));
+  max_bits = std::min(kMaxDigitBits, std::max(1, max_bits));
   int p = std::max(1, (key_bits + max_bits - 1) / max_bits);
   int per = (key_bits + p - 1) / p;
   int left = key_bits;
@@ -448,17 +474,17 @@ void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_
 size_t radix_workspace_bytes(size_t n, int key_bits) {
   (void)key_bits;
   const size_t nb = std::max<size_t>(1, div_up(n, kTileN));
-  const size_t counts = ((256 * nb * sizeof(uint32_t) + 255) / 256) * 256;
-  return counts * 2 + scan_levels_bytes(256 * nb);
+  const size_t counts = ((kMaxDigits * nb * sizeof(uint32_t) + 255) / 256) * 256;
+  return counts * 2 + scan_levels_bytes(kMaxDigits * nb);
 }
 
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
-                      size_t n, int key_bits, void *ws, int max_digit_bits) {
+                      size_t n, int key_bits, void *ws, int max_digit_bits, const RadixKeyMap &map) {
   if (n <= 1 || key_bits <= 0) return false;
   if (key_bits > 32) key_bits = 32;
   const uint32_t nb = div_up(n, kTileN);
   uint8_t *w = static_cast<uint8_t *>(ws);
-  const size_t counts_bytes = ((256 * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
+  const size_t counts_bytes = ((kMaxDigits * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
   uint32_t *counts = reinterpret_cast<uint32_t *>(w);
   uint32_t *offsets = reinterpret_cast<uint32_t *>(w + counts_bytes);
   uint8_t *scan_ws = w + 2 * counts_bytes;
@@ -469,10 +495,10 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
   const SegGeom none{};
   for (int p = 0; p < passes; p++) {
     const size_t ndig = (size_t)1 << bits[p];
-    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none);
+    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb, none);
+    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb, none, map);
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
@@ -522,15 +548,16 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
   DL_HIP(hipGetLastError());
   const SegGeom g{d_seg_start, blk0, blk_seg, nseg};
   int passes, bits[32];
-  pass_layout(key_bits, max_digit_bits, &passes, bits);
+  // the segmented counts are sized for 8-bit digits
+  pass_layout(key_bits, std::min(max_digit_bits, 8), &passes, bits);
   uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
   int shift = 0;
   for (int p = 0; p < passes; p++) {
     const size_t ndig = (size_t)1 << bits[p];
-    k_radix_upsweep<<<mb, kBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g);
+    k_radix_upsweep<<<mb, kBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<mb, kBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g);
+    downsweep_for(bits[p])<<<mb, kBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);

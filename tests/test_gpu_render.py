@@ -13,7 +13,7 @@ import pytest
 import oracle
 from droplet_amd import synthetic as S
 from droplet_amd.renderer import Renderer, RendererOptions, Scene, debug_math, scan_u32, sort_pairs_u32
-from droplet_amd.types import BackgroundRGBA
+from droplet_amd.types import BackgroundRGBA, SplatData
 
 pytestmark = pytest.mark.gpu
 
@@ -323,6 +323,36 @@ def test_repeated_frames_match_oracle(device, bands):
         img = r.render(big, cam)
         np.testing.assert_array_equal(img, oracle.render(big, cam, taps=False)["rgba"])
         assert r.stats()["num_pairs"] > ref["num_pairs"]
+    finally:
+        r.destroy()
+
+
+def test_depth_key_span_speculation_matches_oracle(device):
+    """After a frame, the renderer sorts on depth keys less their minimum over
+    the last frame's key span (three 9-bit passes instead of four 8-bit
+    ones). The order equals the 32-bit sort's while the span fits; a frame
+    whose keys span more bits is rendered again with the full sort. Splats
+    behind the camera (invisible) must still sort after every visible one."""
+    cam = S.default_camera(320, 180)
+    # a narrow depth range (22-bit key span), with every 50th splat moved
+    # behind the camera (z = 6 > 4): invisible splats inside a short span
+    narrow = S.gaussian_cloud(60_000, seed=91, sh_bands=1, pos_std=0.15, pos_clip=0.4)
+    pos = narrow.positions.reshape(-1, 3).copy()
+    pos[::50, 2] = 6.0
+    narrow = SplatData(narrow.count, pos.reshape(-1), narrow.rotations, narrow.log_scales, narrow.opacity_logits,
+                       narrow.colors_dc, narrow.colors_sh, narrow.sh_bands)
+    wide = S.gaussian_cloud(60_000, seed=92, sh_bands=1, pos_std=3.0, pos_clip=9.0)  # 27-bit span
+    r = Renderer(device)
+    try:
+        for name, data, frames in (("narrow", narrow, 3), ("wide", wide, 3), ("narrow", narrow, 2)):
+            ref = oracle.render(data, cam, taps=False)
+            assert ref["num_visible"] < data.count
+            for frame in range(frames):
+                img = r.render(data, cam)
+                st = r.stats()
+                print(f"case: span {name} frame={frame} visible={st['num_visible']}", flush=True)
+                np.testing.assert_array_equal(img, ref["rgba"])
+                assert st["num_visible"] == ref["num_visible"]
     finally:
         r.destroy()
 
