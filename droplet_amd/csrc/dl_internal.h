@@ -92,7 +92,8 @@ void exclusive_scan_u32_dev(hipStream_t s, const uint32_t *d_in, uint32_t *d_out
 
 // Stable LSD radix sort of (key, value) pairs on bits [0, key_bits) of the
 // key. Results end in either (keys, vals) or (tmp_keys, tmp_vals); the
-// return value is true when they are in the tmp buffers.
+// return value is true when they are in the tmp buffers. With index_vals
+// the input values are not read: value i is i (vals is still scratch).
 // Digits are at most max_digit_bits (1..9) wide; passes = ceil(key_bits /
 // max_digit_bits).
 // With map.kmin set, the digits are those of (key == map.invisible ?
@@ -107,7 +108,7 @@ struct RadixKeyMap {
 size_t radix_workspace_bytes(size_t n, int key_bits);
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
                       uint32_t *tmp_vals, size_t n, int key_bits, void *ws, int max_digit_bits = 8,
-                      const RadixKeyMap &map = RadixKeyMap{});
+                      const RadixKeyMap &map = RadixKeyMap{}, bool index_vals = false);
 int radix_passes(int key_bits, int max_digit_bits);
 
 // Segment table of the segmented sort (device pointers): segments are the

@@ -385,8 +385,10 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     cp.kmax = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 3 * pstride);
     mark(DL_STAGE_CULL);
     cull_project_for(sc.bands, !defer_color)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
-                                                             keys.as<uint32_t>(), ids.as<uint32_t>(), rec.as<float4>(),
-                                                             counts.as<uint32_t>(),
+                                                             keys.as<uint32_t>(),
+                                                             speculative ? nullptr : ids.as<uint32_t>(),
+                                                             rec.as<float4>(),
+                                                             speculative ? nullptr : counts.as<uint32_t>(),
                                                              want_rect8 ? rect8.as<uint2>() : nullptr, cp);
     DL_HIP(hipGetLastError());
     k_reduce_totals<<<1, 1024, 0, s>>>(cp, nparts, d_nvis, d_pairs_all);
@@ -407,8 +409,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       }
     }
     key_bits_used = km.kmin ? (uint32_t)sort_bits : 0u;
+    // a speculative frame needs no per-splat tile counts (span binning
+    // counts its own) and lets the sort's first pass make the ids 0..n-1
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
-                                         tids.as<uint32_t>(), n, sort_bits, sort_ws.get(), sort_digits, km);
+                                         tids.as<uint32_t>(), n, sort_bits, sort_ws.get(), sort_digits, km,
+                                         speculative);
     sorted_ids = in_tmp ? tids.as<uint32_t>() : ids.as<uint32_t>();
     const uint32_t *sorted_keys = in_tmp ? tkeys.as<uint32_t>() : keys.as<uint32_t>();
     if (speculative) {

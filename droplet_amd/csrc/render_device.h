@@ -280,7 +280,7 @@ __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms 
     depth_f = (float)__builtin_sqrt(r2);
   }
   keys[o] = vis ? sortable_key(depth_f) : kInvisibleKey;
-  vals[o] = val;
+  if (vals) vals[o] = val;
   float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f), o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o2 = o1;
   count = 0;
   if (vis) count = project_splat<BANDS, COLOR>(sc, i, u, o0, o1, o2);

@@ -265,7 +265,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
     const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
     const bool valid = li < tile_n;
     key[k] = valid ? keys_in[tile0 + li] : 0u;
-    val[k] = valid ? vals_in[tile0 + li] : 0u;
+    // without vals_in the values are the elements' indices
+    val[k] = valid ? (vals_in ? vals_in[tile0 + li] : (uint32_t)(tile0 + li)) : 0u;
     const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
     uint64_t peers = __ballot(valid);
     if (!valid) peers = 0;
@@ -479,7 +480,8 @@ size_t radix_workspace_bytes(size_t n, int key_bits) {
 }
 
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
-                      size_t n, int key_bits, void *ws, int max_digit_bits, const RadixKeyMap &map) {
+                      size_t n, int key_bits, void *ws, int max_digit_bits, const RadixKeyMap &map,
+                      bool index_vals) {
   if (n <= 1 || key_bits <= 0) return false;
   if (key_bits > 32) key_bits = 32;
   const uint32_t nb = div_up(n, kTileN);
@@ -498,7 +500,8 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
     k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, vi, ko, vo, n, shift, offsets, nb, none, map);
+    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko, vo, n, shift,
+                                                 offsets, nb, none, map);
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
