@@ -125,6 +125,8 @@ struct SpanBinArgs {
   // min(*d_items, one.n)) and ceil(items / 256); null: one.n items
   const uint32_t *d_items = nullptr;
   const uint32_t *d_nb1 = nullptr;
+  // when set, the pass's list entries are added to *entries on the device
+  unsigned long long *entries = nullptr;
 };
 
 // Per-tile lists of every virtual tile: d_off[0..rows*cols] (lower-bound
@@ -141,6 +143,6 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
 // rect enter level 1 (a stable compaction); the lists are the same.
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
-              uint32_t *pvals, uint32_t *d_small);
+              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries = nullptr);
 
 }  // namespace dl

@@ -533,12 +533,14 @@ __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restri
                                                         uint32_t chunk_cap, uint32_t n_out, uint32_t groups,
                                                         uint32_t tiles_x, int tile_mode,
                                                         const uint32_t *__restrict__ d_total, uint32_t out_cap,
-                                                        uint32_t *__restrict__ out) {
+                                                        uint32_t *__restrict__ out,
+                                                        unsigned long long *__restrict__ acc) {
   const uint32_t o = blockIdx.x * 256 + threadIdx.x;
   if (o > n_out) return;
   const uint32_t total = min(*d_total, out_cap);
   if (o == n_out) {
     out[o] = total;
+    if (acc) *acc += *d_total;
     return;
   }
   const size_t ncounts = (size_t)kK * min(chunk_base[nseg], chunk_cap);
@@ -694,7 +696,7 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   ws.seg_start_b.reserve(((size_t)nseg_b + 1) * 4);
   k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(
       ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), a.rows, max_ch_a, nseg_b, groups, a.cols, 0,
-      d_num_gspans, cap, ws.seg_start_b.as<uint32_t>());
+      d_num_gspans, cap, ws.seg_start_b.as<uint32_t>(), nullptr);
   DL_HIP(hipGetLastError());
 
   // level 2b: group spans -> columns (segments = (row, group))
@@ -716,7 +718,7 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   DL_HIP(hipGetLastError());
   k_bucket_offsets<<<div_up((size_t)num_vtiles + 1, 256), 256, 0, s>>>(
       ws.base3.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), nseg_b, max_ch_b, num_vtiles, groups, a.cols, 1,
-      d_npairs, cap, d_off);
+      d_npairs, cap, d_off, a.entries);
   DL_HIP(hipGetLastError());
 }
 
@@ -729,7 +731,7 @@ void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1
 
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
-              uint32_t *pvals, uint32_t *d_small) {
+              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries) {
   ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
   const bool compact = live.pre && rect8 && cn > 0;
   const uint32_t nblk = div_up(cn, 256);
@@ -776,6 +778,7 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     a.d_items = ws.live_small.as<uint32_t>();
     a.d_nb1 = ws.live_small.as<uint32_t>() + 1;
   }
+  a.entries = d_entries;
   span_bin_rects(s, a, ws, d_off, pvals, d_small);
 }
 

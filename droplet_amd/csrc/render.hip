@@ -200,10 +200,6 @@ __global__ void k_zero_u64(unsigned long long *__restrict__ p) {
   if (threadIdx.x == 0) *p = 0;
 }
 
-__global__ void k_add_entries(const uint32_t *__restrict__ n, unsigned long long *__restrict__ acc) {
-  if (threadIdx.x == 0) *acc += *n;
-}
-
 __global__ void k_add_const(uint32_t n, unsigned long long *__restrict__ acc) {
   if (threadIdx.x == 0) *acc += n;
 }
@@ -502,7 +498,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       const uint32_t *cids = sorted_ids + c0;
       stats.num_chunks++;
       // ---- pair count of this chunk (and scanned offsets for emit) ----
-      mark(DL_STAGE_SCAN);
+      if (exact_counts) mark(DL_STAGE_SCAN);
       uint64_t npairs64 = pairs_all;  // bound of every chunk
       if (exact_counts) {
         chunk_counts.reserve((size_t)cn * 4);
@@ -546,9 +542,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         const LiveMap live = (track_closed && c > 0) ? LiveMap{open_pre.as<uint16_t>(), tiles_x + 1} : LiveMap{};
         span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, live,
                  want_rect8 ? rect8.as<uint2>() : nullptr, n, bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(),
-                 d_bin);
-        k_add_entries<<<1, 64, 0, s>>>(d_bin + 2, d_entries);
-        DL_HIP(hipGetLastError());
+                 d_bin, d_entries);
         if (defer_color) {
           mark(DL_STAGE_PROJECT);
           if (c == 0)
