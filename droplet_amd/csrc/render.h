@@ -117,6 +117,9 @@ struct Renderer {
   // the full 32-bit sort when its own span did not fit. 0 until a frame ran.
   uint32_t spec_key_bits = 0;
   uint32_t key_bits_used = 0;  // bits of the current frame's mapped sort (0: raw 32-bit keys)
+  // extra bits over the last span: 0, raised after each frame whose span
+  // outgrew the bits (up to 2), so a growing range stops re-rendering
+  uint32_t key_headroom = 0;
   bool force_exact = false;
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)

@@ -106,6 +106,8 @@ __global__ __launch_bounds__(1024) void k_reduce_totals(CullParts parts, uint32_
   __shared__ unsigned long long s_p[16];
   uint32_t v = 0, lo = 0xFFFFFFFFu, hi = 0u;
   unsigned long long p = 0;
+  // independent iterations, unrolled so that their loads are in flight together
+#pragma unroll 8
   for (uint32_t k = threadIdx.x; k < nparts; k += 1024) {
     v += parts.vis[k];
     p += parts.pairs[k];
@@ -391,9 +393,10 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     DL_HIP(hipGetLastError());
     mark(DL_STAGE_DEPTH_SORT);
     // A speculative frame sorts on key - (least visible key) over the span
-    // bits of the last frame, with up to 9-bit digits, when that takes fewer
-    // passes; invisible splats map past every visible key, so the order is
-    // the 32-bit sort's whenever the span fits (checked after the frame).
+    // bits of the last frame (plus key_headroom), with up to 9-bit digits,
+    // when that takes fewer passes; invisible splats map past every visible
+    // key, so the order is the 32-bit sort's whenever the span fits (checked
+    // after the frame).
     int sort_bits = 32, sort_digits = digit_bits();
     RadixKeyMap km{};
     if (speculative && spec_key_bits > 0 && spec_key_bits < 32) {
@@ -620,7 +623,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         uint32_t need = 0;
         while (need < 33 && (span >> need) != 0) need++;
         keys_fit = key_bits_used == 0 || need <= key_bits_used;
-        spec_key_bits = std::min<uint32_t>(32, need + 1);
+        if (!keys_fit) key_headroom = std::min<uint32_t>(2, key_headroom + 1);
+        spec_key_bits = std::min<uint32_t>(32, need + key_headroom);
       }
       if (speculative) {
         stats.num_visible = sm[0];
