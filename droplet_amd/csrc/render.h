@@ -51,6 +51,9 @@ constexpr uint64_t kPassMinPairs = 64ull << 20;
 constexpr uint64_t kPassMinFirst = 65536;
 constexpr uint64_t kPassFirstDiv = 8;
 constexpr uint64_t kPassGrowth = 8;
+// tiles from which the first pass launches its tiles in last frame's work
+// order (the ordering sort costs about four launches)
+constexpr uint32_t kOrderMinTiles = 4096;
 
 struct RastParams {
   uint32_t tiles_x, width, height;
@@ -60,6 +63,10 @@ struct RastParams {
   uint32_t final_on_close;
   float bg_r, bg_g, bg_b, bg_a;
   uint8_t *closed;  // per (virtual) tile: 1 once every pixel has T < 1e-4 (null: not tracked)
+  // launch order of the tiles (null: tile = workgroup index), and where each
+  // tile records the sort key of its work (null: not recorded)
+  const uint32_t *order;
+  uint32_t *cost_key;
 };
 
 struct Scene {
@@ -121,6 +128,14 @@ struct Renderer {
   // outgrew the bits (up to 2), so a growing range stops re-rendering
   uint32_t key_headroom = 0;
   bool force_exact = false;
+  // Launch order of the first pass's tiles in large frames: decreasing work
+  // of the same tiles in the last frame, in bands of about 1.4x (a 6-bit key
+  // the rasterizer writes), and tile order inside a band, which keeps the
+  // concurrently running tiles close together for L2. The heaviest tiles
+  // then start first and the launch does not end on a few late heavy ones.
+  // The order changes no result: the tiles are independent.
+  DeviceBuffer tile_key, tile_key_tmp, tile_order, tile_order_tmp, tile_sort_ws;
+  uint32_t tile_key_tiles = 0;  // tile count of the recorded keys (0: none)
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)
   std::vector<hipEvent_t> events;

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                                                   uint32_t *__restrict__ out_rgba, RastParams p,
                                                   unsigned long long *__restrict__ d_iters) {
   __shared__ RastLds lds;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = p.order ? p.order[blockIdx.x] : blockIdx.x;
   const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
   rasterize_tile<COUNT>(tile % p.tiles_x, tile / p.tiles_x, tile, start, end, rec, pvals, accum, out_rgba, p, d_iters,
                         lds);
@@ -589,8 +589,25 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       rp.has_pairs = 1;
       train_vals = svals;
       mark(DL_STAGE_RASTERIZE);
+      RastParams rpc = rp;
+      if (c == 0 && num_tiles >= kOrderMinTiles) {
+        tile_key.reserve((size_t)num_tiles * 4);
+        if (tile_key_tiles == num_tiles) {
+          tile_key_tmp.reserve((size_t)num_tiles * 4);
+          tile_order.reserve((size_t)num_tiles * 4);
+          tile_order_tmp.reserve((size_t)num_tiles * 4);
+          tile_sort_ws.reserve(radix_workspace_bytes(num_tiles, 6));
+          // one stable 6-bit pass; the values are the tile indices
+          const bool in_tmp = radix_sort_pairs(s, tile_key.as<uint32_t>(), tile_order.as<uint32_t>(),
+                                               tile_key_tmp.as<uint32_t>(), tile_order_tmp.as<uint32_t>(), num_tiles,
+                                               6, tile_sort_ws.get(), 6, RadixKeyMap{}, true);
+          rpc.order = in_tmp ? tile_order_tmp.as<uint32_t>() : tile_order.as<uint32_t>();
+        }
+        rpc.cost_key = tile_key.as<uint32_t>();
+        tile_key_tiles = num_tiles;
+      }
       launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
-                       rgba.as<uint32_t>(), rp, d_iters);
+                       rgba.as<uint32_t>(), rpc, d_iters);
       accum_valid = true;
       if (track_closed && !last) {
         k_open_prefix<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tiles_x, open_pre.as<uint16_t>());

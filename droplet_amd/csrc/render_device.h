@@ -380,6 +380,15 @@ __device__ inline bool rect_may_hit(double sx, double sy, double r, double A, do
   return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
 }
 
+// Ascending 6-bit sort key of a tile's work: 63 for none, then two bands per
+// octave, heavier work first.
+__device__ inline uint32_t work_key(uint32_t w) {
+  if (w == 0) return 63u;
+  const uint32_t lz = 31u - (uint32_t)__clz(w);
+  const uint32_t half = lz >= 1 ? (w >> (lz - 1)) & 1u : 0u;
+  return 62u - min(62u, 2u * lz + half);
+}
+
 // LDS of one rasterizer wave: a batch of kStage staged splat records and the
 // lanes' 16 accumulators while the batch is staged.
 constexpr uint32_t kStage = 32;
@@ -404,11 +413,12 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
                                       float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
                                       const RastParams &p, unsigned long long *__restrict__ d_iters,
                                       RastLds &L) {
-  if (p.closed && p.resume) {
-    if (p.closed[vtile] && (p.final_on_close || !p.finalize)) return;
-    if (!p.finalize && start == end) return;
+  if (p.closed && p.resume && ((p.closed[vtile] && (p.final_on_close || !p.finalize)) || (!p.finalize && start == end))) {
+    if (p.cost_key && threadIdx.x == 0) p.cost_key[vtile] = work_key(0);
+    return;
   }
   uint32_t iters = 0;
+  uint32_t work = 0;  // staged batches x 8 + visited (splat, strip) bits
   const int lane = threadIdx.x;
   // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
   // this lane's row in strip k is 4k + lane/16
@@ -483,6 +493,8 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
       const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
       const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
       const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
+      work += 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1) + __builtin_popcount(s2) +
+                              __builtin_popcount(s3));
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -548,6 +560,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
     if (write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
     if (finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
   }
+  if (p.cost_key && lane == 0) p.cost_key[vtile] = work_key(work);
   if (COUNT) {
     uint32_t s = iters;
 #pragma unroll
