@@ -134,8 +134,14 @@ struct Renderer {
   // concurrently running tiles close together for L2. The heaviest tiles
   // then start first and the launch does not end on a few late heavy ones.
   // The order changes no result: the tiles are independent.
+  // The sort that turns one frame's keys into the next frame's order runs on
+  // a second stream (default priority) while pass 2 bins, and the next
+  // frame's first pass waits for it (order_done).
   DeviceBuffer tile_key, tile_key_tmp, tile_order, tile_order_tmp, tile_sort_ws;
   uint32_t tile_key_tiles = 0;  // tile count of the recorded keys (0: none)
+  hipStream_t order_stream = nullptr;
+  hipEvent_t keys_done = nullptr, order_done = nullptr;
+  const uint32_t *next_order = nullptr;  // set once an ordering sort was launched
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)
   std::vector<hipEvent_t> events;

@@ -590,24 +590,40 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       train_vals = svals;
       mark(DL_STAGE_RASTERIZE);
       RastParams rpc = rp;
-      if (c == 0 && num_tiles >= kOrderMinTiles) {
-        tile_key.reserve((size_t)num_tiles * 4);
-        if (tile_key_tiles == num_tiles) {
-          tile_key_tmp.reserve((size_t)num_tiles * 4);
-          tile_order.reserve((size_t)num_tiles * 4);
-          tile_order_tmp.reserve((size_t)num_tiles * 4);
-          tile_sort_ws.reserve(radix_workspace_bytes(num_tiles, 6));
-          // one stable 6-bit pass; the values are the tile indices
-          const bool in_tmp = radix_sort_pairs(s, tile_key.as<uint32_t>(), tile_order.as<uint32_t>(),
-                                               tile_key_tmp.as<uint32_t>(), tile_order_tmp.as<uint32_t>(), num_tiles,
-                                               6, tile_sort_ws.get(), 6, RadixKeyMap{}, true);
-          rpc.order = in_tmp ? tile_order_tmp.as<uint32_t>() : tile_order.as<uint32_t>();
+      const bool ordered = c == 0 && num_tiles >= kOrderMinTiles;
+      if (ordered) {
+        if (next_order && tile_key_tiles == num_tiles) {
+          // the order sorted from the last frame's keys; the wait also keeps
+          // this launch from overwriting the keys before that sort read them
+          DL_HIP(hipStreamWaitEvent(s, order_done, 0));
+          rpc.order = next_order;
         }
+        tile_key.reserve((size_t)num_tiles * 4);
         rpc.cost_key = tile_key.as<uint32_t>();
-        tile_key_tiles = num_tiles;
       }
       launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
                        rgba.as<uint32_t>(), rpc, d_iters);
+      if (ordered) {
+        // sort this frame's keys into the next frame's order on the second
+        // stream: one stable 6-bit pass whose values are the tile indices
+        if (!order_stream) {
+          DL_HIP(hipStreamCreateWithFlags(&order_stream, hipStreamNonBlocking));
+          DL_HIP(hipEventCreateWithFlags(&keys_done, hipEventDisableTiming));
+          DL_HIP(hipEventCreateWithFlags(&order_done, hipEventDisableTiming));
+        }
+        tile_key_tmp.reserve((size_t)num_tiles * 4);
+        tile_order.reserve((size_t)num_tiles * 4);
+        tile_order_tmp.reserve((size_t)num_tiles * 4);
+        tile_sort_ws.reserve(radix_workspace_bytes(num_tiles, 6));
+        DL_HIP(hipEventRecord(keys_done, s));
+        DL_HIP(hipStreamWaitEvent(order_stream, keys_done, 0));
+        const bool in_tmp = radix_sort_pairs(order_stream, tile_key.as<uint32_t>(), tile_order.as<uint32_t>(),
+                                             tile_key_tmp.as<uint32_t>(), tile_order_tmp.as<uint32_t>(), num_tiles, 6,
+                                             tile_sort_ws.get(), 6, RadixKeyMap{}, true);
+        DL_HIP(hipEventRecord(order_done, order_stream));
+        next_order = in_tmp ? tile_order_tmp.as<uint32_t>() : tile_order.as<uint32_t>();
+        tile_key_tiles = num_tiles;
+      }
       accum_valid = true;
       if (track_closed && !last) {
         k_open_prefix<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tiles_x, open_pre.as<uint16_t>());
@@ -683,6 +699,12 @@ void Renderer::read_image(uint8_t *out) {
 
 Renderer::~Renderer() {
   for (auto e : events) (void)hipEventDestroy(e);
+  if (order_stream) {
+    (void)hipStreamSynchronize(order_stream);
+    (void)hipStreamDestroy(order_stream);
+    (void)hipEventDestroy(keys_done);
+    (void)hipEventDestroy(order_done);
+  }
 }
 
 }  // namespace dl
