@@ -326,12 +326,10 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ 
 // row_start[r] = spans before virtual row r (the (row r, block 0) entry of
 // the level-1 scan, or the running total for a view without blocks),
 // row_start[rows] = all spans; clamped to the span capacity cap
-__global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, size_t n1, L1View one,
-                                                   const L1View *__restrict__ tab, uint32_t ntab, uint32_t rows,
-                                                   const uint32_t *__restrict__ d_items,
-                                                   const uint32_t *__restrict__ d_num_spans, uint32_t cap,
-                                                   uint32_t *__restrict__ row_start) {
-  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void row_start_one(uint32_t r, const uint32_t *__restrict__ base1, size_t n1, const L1View &one,
+                                     const L1View *__restrict__ tab, uint32_t ntab, uint32_t rows,
+                                     const uint32_t *__restrict__ d_items, const uint32_t *__restrict__ d_num_spans,
+                                     uint32_t cap, uint32_t *__restrict__ row_start) {
   const uint32_t total = min(*d_num_spans, cap);
   if (r == rows) row_start[r] = total;
   if (r >= rows) return;
@@ -350,6 +348,15 @@ __global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ 
   const size_t n1e = d_items ? (size_t)v.rows * nb1 : n1;
   const size_t idx = (size_t)v.c1_base + (size_t)(r - v.row0) * nb1;
   row_start[r] = idx < n1e ? min(base1[idx], cap) : total;
+}
+
+__global__ __launch_bounds__(256) void k_row_start(const uint32_t *__restrict__ base1, size_t n1, L1View one,
+                                                   const L1View *__restrict__ tab, uint32_t ntab, uint32_t rows,
+                                                   const uint32_t *__restrict__ d_items,
+                                                   const uint32_t *__restrict__ d_num_spans, uint32_t cap,
+                                                   uint32_t *__restrict__ row_start) {
+  row_start_one(blockIdx.x * 256 + threadIdx.x, base1, n1, one, tab, ntab, rows, d_items, d_num_spans, cap,
+                row_start);
 }
 
 // ----------------------- MBS chunk tables ---------------------------------
@@ -528,14 +535,11 @@ __global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ ite
 // (segment, bucket, chunk 0) entry; a segment without chunks takes the entry
 // where it would start, or the total at the end. Level 2a: o = row*groups +
 // group. Level 2b: o = tile = row*tiles_x + x -> (row*groups + x/16, x%16).
-__global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restrict__ base,
-                                                        const uint32_t *__restrict__ chunk_base, uint32_t nseg,
-                                                        uint32_t chunk_cap, uint32_t n_out, uint32_t groups,
-                                                        uint32_t tiles_x, int tile_mode,
-                                                        const uint32_t *__restrict__ d_total, uint32_t out_cap,
-                                                        uint32_t *__restrict__ out,
-                                                        unsigned long long *__restrict__ acc) {
-  const uint32_t o = blockIdx.x * 256 + threadIdx.x;
+__device__ inline void bucket_offset_one(uint32_t o, const uint32_t *__restrict__ base,
+                                         const uint32_t *__restrict__ chunk_base, uint32_t nseg, uint32_t chunk_cap,
+                                         uint32_t n_out, uint32_t groups, uint32_t tiles_x, int tile_mode,
+                                         const uint32_t *__restrict__ d_total, uint32_t out_cap,
+                                         uint32_t *__restrict__ out, unsigned long long *__restrict__ acc) {
   if (o > n_out) return;
   const uint32_t total = min(*d_total, out_cap);
   if (o == n_out) {
@@ -558,14 +562,24 @@ __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restri
   out[o] = pos < ncounts ? min(base[pos], total) : total;
 }
 
+__global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restrict__ base,
+                                                        const uint32_t *__restrict__ chunk_base, uint32_t nseg,
+                                                        uint32_t chunk_cap, uint32_t n_out, uint32_t groups,
+                                                        uint32_t tiles_x, int tile_mode,
+                                                        const uint32_t *__restrict__ d_total, uint32_t out_cap,
+                                                        uint32_t *__restrict__ out,
+                                                        unsigned long long *__restrict__ acc) {
+  bucket_offset_one(blockIdx.x * 256 + threadIdx.x, base, chunk_base, nseg, chunk_cap, n_out, groups, tiles_x,
+                    tile_mode, d_total, out_cap, out, acc);
+}
+
 // The whole chunk table of a level with at most kSmallSegs segments in one
 // 1024-thread workgroup: each thread owns 4 consecutive segments.
 constexpr uint32_t kSmallSegs = 4096;
 
-__global__ __launch_bounds__(1024) void k_chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg,
-                                                            uint32_t cap, uint32_t *__restrict__ chunk_base,
-                                                            uint32_t *__restrict__ seg_of) {
-  __shared__ uint32_t s_wave[16];
+__device__ inline void chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg, uint32_t cap,
+                                         uint32_t *__restrict__ chunk_base, uint32_t *__restrict__ seg_of,
+                                         uint32_t *s_wave) {
   const uint32_t t = threadIdx.x, s0 = 4 * t;
   uint32_t nch[4], sum = 0;
 #pragma unroll
@@ -600,6 +614,46 @@ __global__ __launch_bounds__(1024) void k_chunk_table_small(const uint32_t *__re
     run += nch[k];
   }
   if (t == 0) chunk_base[nseg] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg,
+                                                            uint32_t cap, uint32_t *__restrict__ chunk_base,
+                                                            uint32_t *__restrict__ seg_of) {
+  __shared__ uint32_t s_wave[16];
+  chunk_table_small(seg_start, nseg, cap, chunk_base, seg_of, s_wave);
+}
+
+// Level 1's row starts, then level 2a's chunk table over them, in one
+// workgroup (rows <= kSmallSegs): the barrier makes the row starts visible.
+__global__ __launch_bounds__(1024) void k_row_start_chunks(const uint32_t *__restrict__ base1, size_t n1, L1View one,
+                                                          const L1View *__restrict__ tab, uint32_t ntab,
+                                                          uint32_t rows, const uint32_t *__restrict__ d_items,
+                                                          const uint32_t *__restrict__ d_num_spans, uint32_t cap,
+                                                          uint32_t *__restrict__ row_start, uint32_t max_chunks,
+                                                          uint32_t *__restrict__ chunk_base,
+                                                          uint32_t *__restrict__ seg_of) {
+  __shared__ uint32_t s_wave[16];
+  for (uint32_t r = threadIdx.x; r <= rows; r += 1024)
+    row_start_one(r, base1, n1, one, tab, ntab, rows, d_items, d_num_spans, cap, row_start);
+  __syncthreads();
+  chunk_table_small(row_start, rows, max_chunks, chunk_base, seg_of, s_wave);
+}
+
+// Level 2a's group offsets (the segment starts of level 2b), then level 2b's
+// chunk table over them, in one workgroup (segments <= kSmallSegs).
+__global__ __launch_bounds__(1024) void k_offsets_chunks(const uint32_t *__restrict__ base,
+                                                        const uint32_t *__restrict__ chunk_base_a, uint32_t rows,
+                                                        uint32_t max_ch_a, uint32_t nseg_b, uint32_t groups,
+                                                        uint32_t tiles_x, const uint32_t *__restrict__ d_total,
+                                                        uint32_t cap, uint32_t *__restrict__ seg_start_b,
+                                                        uint32_t max_ch_b, uint32_t *__restrict__ chunk_base_b,
+                                                        uint32_t *__restrict__ seg_of_b) {
+  __shared__ uint32_t s_wave[16];
+  for (uint32_t o = threadIdx.x; o <= nseg_b; o += 1024)
+    bucket_offset_one(o, base, chunk_base_a, rows, max_ch_a, nseg_b, groups, tiles_x, 0, d_total, cap, seg_start_b,
+                      nullptr);
+  __syncthreads();
+  chunk_table_small(seg_start_b, nseg_b, max_ch_b, chunk_base_b, seg_of_b, s_wave);
 }
 
 // Chunk table of one MBS level, built on the device: chunks per segment,
@@ -669,15 +723,23 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
     DL_HIP(hipGetLastError());
   }
   ws.row_start.reserve(((size_t)a.rows + 1) * 4);
-  k_row_start<<<div_up((size_t)a.rows + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, a.one,
-                                                              a.tab, a.ntab, a.rows, a.d_items, d_num_spans, cap,
-                                                              ws.row_start.as<uint32_t>());
-  DL_HIP(hipGetLastError());
-
   // level 2a: row spans -> 16-column groups (segments = rows)
   const uint32_t max_ch_a = div_up(cap, kChunk) + a.rows;
-  build_chunk_table(s, ws.row_start.as<uint32_t>(), a.rows, ws.tmp, ws.chunk_base_a, ws.seg_of_a, ws.scan_ws,
-                    max_ch_a);
+  if (a.rows <= kSmallSegs) {
+    ws.chunk_base_a.reserve(((size_t)a.rows + 1) * 4);
+    ws.seg_of_a.reserve((size_t)std::max<uint32_t>(max_ch_a, 1) * 4);
+    k_row_start_chunks<<<1, 1024, 0, s>>>(ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0, a.one, a.tab, a.ntab,
+                                          a.rows, a.d_items, d_num_spans, cap, ws.row_start.as<uint32_t>(), max_ch_a,
+                                          ws.chunk_base_a.as<uint32_t>(), ws.seg_of_a.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+  } else {
+    k_row_start<<<div_up((size_t)a.rows + 1, 256), 256, 0, s>>>(ws.base1.as<uint32_t>(), a.l1_blocks ? a.n1 : 0,
+                                                                a.one, a.tab, a.ntab, a.rows, a.d_items, d_num_spans,
+                                                                cap, ws.row_start.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+    build_chunk_table(s, ws.row_start.as<uint32_t>(), a.rows, ws.tmp, ws.chunk_base_a, ws.seg_of_a, ws.scan_ws,
+                      max_ch_a);
+  }
   MbsGeom ga{ws.seg_of_a.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), ws.row_start.as<uint32_t>(), groups,
              a.rows, max_ch_a, cap, a.live};
   const size_t nc_a = (size_t)kK * max_ch_a;
@@ -694,15 +756,23 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   DL_HIP(hipGetLastError());
   const uint32_t nseg_b = a.rows * groups;
   ws.seg_start_b.reserve(((size_t)nseg_b + 1) * 4);
-  k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(
-      ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), a.rows, max_ch_a, nseg_b, groups, a.cols, 0,
-      d_num_gspans, cap, ws.seg_start_b.as<uint32_t>(), nullptr);
-  DL_HIP(hipGetLastError());
-
   // level 2b: group spans -> columns (segments = (row, group))
   const uint32_t max_ch_b = div_up(cap, kChunk) + nseg_b;
-  build_chunk_table(s, ws.seg_start_b.as<uint32_t>(), nseg_b, ws.tmp, ws.chunk_base_b, ws.seg_of_b, ws.scan_ws,
-                    max_ch_b);
+  if (nseg_b <= kSmallSegs) {
+    ws.chunk_base_b.reserve(((size_t)nseg_b + 1) * 4);
+    ws.seg_of_b.reserve((size_t)std::max<uint32_t>(max_ch_b, 1) * 4);
+    k_offsets_chunks<<<1, 1024, 0, s>>>(ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), a.rows, max_ch_a,
+                                        nseg_b, groups, a.cols, d_num_gspans, cap, ws.seg_start_b.as<uint32_t>(),
+                                        max_ch_b, ws.chunk_base_b.as<uint32_t>(), ws.seg_of_b.as<uint32_t>());
+    DL_HIP(hipGetLastError());
+  } else {
+    k_bucket_offsets<<<div_up((size_t)nseg_b + 1, 256), 256, 0, s>>>(
+        ws.base2.as<uint32_t>(), ws.chunk_base_a.as<uint32_t>(), a.rows, max_ch_a, nseg_b, groups, a.cols, 0,
+        d_num_gspans, cap, ws.seg_start_b.as<uint32_t>(), nullptr);
+    DL_HIP(hipGetLastError());
+    build_chunk_table(s, ws.seg_start_b.as<uint32_t>(), nseg_b, ws.tmp, ws.chunk_base_b, ws.seg_of_b, ws.scan_ws,
+                      max_ch_b);
+  }
   MbsGeom gb{ws.seg_of_b.as<uint32_t>(), ws.chunk_base_b.as<uint32_t>(), ws.seg_start_b.as<uint32_t>(), groups,
              nseg_b, max_ch_b, cap, a.live};
   const size_t nc_b = (size_t)kK * max_ch_b;
