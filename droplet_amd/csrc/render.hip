@@ -488,8 +488,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     if (track_closed) {
       closed.reserve(num_tiles);
       open_pre.reserve((size_t)tiles_y * (tiles_x + 1) * 2);
-      k_fill_u8<<<div_up(num_tiles, 256), 256, 0, s>>>(closed.as<uint8_t>(), num_tiles, 0);
-      DL_HIP(hipGetLastError());
+      // no clear: the first pass is not the last, reads no flag and writes
+      // every tile's
     }
     const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(num_tiles - 1));
     stats.pair_key_bits = key_bits;

@@ -437,14 +437,16 @@ void scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32
   DL_HIP(hipGetLastError());
 }
 
-// Fewest passes of at most max_bits, with widths as even as possible (the
-// wider digits first): 25 bits at 9 are 9 + 8 + 8, not 9 + 9 + 7, since a
-// 9-bit pass costs more than an 8-bit one.
+// Fewest passes of at most max_bits, with widths as even as possible: 25
+// bits at 9 are 8 + 8 + 9, not 9 + 9 + 7, since a 9-bit pass costs more than
+// an 8-bit one. The wider digits are the high ones: high digits of depth keys
+// take few values, so a wide high pass scatters into long runs, while a wide
+// low pass splits every tile into 512 short runs of partly written lines.
 void pass_layout(int key_bits, int max_bits, int *passes, int bits[32]) {
   max_bits = std::min(kMaxDigitBits, std::max(1, max_bits));
   const int p = std::max(1, (key_bits + max_bits - 1) / max_bits);
   const int base = key_bits / p, wide = key_bits % p;
-  for (int i = 0; i < p; i++) bits[i] = base + (i < wide ? 1 : 0);
+  for (int i = 0; i < p; i++) bits[i] = base + (i >= p - wide ? 1 : 0);
   *passes = p;
 }
 
