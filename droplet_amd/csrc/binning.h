@@ -66,6 +66,11 @@ struct BinGrid {
   uint32_t tiles_x, tiles_y, max_cov;
 };
 
+// Level 1 reads each splat as one 16-byte record: id, then the rect.
+__device__ inline uint4 pack_rect(uint32_t id, const Rect &R) {
+  return make_uint4(id, R.x0 | (R.x1 << 16), R.y0 | (R.full_rows << 16), R.rem);
+}
+
 // Per-row prefix counts of the tiles that are still open: pre[y * stride +
 // x] = open tiles of (virtual) row y in columns < x; stride = columns + 1.
 // A null map means every tile is open.
@@ -143,6 +148,7 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
 // rect enter level 1 (a stable compaction); the lists are the same.
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
-              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries = nullptr);
+              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries = nullptr,
+              bool rects_ready = false);
 
 }  // namespace dl

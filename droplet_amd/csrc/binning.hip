@@ -38,9 +38,6 @@ constexpr int kK = 16;                   // buckets per MBS segment
 // Depth-ordered splat records of one chunk: (id, x0 | x1 << 16, y0 |
 // full_rows << 16, rem). One gather of the id-ordered projection records, so
 // that both level-1 passes read coalesced 16-byte records.
-__device__ inline uint4 pack_rect(uint32_t id, const Rect &R) {
-  return make_uint4(id, R.x0 | (R.x1 << 16), R.y0 | (R.full_rows << 16), R.rem);
-}
 
 __device__ inline Rect unpack_rect(const uint4 p) {
   Rect R;
@@ -801,7 +798,8 @@ void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1
 
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
               const LiveMap &live, const uint2 *rect8, uint32_t n_all, BinWorkspace &ws, uint32_t *d_off,
-              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries) {
+              uint32_t *pvals, uint32_t *d_small, unsigned long long *d_entries, bool rects_ready) {
+  // with rects_ready the caller has written ws.rects (reserved at this size)
   ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
   const bool compact = live.pre && rect8 && cn > 0;
   const uint32_t nblk = div_up(cn, 256);
@@ -828,7 +826,7 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     k_rects_live<<<grid, 256, 0, s>>>(rect8, ids, cn, ws.live_bits.as<unsigned long long>(),
                                       ws.live_off.as<uint32_t>(), ls, ws.rects.as<uint4>());
     DL_HIP(hipGetLastError());
-  } else if (cn) {
+  } else if (cn && !rects_ready) {
     k_rects<<<nblk, 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());
     DL_HIP(hipGetLastError());
   }

@@ -164,6 +164,14 @@ namespace {
 // eight passes, never won on MI355X) and pass schedules (full lists; first
 // pass n/32 .. n/4 with growth 4, 8 or 16; the default rule), n = the
 // largest scene.
+bool same_schedule(const dl_renderer_opts &a, const dl_renderer_opts &b) {
+  // digit width 0 means 8
+  const uint32_t da = a.sort_digit_bits ? a.sort_digit_bits : 8, db = b.sort_digit_bits ? b.sort_digit_bits : 8;
+  return da == db && a.first_pass == b.first_pass && a.pass_growth == b.pass_growth &&
+         (a.full_lists != 0) == (b.full_lists != 0);
+}
+
+// Candidates other than base itself, which the tuner times first.
 std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint64_t n) {
   std::vector<dl_renderer_opts> out;
   const uint32_t digits[] = {8, 6};
@@ -179,10 +187,15 @@ std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint
       o.full_lists = sc.full;
       o.first_pass = sc.div ? (uint32_t)std::max<uint64_t>(65536, n / sc.div) : 0;
       o.pass_growth = sc.growth;
-      out.push_back(o);
+      if (!same_schedule(o, base)) out.push_back(o);
     }
   return out;
 }
+
+// Frames run before the first timing, so that the default schedule is not
+// timed while the GPU's clocks are still rising.
+constexpr int kTuneSettle = 8;
+constexpr int kTuneReps = 5;
 
 template <class F>
 float wall_ms_per_call(F &&f, int reps) {
@@ -426,13 +439,14 @@ int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera 
     R.keep_taps = R.timing = R.counting = false;
     const dl_background bg{0.0f, 0.0f, 0.0f, 1.0f};
     auto frame = [&] { R.render(scene->s.view(), *cam, bg); };
+    for (int k = 0; k < kTuneSettle; k++) frame();
     dl_tune_result res{};
-    res.default_ms = wall_ms_per_call(frame, 3);
+    res.default_ms = wall_ms_per_call(frame, kTuneReps);
     res.chosen = before;
     res.chosen_ms = res.default_ms;
     for (const auto &o : tune_candidates(before, scene->s.count)) {
       R.opts = o;
-      const float ms = wall_ms_per_call(frame, 3);
+      const float ms = wall_ms_per_call(frame, kTuneReps);
       res.candidates++;
       if (ms < res.chosen_ms) {
         res.chosen_ms = ms;
@@ -462,6 +476,7 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
     Batch &B = b->b;
     const dl_renderer_opts before = B.opts;
     auto call = [&] { B.render(sv, views, num_views); };
+    for (int k = 0; k < 2; k++) call();
     dl_tune_result res{};
     res.default_ms = wall_ms_per_call(call, 2);
     res.chosen = before;

@@ -231,6 +231,37 @@ CullProjFn cull_project_for(uint32_t bands, bool color) {
 
 using ColorFn = void (*)(SceneView, const uint32_t *, const uint4 *, uint32_t, const uint32_t *, float4 *);
 
+// The first pass's level-1 records and colours in one gather of each
+// splat's projection record: item j names splat ids[j]. Same arithmetic as
+// k_rects followed by k_splat_colors.
+template <int BANDS>
+__global__ __launch_bounds__(256) void k_rects_colors(SceneView sc, const uint32_t *__restrict__ ids, uint32_t cn,
+                                                      BinGrid g, float4 *__restrict__ rec,
+                                                      uint4 *__restrict__ rects) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= cn) return;
+  const uint32_t id = ids[j];
+  const float4 v0 = rec[3 * (size_t)id];
+  float *slot = reinterpret_cast<float *>(rec + 3 * (size_t)id + 2);
+  const float pcx = slot[0], pcy = slot[1], pcz = slot[2];
+  rects[j] = pack_rect(id, splat_rect(v0, g.tiles_x, g.tiles_y, g.max_cov));
+  float col[3];
+  splat_color<BANDS>(sc, id, pcx, pcy, pcz, col);
+  slot[0] = col[0];
+  slot[1] = col[1];
+  slot[2] = col[2];
+}
+
+void launch_rects_colors(hipStream_t s, const SceneView &sc, const uint32_t *ids, uint32_t cn, const BinGrid &g,
+                         float4 *rec, uint4 *rects) {
+  if (cn == 0) return;
+  using Fn = void (*)(SceneView, const uint32_t *, uint32_t, BinGrid, float4 *, uint4 *);
+  Fn f = sc.bands == 0 ? k_rects_colors<0> : sc.bands == 1 ? k_rects_colors<1>
+       : sc.bands == 2 ? k_rects_colors<2> : k_rects_colors<3>;
+  f<<<div_up(cn, 256), 256, 0, s>>>(sc, ids, cn, g, rec, rects);
+  DL_HIP(hipGetLastError());
+}
+
 void launch_colors(hipStream_t s, const SceneView &sc, const uint32_t *ids, const uint4 *rects, uint32_t n_cap,
                    const uint32_t *d_n, float4 *rec) {
   if (n_cap == 0) return;
@@ -541,18 +572,22 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       const uint32_t *svals = pvals.as<uint32_t>();
       if (span) {
         // stable two-level bucketing straight into per-tile lists (binning.hip)
-        mark(DL_STAGE_EMIT);
         const LiveMap live = (track_closed && c > 0) ? LiveMap{open_pre.as<uint16_t>(), tiles_x + 1} : LiveMap{};
-        span_bin(s, rec.as<float4>(), cids, cn, BinGrid{tiles_x, tiles_y, max_cov}, npairs, live,
-                 want_rect8 ? rect8.as<uint2>() : nullptr, n, bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(),
-                 d_bin, d_entries);
-        if (defer_color) {
+        const BinGrid grid{tiles_x, tiles_y, max_cov};
+        // first pass: its level-1 records and colours in one kernel
+        const bool fused = defer_color && c == 0;
+        if (fused) {
           mark(DL_STAGE_PROJECT);
-          if (c == 0)
-            launch_colors(s, sc, cids, nullptr, cn, nullptr, rec.as<float4>());
-          else
-            launch_colors(s, sc, nullptr, bin_ws.rects.as<uint4>(), cn, bin_ws.live_small.as<uint32_t>(),
-                          rec.as<float4>());
+          bin_ws.rects.reserve((size_t)std::max<uint32_t>(cn, 1) * 16);
+          launch_rects_colors(s, sc, cids, cn, grid, rec.as<float4>(), bin_ws.rects.as<uint4>());
+        }
+        mark(DL_STAGE_EMIT);
+        span_bin(s, rec.as<float4>(), cids, cn, grid, npairs, live, want_rect8 ? rect8.as<uint2>() : nullptr, n,
+                 bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin, d_entries, fused);
+        if (defer_color && !fused) {
+          mark(DL_STAGE_PROJECT);
+          launch_colors(s, sc, nullptr, bin_ws.rects.as<uint4>(), cn, bin_ws.live_small.as<uint32_t>(),
+                        rec.as<float4>());
         }
       } else {
         pkeys.reserve((size_t)npairs * 4);
