@@ -80,6 +80,11 @@ struct LiveMap {
   // summed-area table of the open tiles, same stride, rows + 1 rows (set by
   // the live-splat compaction)
   const uint32_t *sat = nullptr;
+  // the same table at the corners of 4x4-tile blocks: csat[cy * cstride +
+  // cx] = sat[min(4 cy, rows) * stride + min(4 cx, cols)], small enough to
+  // stay in a CU's L1
+  const uint32_t *csat = nullptr;
+  uint32_t cstride = 0;
 };
 
 // Level-1 geometry of one view. A batch stacks the tile rows of its views
@@ -107,7 +112,7 @@ struct BinWorkspace {
   DeviceBuffer scan_ws;
   // live-splat compaction of later passes: id-order live bits, per-block
   // counts and offsets, [0] live items and [1] their level-1 blocks
-  DeviceBuffer live_bits, live_cnt, live_off, live_small, live_sat;
+  DeviceBuffer live_bits, live_cnt, live_off, live_small, live_sat, live_csat;
 };
 
 bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y);

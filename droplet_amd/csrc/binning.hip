@@ -74,9 +74,10 @@ __device__ inline bool live_range(const LiveMap &L, uint32_t y, uint32_t xa, uin
 
 // Summed-area table of the open tiles from the row prefixes: sat[y * (cols
 // + 1) + x] = open tiles in rows < y and columns < x, for y <= rows, x <=
-// cols. One thread per column accumulates down the rows.
+// cols. One thread per column accumulates down the rows. Then csat, the
+// table at the corners of 4x4-tile blocks (LiveMap::csat).
 __global__ __launch_bounds__(256) void k_open_sat(const uint16_t *__restrict__ pre, uint32_t rows, uint32_t cols,
-                                                  uint32_t *__restrict__ sat) {
+                                                  uint32_t *__restrict__ sat, uint32_t *__restrict__ csat) {
   const uint32_t stride = cols + 1;
   for (uint32_t x = threadIdx.x; x <= cols; x += 256) {
     uint32_t acc = 0;
@@ -86,6 +87,12 @@ __global__ __launch_bounds__(256) void k_open_sat(const uint16_t *__restrict__ p
       acc += pre[(size_t)y * stride + x];
       sat[(size_t)(y + 1) * stride + x] = acc;
     }
+  }
+  __syncthreads();
+  const uint32_t cstride = (cols + 3) / 4 + 1, crows = (rows + 3) / 4 + 1;
+  for (uint32_t k = threadIdx.x; k < crows * cstride; k += 256) {
+    const uint32_t cy = k / cstride, cx = k % cstride;
+    csat[k] = sat[(size_t)min(4 * cy, rows) * stride + min(4 * cx, cols)];
   }
 }
 
@@ -111,14 +118,23 @@ __global__ __launch_bounds__(256) void k_live_bits(const uint2 *__restrict__ rec
   for (int k = 0; k < kLiveItems; k++) {
     const size_t i = base + (size_t)k * 256;
     const Rect R = unpack_rect8(r[k]);
+    // the 4x4-block table first: no open tile in the blocks that cover the
+    // rect (rem row included) means none in the rect
+    bool maybe = false;
+    if (R.nrows > 0) {
+      const uint32_t cs = live.cstride;
+      const uint32_t cy0 = R.y0 >> 2, cy1 = (R.y0 + R.nrows + 3u) >> 2, cx0 = R.x0 >> 2, cx1 = (R.x1 + 4u) >> 2;
+      maybe = live.csat[cy1 * cs + cx1] - live.csat[cy0 * cs + cx1] - live.csat[cy1 * cs + cx0] +
+                  live.csat[cy0 * cs + cx0] != 0u;
+    }
     bool lv = false;
-    if (R.full_rows > 0) {
+    if (maybe && R.full_rows > 0) {
       const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
       const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
                             live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
       lv = open != 0;
     }
-    if (!lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
+    if (maybe && !lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
     const unsigned long long b = __ballot(lv);
     const size_t first = i - (size_t)__lane_id();
     if (__lane_id() == 0 && first < n) bits[first >> 6] = b;
@@ -812,10 +828,15 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     ws.scan_ws.reserve(scan_workspace_bytes(nblk));
     uint32_t *ls = ws.live_small.as<uint32_t>();
     ws.live_sat.reserve(((size_t)g.tiles_y + 1) * (g.tiles_x + 1) * 4);
-    k_open_sat<<<1, 256, 0, s>>>(live.pre, g.tiles_y, g.tiles_x, ws.live_sat.as<uint32_t>());
+    const uint32_t cstride = div_up(g.tiles_x, 4) + 1, crows = div_up(g.tiles_y, 4) + 1;
+    ws.live_csat.reserve((size_t)crows * cstride * 4);
+    k_open_sat<<<1, 256, 0, s>>>(live.pre, g.tiles_y, g.tiles_x, ws.live_sat.as<uint32_t>(),
+                                 ws.live_csat.as<uint32_t>());
     DL_HIP(hipGetLastError());
     LiveMap lm = live;
     lm.sat = ws.live_sat.as<uint32_t>();
+    lm.csat = ws.live_csat.as<uint32_t>();
+    lm.cstride = cstride;
     k_live_bits<<<div_up(n_all, 256 * kLiveItems), 256, 0, s>>>(rect8, n_all, lm,
                                                                 ws.live_bits.as<unsigned long long>());
     DL_HIP(hipGetLastError());
