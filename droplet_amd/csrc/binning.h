@@ -112,7 +112,7 @@ struct BinWorkspace {
   DeviceBuffer scan_ws;
   // live-splat compaction of later passes: id-order live bits, per-block
   // counts and offsets, [0] live items and [1] their level-1 blocks
-  DeviceBuffer live_bits, live_cnt, live_off, live_small, live_sat, live_csat;
+  DeviceBuffer live_bits, live_cnt, live_small, live_sat, live_csat;
 };
 
 bool span_binning_supported(uint32_t tiles_x, uint32_t tiles_y);

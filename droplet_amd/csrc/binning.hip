@@ -145,10 +145,10 @@ __device__ inline bool live_bit(const unsigned long long *__restrict__ bits, uin
   return (bits[id >> 6] >> (id & 63u)) & 1ull;
 }
 
-// live splats per 256-item sub-block of the pass (nsub = ceil(cn / 256))
+// live splats per workgroup of kLiveItems 256-item sub-blocks of the pass
 __global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__ ids, uint32_t cn,
                                                     const unsigned long long *__restrict__ bits,
-                                                    uint32_t *__restrict__ cnt) {
+                                                    uint32_t *__restrict__ blk_tot) {
   __shared__ uint32_t s_w[kLiveItems][4];
   const uint32_t sub0 = blockIdx.x * kLiveItems;
   uint32_t id[kLiveItems];
@@ -167,21 +167,24 @@ __global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__
     if (__lane_id() == 0) s_w[k][threadIdx.x >> 6] = (uint32_t)__popcll(b);
   }
   __syncthreads();
-  const uint32_t nsub = (cn + 255) / 256;
-  if (threadIdx.x < kLiveItems && sub0 + threadIdx.x < nsub) {
-    const uint32_t k = threadIdx.x;
-    cnt[sub0 + k] = s_w[k][0] + s_w[k][1] + s_w[k][2] + s_w[k][3];
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) t += s_w[k][0] + s_w[k][1] + s_w[k][2] + s_w[k][3];
+    blk_tot[blockIdx.x] = t;
   }
 }
 
-// the pass's live splats as packed depth-ordered rects, compacted: sub-block
-// q's live items start at off[q]; also live_small[1] = level-1 blocks of
-// the live_small[0] live items
+// the pass's live splats as packed depth-ordered rects, compacted: each
+// workgroup sums the totals of the workgroups before it (k_live_count) for
+// its start; the last one writes live_small[0] = live items and
+// live_small[1] = their level-1 blocks
 __global__ __launch_bounds__(256) void k_rects_live(const uint2 *__restrict__ rect8, const uint32_t *__restrict__ ids,
                                                     uint32_t cn, const unsigned long long *__restrict__ bits,
-                                                    const uint32_t *__restrict__ off, uint32_t *__restrict__ live_small,
-                                                    uint4 *__restrict__ rects) {
+                                                    const uint32_t *__restrict__ blk_tot,
+                                                    uint32_t *__restrict__ live_small, uint4 *__restrict__ rects) {
   __shared__ uint32_t s_w[kLiveItems][4];
+  __shared__ uint32_t s_pre[4];
   const uint32_t sub0 = blockIdx.x * kLiveItems;
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   uint32_t id[kLiveItems];
@@ -202,17 +205,28 @@ __global__ __launch_bounds__(256) void k_rects_live(const uint2 *__restrict__ re
     r[k] = lv ? rect8[id[k]] : make_uint2(0u, 0u);
     if (lane == 0) s_w[k][wave] = (uint32_t)__popcll(bal[k]);
   }
+  // live items of the workgroups before this one
+  uint32_t pre = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) pre += blk_tot[b];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) pre += (uint32_t)__shfl_xor((int)pre, d, 64);
+  if (lane == 0) s_pre[wave] = pre;
   __syncthreads();
-  const uint32_t nsub = (cn + 255) / 256;
+  uint32_t start = s_pre[0] + s_pre[1] + s_pre[2] + s_pre[3];
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
   for (int k = 0; k < kLiveItems; k++) {
-    if (sub0 + k >= nsub || !((bal[k] >> lane) & 1ull)) continue;
-    uint32_t pos = off[sub0 + k] + (uint32_t)__popcll(bal[k] & below);
-    for (int q = 0; q < wave; q++) pos += s_w[k][q];
-    rects[pos] = pack_rect(id[k], unpack_rect8(r[k]));
+    if ((bal[k] >> lane) & 1ull) {
+      uint32_t pos = start + (uint32_t)__popcll(bal[k] & below);
+      for (int q = 0; q < wave; q++) pos += s_w[k][q];
+      rects[pos] = pack_rect(id[k], unpack_rect8(r[k]));
+    }
+    start += s_w[k][0] + s_w[k][1] + s_w[k][2] + s_w[k][3];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) live_small[1] = (live_small[0] + kB1 - 1) / kB1;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    live_small[0] = start;
+    live_small[1] = (start + kB1 - 1) / kB1;
+  }
 }
 
 // The level-1 view that owns level-1 block b: the single view `one`, or a
@@ -823,9 +837,7 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     // live splats only, in depth order
     ws.live_bits.reserve(((size_t)div_up(n_all, 64) + 1) * 8);
     ws.live_cnt.reserve((size_t)nblk * 4);
-    ws.live_off.reserve((size_t)nblk * 4);
     ws.live_small.reserve(256);
-    ws.scan_ws.reserve(scan_workspace_bytes(nblk));
     uint32_t *ls = ws.live_small.as<uint32_t>();
     ws.live_sat.reserve(((size_t)g.tiles_y + 1) * (g.tiles_x + 1) * 4);
     const uint32_t cstride = div_up(g.tiles_x, 4) + 1, crows = div_up(g.tiles_y, 4) + 1;
@@ -843,9 +855,8 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     const uint32_t grid = div_up(nblk, kLiveItems);
     k_live_count<<<grid, 256, 0, s>>>(ids, cn, ws.live_bits.as<unsigned long long>(), ws.live_cnt.as<uint32_t>());
     DL_HIP(hipGetLastError());
-    exclusive_scan_u32(s, ws.live_cnt.as<uint32_t>(), ws.live_off.as<uint32_t>(), nblk, ls, ws.scan_ws.get());
     k_rects_live<<<grid, 256, 0, s>>>(rect8, ids, cn, ws.live_bits.as<unsigned long long>(),
-                                      ws.live_off.as<uint32_t>(), ls, ws.rects.as<uint4>());
+                                      ws.live_cnt.as<uint32_t>(), ls, ws.rects.as<uint4>());
     DL_HIP(hipGetLastError());
   } else if (cn && !rects_ready) {
     k_rects<<<nblk, 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());
