@@ -32,7 +32,12 @@ namespace {
 
 constexpr int kB1 = 256;                 // splats per level-1 block (one per thread)
 constexpr int kW1 = kB1 / 32;            // mask words per row
-constexpr int kChunk = 1024;             // items per MBS chunk: 4 waves x 4 steps x 64 lanes
+constexpr int kChunk = 1024;             // items per MBS chunk: kMbsWaves waves x kMbsSteps steps x 64 lanes
+// 4 waves of 4 steps: 2 waves of 8 steps cost ~8 us per 6M-splat 4K frame,
+// 8 waves of 2 steps ~60 us
+constexpr int kMbsWaves = 4;
+constexpr int kMbsSteps = kChunk / (64 * kMbsWaves);
+constexpr int kMbsThreads = 64 * kMbsWaves;
 constexpr int kK = 16;                   // buckets per MBS segment
 
 // Depth-ordered splat records of one chunk: (id, x0 | x1 << 16, y0 |
@@ -467,9 +472,9 @@ __device__ inline ChunkSpan chunk_span(const MbsGeom &G, uint32_t c) {
 // Both MBS kernels run a fixed grid that strides over the chunks, whose
 // number is only known on the device.
 template <int LEVEL>
-__global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ items, MbsGeom G,
-                                                   uint32_t *__restrict__ counts) {
-  __shared__ uint32_t s_cnt[4][kK];
+__global__ __launch_bounds__(kMbsThreads) void k_mbs_count(const uint2 *__restrict__ items, MbsGeom G,
+                                                           uint32_t *__restrict__ counts) {
+  __shared__ uint32_t s_cnt[kMbsWaves][kK];
   const uint32_t nch = mbs_chunks(G);
   for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
   const ChunkSpan cs = chunk_span(G, c);
@@ -478,8 +483,8 @@ __global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ ite
 #pragma unroll
   for (int b = 0; b < kK; b++) cnt[b] = 0;
 #pragma unroll
-  for (int step = 0; step < 4; step++) {
-    const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
+  for (int step = 0; step < kMbsSteps; step++) {
+    const uint32_t i = cs.i0 + (uint32_t)w * (64u * kMbsSteps) + (uint32_t)step * 64u + (uint32_t)lane;
     const uint32_t bm = i < cs.iend ? item_buckets<LEVEL>(items[i], cs.seg, G) : 0u;
 #pragma unroll
     for (int b = 0; b < kK; b++) cnt[b] += (uint32_t)__popcll(__ballot((bm >> b) & 1u));
@@ -491,30 +496,34 @@ __global__ __launch_bounds__(256) void k_mbs_count(const uint2 *__restrict__ ite
   __syncthreads();
   if (threadIdx.x < kK) {
     const uint32_t b = threadIdx.x;
-    counts[(size_t)kK * cs.cb + (size_t)b * cs.nch + cs.k] = s_cnt[0][b] + s_cnt[1][b] + s_cnt[2][b] + s_cnt[3][b];
+    uint32_t t = 0;
+#pragma unroll
+    for (int v = 0; v < kMbsWaves; v++) t += s_cnt[v][b];
+    counts[(size_t)kK * cs.cb + (size_t)b * cs.nch + cs.k] = t;
   }
   __syncthreads();
   }
 }
 
 template <int LEVEL>
-__global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ items, MbsGeom G,
-                                                   const uint32_t *__restrict__ base, uint2 *__restrict__ out_items,
-                                                   uint32_t *__restrict__ out_ids) {
-  __shared__ uint32_t s_cnt[4][kK];
+__global__ __launch_bounds__(kMbsThreads) void k_mbs_write(const uint2 *__restrict__ items, MbsGeom G,
+                                                           const uint32_t *__restrict__ base,
+                                                           uint2 *__restrict__ out_items,
+                                                           uint32_t *__restrict__ out_ids) {
+  __shared__ uint32_t s_cnt[kMbsWaves][kK];
   const uint32_t nch = mbs_chunks(G);
   for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
   const ChunkSpan cs = chunk_span(G, c);
   const int lane = __lane_id(), w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  uint2 it[4];
-  uint32_t bm[4];
+  uint2 it[kMbsSteps];
+  uint32_t bm[kMbsSteps];
   uint32_t cnt[kK];
 #pragma unroll
   for (int b = 0; b < kK; b++) cnt[b] = 0;
 #pragma unroll
-  for (int step = 0; step < 4; step++) {
-    const uint32_t i = cs.i0 + (uint32_t)w * 256u + (uint32_t)step * 64u + (uint32_t)lane;
+  for (int step = 0; step < kMbsSteps; step++) {
+    const uint32_t i = cs.i0 + (uint32_t)w * (64u * kMbsSteps) + (uint32_t)step * 64u + (uint32_t)lane;
     bm[step] = 0;
     it[step] = make_uint2(0u, 0u);
     if (i < cs.iend) {
@@ -537,7 +546,7 @@ __global__ __launch_bounds__(256) void k_mbs_write(const uint2 *__restrict__ ite
     run[b] = r;
   }
 #pragma unroll
-  for (int step = 0; step < 4; step++) {
+  for (int step = 0; step < kMbsSteps; step++) {
 #pragma unroll
     for (int b = 0; b < kK; b++) {
       const bool in = (bm[step] >> b) & 1u;
@@ -777,11 +786,11 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   ws.base2.reserve(nc_a * 4);
   ws.gspans.reserve((size_t)cap * 8);
   ws.scan_ws.reserve(scan_workspace_bytes(nc_a));
-  k_mbs_count<0><<<mbs_grid(max_ch_a), 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.c2.as<uint32_t>());
+  k_mbs_count<0><<<mbs_grid(max_ch_a), kMbsThreads, 0, s>>>(ws.spans.as<uint2>(), ga, ws.c2.as<uint32_t>());
   DL_HIP(hipGetLastError());
   exclusive_scan_u32_dev(s, ws.c2.as<uint32_t>(), ws.base2.as<uint32_t>(), nc_a, ws.chunk_base_a.as<uint32_t>() + a.rows,
                          kK, d_num_gspans, ws.scan_ws.get());
-  k_mbs_write<0><<<mbs_grid(max_ch_a), 256, 0, s>>>(ws.spans.as<uint2>(), ga, ws.base2.as<uint32_t>(),
+  k_mbs_write<0><<<mbs_grid(max_ch_a), kMbsThreads, 0, s>>>(ws.spans.as<uint2>(), ga, ws.base2.as<uint32_t>(),
                                                     ws.gspans.as<uint2>(), nullptr);
   DL_HIP(hipGetLastError());
   const uint32_t nseg_b = a.rows * groups;
@@ -809,11 +818,11 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
   ws.c3.reserve(nc_b * 4);
   ws.base3.reserve(nc_b * 4);
   ws.scan_ws.reserve(scan_workspace_bytes(nc_b));
-  k_mbs_count<1><<<mbs_grid(max_ch_b), 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.c3.as<uint32_t>());
+  k_mbs_count<1><<<mbs_grid(max_ch_b), kMbsThreads, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.c3.as<uint32_t>());
   DL_HIP(hipGetLastError());
   exclusive_scan_u32_dev(s, ws.c3.as<uint32_t>(), ws.base3.as<uint32_t>(), nc_b, ws.chunk_base_b.as<uint32_t>() + nseg_b,
                          kK, d_npairs, ws.scan_ws.get());
-  k_mbs_write<1><<<mbs_grid(max_ch_b), 256, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.base3.as<uint32_t>(), nullptr,
+  k_mbs_write<1><<<mbs_grid(max_ch_b), kMbsThreads, 0, s>>>(ws.gspans.as<uint2>(), gb, ws.base3.as<uint32_t>(), nullptr,
                                                     pvals);
   DL_HIP(hipGetLastError());
   k_bucket_offsets<<<div_up((size_t)num_vtiles + 1, 256), 256, 0, s>>>(
