@@ -28,6 +28,7 @@ namespace dl {
 namespace {
 
 constexpr uint32_t kItemBlock = 256;
+static_assert(kItemBlock == (uint32_t)kB1, "level-1 blocks are counted in items of the level-1 kernels' block");
 constexpr uint64_t kMaxGroupPairs = (1ull << 31);
 
 // view owning block b (views ordered by first block); views with no blocks

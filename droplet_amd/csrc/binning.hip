@@ -30,9 +30,6 @@ namespace dl {
 
 namespace {
 
-// splats per level-1 block (one per thread); equal to batch.hip's kItemBlock.
-// 128 and 512 bin a 6M-splat 4K frame 13 us and 3 us slower.
-constexpr int kB1 = 256;
 constexpr int kW1 = kB1 / 32;            // mask words per row
 constexpr int kChunk = 1024;             // items per MBS chunk: kMbsWaves waves x kMbsSteps steps x 64 lanes
 // 4 waves of 4 steps: 2 waves of 8 steps cost ~8 us per 6M-splat 4K frame,
