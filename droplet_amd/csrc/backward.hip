@@ -1,7 +1,7 @@
 // Backward of the forward splat renderer on gfx950 (no reference counterpart:
 // droplet renders only; README.md:106). Differentiates the forward exactly
 // as render.hip evaluates it: the rasterizer loop of rasterize.ts:62-79 and
-// the EWA projection of project.ts:111-285, quirks included.
+// the EWA projection of project.ts:59-234, quirks included.
 //
 //   k_image_rgb          the differentiable image: clamp(C + T * bg, 0, 1)
 //                        per channel (finalize.ts:31-35 before quantisation)
@@ -277,7 +277,7 @@ __device__ inline void quat_rot_grad(const R q[4], const R v[3], const R go[3], 
   for (int k = 0; k < 3; k++) gq[1 + k] += 2 * q[0] * vxg[k] + 2 * (go[k] * qdv + v[k] * gdq) - 4 * gdv * qv[k];
 }
 
-// quat_rot (project.ts:101-104) for any real type
+// quat_rot (project.ts:50-53) for any real type
 template <class R>
 __device__ inline void quat_rot_r(const R q[4], const R v[3], R o[3]) {
   const R c0 = q[2] * v[2] - q[3] * v[1], c1 = q[3] * v[0] - q[1] * v[2], c2 = q[1] * v[1] - q[2] * v[0];
@@ -287,7 +287,7 @@ __device__ inline void quat_rot_r(const R q[4], const R v[3], R o[3]) {
   o[2] = (v[2] + q[0] * t2) + (q[1] * t1 - q[2] * t0);
 }
 
-// The chain rule of project.ts:111-285 per splat. The forward is f32, but
+// The chain rule of project.ts:59-234 per splat. The forward is f32, but
 // near-singular projected covariances make f32 cancellation (det_orig =
 // cov00 cov11 - cov01^2, the antialiasing factor) matter at the 1e-3 level.
 // So the derivative of the projection is taken in f64, from the same f32

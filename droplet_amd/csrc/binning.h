@@ -25,7 +25,7 @@ struct Rect {
 };
 
 // tile rectangle and maxCoverage truncation of one projected splat, with the
-// exact arithmetic of project.ts:275-283 / emit-pairs.ts:164-181
+// exact arithmetic of project.ts:223-233 / emit-pairs.ts:38-41
 __device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov) {
   Rect R{0, 0, 0, 0, 0, 0};
   const float sx = v0.x, sy = v0.y, radius = v0.z;

@@ -28,10 +28,10 @@ constexpr float kMinAlpha = (float)0.00392156862745098;
 constexpr float kGaussFloor = (float)1.1108996538e-2;  // Math.exp(-4.5).toExponential(10)
 constexpr float kFadeStart = (float)0.9481481481481482;
 constexpr float kFadeEnd = (float)1.8962962962962964;
-constexpr float kDetMin = (float)1.0e-6;               // project.ts:216
-constexpr float kEigMin = (float)0.1;                  // project.ts:232
-constexpr float kTwoPi = (float)(3.14159265 * 2.0);    // project.ts:311 (abstract const-expr)
-constexpr float kPi = (float)3.14159265;               // project.ts:312
+constexpr float kDetMin = (float)1.0e-6;               // project.ts:165
+constexpr float kEigMin = (float)0.1;                  // project.ts:181
+constexpr float kTwoPi = (float)(3.14159265 * 2.0);    // project.ts:260 (abstract const-expr)
+constexpr float kPi = (float)3.14159265;               // project.ts:261
 constexpr uint32_t kMaxCoverageDefault = 16384;        // pipeline.ts:41
 constexpr uint32_t kInvisibleKey = 0xFFFFFFFFu;        // sorts after every finite depth
 
@@ -87,7 +87,7 @@ __host__ __device__ inline float dl_expf_nonpos(float x) {
   return __builtin_ldexpf(p, k);
 }
 
-// sigmoid (project.ts:106, voxelize.ts:388)
+// sigmoid (project.ts:55, voxel/shaders/voxelize.ts:82)
 __host__ __device__ inline float dl_sigmoidf(float x) { return 1.0f / (1.0f + dl_expf(-x)); }
 
 // WGSL u32(f32): round toward zero, clamp to [0, 2^32-1]; NaN -> 0.

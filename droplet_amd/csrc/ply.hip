@@ -26,12 +26,12 @@ namespace dl {
 namespace {
 
 constexpr double kShC0 = 0.28209479177387814;  // ply.ts:23
-constexpr size_t kHeaderMax = 65536;           // ply.ts:44
+constexpr size_t kHeaderMax = 65536;           // ply.ts:45
 constexpr uint32_t kPlyBlock = 256;
 constexpr uint32_t kPlyLdsFloats = 16384;      // 64 KB of staged rows per workgroup
 
 // ---------------------------------------------------------------- header --
-// WHATWG UTF-8 decode (TextDecoder, ply.ts:42), with U+FFFD for each maximal
+// WHATWG UTF-8 decode (TextDecoder, ply.ts:43), with U+FFFD for each maximal
 // invalid subpart; an ASCII byte is never consumed by an invalid sequence.
 std::vector<uint32_t> utf8_decode(const uint8_t *b, size_t n) {
   std::vector<uint32_t> out;
@@ -258,7 +258,7 @@ struct PlyCols {
   uint32_t flip;
 };
 
-// Math.hypot of four f64 values as the JS engines compute it (ply.ts:215):
+// Math.hypot of four f64 values as the JS engines compute it (ply.ts:220):
 // +inf if any is infinite, NaN if any is NaN, 0 if all are 0, else the
 // max-scaled, Kahan-compensated sum of squares.
 __device__ inline double js_hypot4(double a, double b, double c, double d) {

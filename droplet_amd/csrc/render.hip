@@ -5,7 +5,7 @@
 //                   f64 near/far (or radius) cull and sortable f32 depth key
 //                   (src/render/camera.ts:56-99, src/spatial/radix-sort.ts:20-24)
 //                   and, for visible splats, the EWA projection, SH colour and
-//                   tile count (src/render/shaders/project.ts:73-383)
+//                   tile count (src/render/shaders/project.ts:22-332)
 //   depth sort      stable LSD radix over (key, splat id) (radix-sort.ts:65-117)
 //   binning         per-tile lists of splat ids in depth order: span_bin
 //                   (binning.hip), or k_emit + stable radix sort + ranges

@@ -13,7 +13,7 @@ namespace dl {
 namespace {
 
 // ---------------------------------------------------------------------------
-// project (project.ts:73-383)
+// project (project.ts:22-332)
 // ---------------------------------------------------------------------------
 __device__ inline void quat_rot(const float q[4], float v0, float v1, float v2, float o[3]) {
   const float c0 = q[2] * v2 - q[3] * v1;
@@ -77,14 +77,14 @@ __device__ inline void eval_sh(const float *__restrict__ sh, float x, float y, f
   out[2] = b;
 }
 
-// Camera-space position of a splat centre (project.ts:111-113).
+// Camera-space position of a splat centre (project.ts:80-82).
 __device__ inline void cam_pos(const float *m, float px, float py, float pz, float &pcx, float &pcy, float &pcz) {
   pcx = ((m[0] * px + m[4] * py) + m[8] * pz) + m[12] * 1.0f;
   pcy = ((m[1] * px + m[5] * py) + m[9] * pz) + m[13] * 1.0f;
   pcz = ((m[2] * px + m[6] * py) + m[10] * pz) + m[14] * 1.0f;
 }
 
-// Colour of a splat (project.ts:253-266): DC + view-dependent SH along the
+// Colour of a splat (project.ts:203-212): DC + view-dependent SH along the
 // camera-space direction, + 0.5, clamped at 0.
 template <int BANDS>
 __device__ inline void color_from(const float *dc, const float *shc, float pcx, float pcy, float pcz, float col[3]) {
@@ -110,8 +110,8 @@ __device__ inline void splat_color(const SceneView &sc, uint32_t id, float pcx, 
   color_from<BANDS>(r, r + 3, pcx, pcy, pcz, col);
 }
 
-// One splat through project.ts:111-285. Returns the tile count; a culled
-// splat gets slot 0 = (0, 0, -1, 0) and count 0 (project.ts:216-219, 247-250).
+// One splat through project.ts:59-234. Returns the tile count; a culled
+// splat gets slot 0 = (0, 0, -1, 0) and count 0 (project.ts:165-169, 196-200, 238-242, 253-257).
 // Without COLOR the colour lanes of slot 2 stay 0: k_splat_colors fills them
 // later for the splats that reach a tile list.
 template <int BANDS, bool COLOR = true>
@@ -219,7 +219,7 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
   if constexpr (COLOR) splat_color<BANDS>(sc, id, pcx, pcy, pcz, col);
   const float alpha = (dl_sigmoidf(logit) * fade) * aa;
   // Rasterizer skip threshold in the record's padding lane (0 in the
-  // reference, project.ts:272): for power < tau the reference's pixel alpha,
+  // reference, project.ts:221): for power < tau the reference's pixel alpha,
   // min(cap, alpha*max(0, exp(power) - floor)), is below 1/255. tau is
   // ln(floor + (1/255)/alpha) lowered by margins that cover exp's error and
   // the f32 roundings of that expression.
@@ -336,7 +336,7 @@ __device__ inline uint32_t pack_rgba(float cr, float cg, float cb, float T, cons
   return R | (G << 8) | (B << 16) | (A << 24);
 }
 
-// The Gaussian exponent of rasterize.ts:70, -0.5 (A dx^2 + 2B dx dy + C dy^2),
+// The Gaussian exponent of rasterize.ts:71, -0.5 (A dx^2 + 2B dx dy + C dy^2),
 // with the two contractions the specification fixes (WGSL lets a compiler
 // fuse a * b + c): fma(C dy, dy, fma(2B dx, dy, A dx dx)). The colour
 // accumulation is the other fused expression. Both the backward's forward

@@ -3,8 +3,8 @@
 // Reference (citations relative to the droplet repository):
 //   computeAabbExtents        src/spatial/aabb.ts:22-62            (f64)
 //   computeBlockAlignedBounds src/voxel/voxelize.ts:278-306        (f64)
-//   batch loop / BVH query    src/voxel/voxelize.ts:185-234, src/spatial/bvh.ts:198-239
-//   voxelizeShader            src/voxel/shaders/voxelize.ts:360-453
+//   batch loop / BVH query    src/voxel/voxelize.ts:185-234, src/spatial/bvh.ts:136-177
+//   voxelizeShader            src/voxel/shaders/voxelize.ts:17-148
 //   mask decode + addBlock    src/voxel/voxelize.ts:160-179, src/voxel/block-mask-buffer.ts:30-41
 //
 // Device design. The reference gives every 4x4x4 block of a 16^3-block batch
@@ -143,7 +143,7 @@ __device__ inline void axis_block_range(const VoxGrid &G, int k, float pf, float
   const double bs = G.bs, bmn = G.bmn[k];
   const int32_t nb = G.nb[k];
   // (a) batches whose f64 box the f64 AABB overlaps: the exact test of
-  // bvh.ts:222-230 against the batch box built as voxelize.ts:193-198
+  // bvh.ts:158-171 against the batch box built as voxelize.ts:193-198
   const double span = 16.0 * bs;
   int32_t kb_lo = (int32_t)floor((gmn - bmn) / span) - 1;
   int32_t kb_hi = (int32_t)floor((gmx - bmn) / span) + 1;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_vox_emit(const VGauss *__restrict__ rec
       }
 }
 
-// evalGaussian (voxel/shaders/voxelize.ts:368-390), reference order of operations
+// evalGaussian (voxel/shaders/voxelize.ts:62-84), reference order of operations
 __device__ inline float eval_gauss(float vx, float vy, float vz, float half, const VGauss &g) {
   const float d0 = vx - g.px, d1 = vy - g.py, d2 = vz - g.pz;
   if (__builtin_fabsf(d0) > g.ex + half || __builtin_fabsf(d1) > g.ey + half || __builtin_fabsf(d2) > g.ez + half)
@@ -247,7 +247,7 @@ __device__ inline float eval_gauss(float vx, float vy, float vz, float half, con
 }
 
 // One wave64 per block; lane = linear voxel index z*16 + y*4 + x, so the
-// solid ballot is the reference's 64-bit block mask (voxelize.ts:443-445).
+// solid ballot is the reference's 64-bit block mask (voxel/shaders/voxelize.ts:137-140).
 __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec, const uint32_t *__restrict__ off,
                                                  const uint32_t *__restrict__ vals, VoxGrid G,
                                                  uint2 *__restrict__ masks) {
@@ -268,8 +268,8 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
   for (int k = 0; k < 3; k++) {
     const uint32_t b0 = (a3[k] / 16u) * 16u;                        // batch origin (voxelize.ts:186-195)
     const float bmin = (float)(G.bmn[k] + (double)b0 * G.bs);       // BatchInfo.blockMin, f32
-    const float boff = ((float)(a3[k] - b0) * 4.0f) * G.res;        // voxelize.ts:410
-    vc[k] = (bmin + boff) + (((float)l3[k] + 0.5f) * G.res);       // voxelize.ts:411
+    const float boff = ((float)(a3[k] - b0) * 4.0f) * G.res;        // voxel/shaders/voxelize.ts:104
+    vc[k] = (bmin + boff) + (((float)l3[k] + 0.5f) * G.res);       // voxel/shaders/voxelize.ts:105
   }
   float sigma = 0.0f;
   for (uint32_t base = start; base < end; base += 64) {

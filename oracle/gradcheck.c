@@ -2,7 +2,7 @@
  * FP64 central-difference checker for the differentiable render.
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h). The device backward differentiates
- * the forward of the renderer (project.ts:111-285 then rasterize.ts:62-79,
+ * the forward of the renderer (project.ts:59-234 then rasterize.ts:62-79,
  * finalize.ts:31-35 before quantisation). This file evaluates the same
  * function in double precision from the same f32 inputs. It takes the f32
  * pipeline's tile lists, depth order and integer radii from orc_render: they

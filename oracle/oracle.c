@@ -29,10 +29,10 @@
 #define K_FLOOR ((float)1.1108996538e-2) /* Math.exp(-4.5).toExponential(10) */
 #define K_FADE_S ((float)0.9481481481481482)
 #define K_FADE_E ((float)1.8962962962962964)
-#define K_DET_MIN ((float)1.0e-6)           /* project.ts:216 */
-#define K_EIG_MIN ((float)0.1)              /* project.ts:232 */
-#define K_TWO_PI ((float)(3.14159265 * 2.0)) /* project.ts:311 abstract const-expr */
-#define K_PI ((float)3.14159265)            /* project.ts:312 */
+#define K_DET_MIN ((float)1.0e-6)           /* project.ts:165 */
+#define K_EIG_MIN ((float)0.1)              /* project.ts:181 */
+#define K_TWO_PI ((float)(3.14159265 * 2.0)) /* project.ts:260 abstract const-expr */
+#define K_PI ((float)3.14159265)            /* project.ts:261 */
 
 #define CHUNK_CAP_DEFAULT 200000u   /* pipeline.ts:32 */
 #define MAX_COV_DEFAULT 16384u      /* pipeline.ts:41 */
@@ -84,7 +84,7 @@ void orc_expf_array(const float *in, float *out, size_t n) {
     for (size_t i = 0; i < n; i++) out[i] = orc_expf(in[i]);
 }
 
-/* project.ts:106 */
+/* project.ts:55 */
 float orc_sigmoidf(float x) { return 1.0f / (1.0f + orc_expf(-x)); }
 
 /* radix-sort.ts:20-24 */
@@ -196,7 +196,7 @@ void orc_depth_sort(uint32_t *indices, const float *depths, uint32_t n) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Project (project.ts:73-383)                                              */
+/* Project (project.ts:22-332)                                              */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     float view[16];
@@ -208,7 +208,7 @@ static uint32_t num_sh(uint32_t bands) {
     return bands == 0 ? 0u : bands == 1 ? 3u : bands == 2 ? 8u : 15u;
 }
 
-/* quatRot (project.ts:101-104): t = 2 cross(q.yzw, v); v + q.x t + cross(q.yzw, t) */
+/* quatRot (project.ts:50-53): t = 2 cross(q.yzw, v); v + q.x t + cross(q.yzw, t) */
 static void quat_rot(const float q[4], float v0, float v1, float v2, float o[3]) {
     float c0 = q[2] * v2 - q[3] * v1;
     float c1 = q[3] * v0 - q[1] * v2;
@@ -222,7 +222,7 @@ static void quat_rot(const float q[4], float v0, float v1, float v2, float o[3])
     o[2] = (v2 + q[0] * t2) + d2;
 }
 
-/* evalSH (project.ts:320-381); sh points at the packed record's offset 14 */
+/* evalSH (project.ts:264-330); sh points at the packed record's offset 14 */
 static void eval_sh(const float *sh, uint32_t bands, float x, float y, float z, float out[3]) {
     uint32_t M = num_sh(bands);
     float r = 0.0f, g = 0.0f, b = 0.0f;
@@ -413,7 +413,7 @@ uint32_t orc_project_packed(const float *rec, uint32_t bands, const orc_camera *
     return project_one(rec, &u, out);
 }
 
-/* packChunkInput (preprocess.ts:189-237) for one splat */
+/* packChunkInput (preprocess.ts:90-138) for one splat */
 static void pack_one(const orc_splats *s, uint32_t g, float *o) {
     uint32_t M = num_sh(s->sh_bands);
     o[0] = s->positions[3 * (size_t)g]; o[1] = s->positions[3 * (size_t)g + 1]; o[2] = s->positions[3 * (size_t)g + 2];
@@ -744,7 +744,7 @@ void orc_aabb_extents(const orc_splats *s, float *out) {
     }
 }
 
-/* GaussianBVH (bvh.ts:95-245) */
+/* GaussianBVH (bvh.ts:33-183) */
 typedef struct {
     double mn[3], mx[3];
     uint32_t start, end;
@@ -824,7 +824,7 @@ static void bvh_init(bvh_t *b, const float *pos, const float *ext, uint32_t n) {
 
 static void bvh_free(bvh_t *b) { free(b->nodes); free(b->idx); }
 
-/* queryOverlappingInto (bvh.ts:198-239). out_len plays out.length. */
+/* queryOverlappingInto (bvh.ts:136-177). out_len plays out.length. */
 static int64_t bvh_query(const bvh_t *b, const double qmn[3], const double qmx[3],
                          uint32_t *out, uint64_t out_off, uint64_t out_len) {
     if (b->n_nodes == 0) return 0;
@@ -887,7 +887,7 @@ static double js_round(double x) {
 /* Gaussian record as uploaded (voxelize.ts:250-276) */
 typedef struct { float p[3], logit, q[4], ls[3], ext[3]; } vgauss_t;
 
-/* evalGaussian (voxel/shaders/voxelize.ts:368-390) */
+/* evalGaussian (voxel/shaders/voxelize.ts:62-84) */
 static float eval_gauss(const float vc[3], float half, const vgauss_t *g, int *passed) {
     float diff[3];
     for (int k = 0; k < 3; k++) diff[k] = vc[k] - g->p[k];

@@ -11,7 +11,7 @@
  *   camera / cull ......... src/render/camera.ts:32-99
  *   depth sort ............ src/spatial/radix-sort.ts:20-117
  *   pack .................. src/render/preprocess.ts:90-138
- *   project ............... src/render/shaders/project.ts:73-383
+ *   project ............... src/render/shaders/project.ts:22-332
  *   scan .................. src/render/shaders/prefix-sum.ts:22-86, src/render/pipeline.ts:277-286
  *   emit .................. src/render/shaders/emit-pairs.ts:27-56
  *   pair sort ............. src/render/pipeline.ts:544-578
@@ -20,8 +20,8 @@
  *   finalize .............. src/render/shaders/finalize.ts:29-42
  *   render orchestration .. src/render/pipeline.ts:181-426
  *   AABB .................. src/spatial/aabb.ts:22-62
- *   BVH ................... src/spatial/bvh.ts:104-239
- *   voxelize .............. src/voxel/voxelize.ts:59-306, src/voxel/shaders/voxelize.ts:360-453
+ *   BVH ................... src/spatial/bvh.ts:57-177
+ *   voxelize .............. src/voxel/voxelize.ts:59-306, src/voxel/shaders/voxelize.ts:17-148
  *   block mask buffer ..... src/voxel/block-mask-buffer.ts:23-58
  *
  * Transcendentals. WGSL leaves exp() precision to the adapter, so the oracle
@@ -34,7 +34,7 @@
  *
  * Contraction. C here is compiled without contraction; WGSL lets a compiler
  * fuse `a * b + c`, and GPU back ends do. The specification fuses exactly two
- * expressions of the rasterizer's loop body (rasterize.ts:70,77), both on the
+ * expressions of the rasterizer's loop body (rasterize.ts:71,77), both on the
  * device and here: the Gaussian exponent
  *   power = -0.5 * fma(C*dy, dy, fma((2B)*dx, dy, (A*dx)*dx))
  * (orc_power32) and the colour accumulation color = fma(T*alpha, c, color).
@@ -104,7 +104,7 @@ typedef struct {
 } orc_render_taps;
 
 /* --- primitives --- */
-/* the rasterizer's Gaussian exponent (rasterize.ts:70), contracted as above */
+/* the rasterizer's Gaussian exponent (rasterize.ts:71), contracted as above */
 static inline float orc_power32(float A, float B, float C, float dx, float dy) {
     return (-0.5f) * __builtin_fmaf(C * dy, dy, __builtin_fmaf((2.0f * B) * dx, dy, (A * dx) * dx));
 }

@@ -9,7 +9,7 @@ What is restated, function by function:
   js_hypot          <- Math.hypot as the JS engines implement it (the
                        max-scaled, Kahan-compensated sum of squares used by
                        V8's math.tq and JavaScriptCore's MathObject.cpp), which
-                       ply.ts:215 calls
+                       ply.ts:220 calls
   splat_bounds      <- splatBounds, src/io/ply.ts:242-255
 
 JS numbers are f64: SH_C0 * f and w / n are evaluated in f64 and rounded to

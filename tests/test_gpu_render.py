@@ -43,7 +43,7 @@ def assert_render_parity(renderer, data, cam, bg=None, chunk_cap=None, max_cover
     proj = t["projected"].reshape(-1, 12)
     rp = ref["projected"]
     culled = rp[:, 2] < 0
-    # culled splats: only slot 0 is defined (project.ts:217-219); slot 2.w is
+    # culled splats: only slot 0 is defined (project.ts:166-168); slot 2.w is
     # the reference's padding, which the device uses for the rasterizer's
     # skip threshold
     np.testing.assert_array_equal(proj[culled, :4].view(np.uint32), rp[culled, :4].view(np.uint32))

@@ -15,6 +15,10 @@ What each test fixes (a plausible slip in the oracle fails at least one):
   rectangle (including the row 0 / column 0 clamp quirk and maxCoverage
   truncation), and tile ranges against searchsorted;
 * chunked rendering is bit-identical to one chunk;
+* the equirect screen mapping against its closed form in every direction
+  (ahead, +-x, up, down, both sides of the seam), and the equirect path's
+  reuse of the pinhole Jacobian (records equal to the pinhole projection's
+  apart from the screen position);
 * the reference's browser test page cases (examples/browser/test.ts:114-251).
 """
 import math
@@ -128,6 +132,72 @@ def test_cov01_sign_flip_against_textbook_ewa():
         assert out[1] == pytest.approx(-pc[1] * f / zc + H / 2, rel=1e-5)
 
 
+def test_equirect_screen_mapping_closed_form():
+    """Equirect screen position (project.ts:251-262): a splat in camera-space
+    direction (lon, lat), with +x right, +y up and -z forward, lands at
+    sx = (lon / (2 P) + 0.5) W and sy = (0.5 - lat / P) H, P = 3.14159265
+    (the shader's literal, not pi). Covers straight ahead, towards +-x, up
+    and down, both sides of the seam behind the camera and generic
+    directions; a mirrored or swapped axis moves the splat by a large
+    fraction of the image. Directions stay away from camera-space z = 0,
+    where the reused pinhole Jacobian (project.ts:128-133) makes the
+    covariance blow up and the fade culls the splat."""
+    W, H = 4096, 2048
+    cam = S.equirect_camera(W, H)
+    P = 3.14159265
+    r = 3.0
+    dirs = [(0.0, 0.0), (1.45, 0.0), (-1.45, 0.0), (0.0, 1.4), (0.0, -1.4), (math.pi - 0.05, 0.1),
+            (-math.pi + 0.05, -0.1), (0.7, 0.4), (-2.1, -0.6), (2.9, 1.2), (-0.3, -1.3)]
+    for lon, lat in dirs:
+        pos = r * np.array([math.sin(lon) * math.cos(lat), math.sin(lat), -math.cos(lon) * math.cos(lat)])
+        rec = np.concatenate([pos, [1, 0, 0, 0], [-9, -9, -9], [2.0], [0.1, 0.2, 0.3]]).astype(np.float32)
+        out, _ = oracle.project_packed(rec, 0, cam)
+        assert out[2] > 0, (lon, lat)  # not culled
+        p = rec[:3].astype(np.float64)
+        assert abs(math.atan2(p[0], -p[2]) - lon) < 1e-6 and abs(math.asin(p[1] / np.linalg.norm(p)) - lat) < 1e-6
+        assert out[0] == pytest.approx((lon / (2 * P) + 0.5) * W, abs=2e-2), (lon, lat)
+        assert out[1] == pytest.approx((0.5 - lat / P) * H, abs=2e-2), (lon, lat)
+
+    def at(x, y, z):
+        rec = np.array([x, y, z, 1, 0, 0, 0, -9, -9, -9, 2.0, 0, 0, 0], np.float32)
+        out = oracle.project_packed(rec, 0, cam)[0]
+        assert out[2] > 0
+        return out[:2]
+    # the named directions as plain facts about the image
+    assert at(0, 0, -2) == pytest.approx([W / 2, H / 2], abs=1e-3)     # ahead: the centre
+    assert 0.7 * W < at(2, 0, -0.2)[0] < 0.75 * W                    # right of ahead: towards 3/4
+    assert 0.25 * W < at(-2, 0, -0.2)[0] < 0.3 * W                   # left: towards 1/4
+    assert at(0, 2, -0.2)[1] < 0.05 * H and at(0, -2, -0.2)[1] > 0.95 * H  # up: top rows; down: bottom
+    assert at(0.01, 0, 2)[0] > 0.99 * W and at(-0.01, 0, 2)[0] < 0.01 * W  # behind: the seam at the edges
+    # straight behind with x = +0: atan2(+0, -z) = +pi, just past the right edge
+    assert at(0, 0, 2)[0] == pytest.approx((math.pi / (2 * P) + 0.5) * W, abs=1e-2)
+
+
+def test_equirect_reuses_the_pinhole_jacobian():
+    """project.ts:125-149 runs after either projection body, so an equirect
+    splat's conic, radius, alpha and colour are those of the pinhole
+    projection of the same camera-space point (same focal, same height):
+    only the screen position differs."""
+    rng = np.random.default_rng(11)
+    W, H = 1024, 512
+    eq = S.equirect_camera(W, H)
+    ph = RenderCamera(W, H, eq.view_matrix.copy(), eq.fov_y, eq.near, eq.far, "pinhole")
+    checked = 0
+    for _ in range(40):
+        pos = np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), -rng.uniform(1.5, 4)])
+        q = rng.standard_normal(4)
+        rec = np.concatenate([pos, q / np.linalg.norm(q), rng.uniform(-4, -2, 3), [1.0], [0.1, 0.2, 0.3]])
+        rec = rec.astype(np.float32)
+        a, _ = oracle.project_packed(rec, 0, eq)
+        b, _ = oracle.project_packed(rec, 0, ph)
+        if b[2] <= 0:
+            continue
+        np.testing.assert_array_equal(a[2:], b[2:])
+        assert not np.array_equal(a[:2], b[:2])
+        checked += 1
+    assert checked >= 30
+
+
 def _sh_basis(d):
     """Real SH basis for l = 1..3 in the 3DGS sign convention, f64."""
     x, y, z = d
@@ -145,7 +215,7 @@ def _sh_basis(d):
 
 def test_sh_basis_functions_one_at_a_time():
     """Each coefficient k of channel c contributes basis_k(viewDir) to channel
-    c only, with viewDir = normalize(posCam) (camera space, project.ts:257)."""
+    c only, with viewDir = normalize(posCam) (camera space, project.ts:206)."""
     rng = np.random.default_rng(1)
     cam = S.default_camera(512, 512)
     for bands, M in [(1, 3), (2, 8), (3, 15)]:
@@ -184,7 +254,7 @@ def _brute_force_accum(proj, W, H):
         dx = px - sx
         dy = py - sy
         inbox = live & ~((np.abs(dx) > r) | (np.abs(dy) > r))
-        # rasterize.ts:70 with the specification's two contractions
+        # rasterize.ts:71 with the specification's two contractions
         power = f32(-0.5) * oracle.fmaf(C * dy, dy, oracle.fmaf((f32(2.0) * B) * dx, dy, (A * dx) * dx))
         e = oracle.expf(power).reshape(H, W)
         alpha = np.minimum(cap, a * np.maximum(f32(0.0), e - floor))

@@ -69,6 +69,71 @@ def test_single_gaussian_ball_closed_form(ls, logit, cut):
     assert r["block_idx"].size > 0
 
 
+def _rot(q):
+    """Rotation matrix of a unit quaternion [w, x, y, z] in f64."""
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("q,ls", [
+    ((0.9, 0.2, -0.3, 0.25), (math.log(0.12), math.log(0.03), math.log(0.06))),
+    ((0.5, -0.6, 0.4, 0.45), (math.log(0.02), math.log(0.1), math.log(0.04))),
+    ((math.cos(0.3), 0.0, 0.0, math.sin(0.3)), (math.log(0.15), math.log(0.025), math.log(0.025))),
+])
+def test_rotated_anisotropic_gaussian_closed_form(q, ls):
+    """A single rotated, anisotropic Gaussian (voxel/shaders/voxelize.ts:62-84):
+    the solid voxels are those where 1 - exp(-o exp(-d2/2)) >= cutoff with
+    d2 = |diag(1/s) R(q)^T (closest - g)|^2, evaluated in f64 from the
+    rotation matrix of q, apart from voxels within 1e-5 of the cutoff. The
+    inverse rotation R^T is what the shader's conjugated quaternion applies;
+    the test also checks that R instead of R^T would change many voxels, so
+    a dropped conjugate cannot pass."""
+    qn = np.array(q, np.float64)
+    qn = (qn / np.linalg.norm(qn)).astype(np.float32)
+    p = np.array([0.031, -0.017, 0.044], np.float32)
+    logit, cut, res_v = 2.0, 0.1, 0.02
+    data = SplatData(1, p, qn, np.array(ls, np.float32), [logit], [0, 0, 0])
+    r = oracle.voxelize(data, voxel_resolution=res_v, opacity_cutoff=cut, near_eps=1e-5)
+    nb = [int(v) for v in r["num_blocks_xyz"]]
+    bmn = np.array(r["bounds_min"])
+    got = _decode(r, nb[0], nb[1])
+    R = _rot(qn.astype(np.float64))
+    s = np.exp(np.array(ls, np.float32).astype(np.float64))
+    ext = oracle.aabb_extents(data).astype(np.float64)
+    o = _sig(float(np.float32(logit)))
+    half = res_v / 2
+    pd = p.astype(np.float64)
+    # every voxel centre of the grid
+    bz, by, bx, lz, ly, lx = np.meshgrid(np.arange(nb[2]), np.arange(nb[1]), np.arange(nb[0]),
+                                         np.arange(4), np.arange(4), np.arange(4), indexing="ij")
+    flat = (bx + by * nb[0] + bz * nb[0] * nb[1]).ravel()
+    lin = (lx + 4 * ly + 16 * lz).ravel()
+    vc = bmn + (np.stack([bx * 4 + lx, by * 4 + ly, bz * 4 + lz], -1).reshape(-1, 3) + 0.5) * res_v
+    inside = np.all(np.abs(vc - pd) <= ext + half, axis=1)
+    cd = np.clip(pd, vc - half, vc + half) - pd
+
+    def alpha_of(M):
+        local = cd @ M.T              # rows: M @ cd
+        d2 = np.sum((local / s) ** 2, axis=1)
+        return np.where(inside, 1 - np.exp(-o * np.exp(-0.5 * d2)), 0.0)
+
+    want_a = alpha_of(R.T)
+    wrong_a = alpha_of(R)
+    masks = np.array([got.get(int(f), 0) for f in flat], dtype=object)
+    have = np.array([(int(m) >> int(b)) & 1 for m, b in zip(masks, lin)], bool)
+    want = want_a >= cut
+    near = set(zip(r["near_block"].tolist(), r["near_bit"].tolist()))
+    bad = np.nonzero(want != have)[0]
+    for k in bad:
+        assert (int(flat[k]), int(lin[k])) in near or abs(want_a[k] - cut) < 1e-5, (flat[k], lin[k], want_a[k])
+    assert bad.size <= 2
+    assert want.sum() > 100
+    # the case separates R^T from R
+    assert ((wrong_a >= cut) != want).sum() > 20
+
+
 def test_aabb_extents_against_rotation_matrix():
     data = S.gaussian_cloud(2000, seed=4)
     ext = oracle.aabb_extents(data).reshape(-1, 3).astype(np.float64)
