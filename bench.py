@@ -89,11 +89,9 @@ def batched_views(dev, steps: int, note, tune: bool = True):
     from droplet_amd.batch import BatchRenderer, View
     from droplet_amd.renderer import Renderer, Scene
 
-    datas = [S.gaussian_cloud(n, seed=100 + i) for i, n in enumerate(S.BATCH_SCENE_SIZES)]
+    datas, views = S.batch_workload()
     scenes = [Scene(dev, d) for d in datas]
     del datas
-    cams = S.orbit_cameras(64)
-    views = [View(si, cam) for si in range(len(scenes)) for cam in cams]
     r = Renderer(dev)
     for v in views[::37]:
         r.render_scene(scenes[v.scene], v.camera, readback=False)

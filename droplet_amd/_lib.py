@@ -116,6 +116,8 @@ SIGNATURES = [
     ("dl_renderer_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_set_counting", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_stats", C.c_int, [C.c_void_p, C.POINTER(DlRenderStats)]),
+    ("dl_renderer_row_iters", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, u32p]),
+    ("dl_renderer_set_pair_budget", C.c_int, [C.c_void_p, C.c_uint64]),
     ("dl_renderer_set_taps", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_renderer_tap", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("dl_render_forward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlBackground), f32p]),
@@ -127,6 +129,7 @@ SIGNATURES = [
     ("dl_batch_destroy", None, [C.c_void_p]),
     ("dl_batch_set_graph", C.c_int, [C.c_void_p, C.c_int]),
     ("dl_batch_render", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DlView), C.c_uint32]),
+    ("dl_batch_set_pair_budget", C.c_int, [C.c_void_p, C.c_uint64]),
     ("dl_batch_read_image", C.c_int, [C.c_void_p, C.c_uint32, u8p]),
     ("dl_batch_get_stats", C.c_int, [C.c_void_p, C.POINTER(DlBatchStats)]),
     ("dl_sort_pairs_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint32, C.c_uint32]),

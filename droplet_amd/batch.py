@@ -70,6 +70,12 @@ class BatchRenderer:
         self._sizes = [(v.camera.width, v.camera.height) for v in views]
         return tune_result_dict(res)
 
+    def set_pair_budget(self, pairs: int) -> None:
+        """Groups above the budget split by views and a view above it alone
+        renders through the single-view path (dl_batch_set_pair_budget);
+        0 restores the default 2^31."""
+        L.check(L.load().dl_batch_set_pair_budget(self.handle, int(pairs)))
+
     def image(self, k: int) -> np.ndarray:
         w, h = self._sizes[k]
         out = np.empty(w * h * 4, np.uint8)

@@ -535,7 +535,8 @@ __global__ __launch_bounds__(256) void k_backward_project(SceneView sc, ProjUnif
 
 }  // namespace
 
-void Renderer::forward_train(const SceneView &sc, const dl_camera &cam, const dl_background &bg, float *out_rgb) {
+void Renderer::forward_train(const SceneView &sc, uint64_t scene_serial, const dl_camera &cam,
+                             const dl_background &bg, float *out_rgb) {
   if (cam.projection != DL_PINHOLE)
     fail(DL_ERR_INVALID, "the differentiable render supports pinhole cameras (equirect has no backward)");
   train_mode = true;
@@ -548,6 +549,7 @@ void Renderer::forward_train(const SceneView &sc, const dl_camera &cam, const dl
   }
   train_mode = false;
   train_valid = true;
+  train_scene = scene_serial;
   train_sc = sc;
   train_u = make_uniforms(cam, max_cov);
   train_bg = bg;
@@ -561,7 +563,9 @@ void Renderer::forward_train(const SceneView &sc, const dl_camera &cam, const dl
 }
 
 void Renderer::backward(const float *grad_rgb, const dl_grads &out) {
-  if (!train_valid) fail(DL_ERR_INVALID, "dl_render_backward needs a preceding dl_render_forward");
+  if (!train_valid)
+    fail(DL_ERR_INVALID, "dl_render_backward needs a dl_render_forward with no other render of this renderer since");
+  if (!scene_alive(train_scene)) fail(DL_ERR_INVALID, "the scene of the last dl_render_forward was destroyed");
   const SceneView &sc = train_sc;
   const uint32_t n = sc.count;
   const size_t npx = (size_t)width * height;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "binning.h"
@@ -42,6 +43,15 @@ struct BatchGroup {
   uint32_t pair_cap = 0;   // capacity of the group's tile lists (any pass)
 };
 
+// (tile, splat) pairs a group's lists may hold (the pair total of all its
+// views, before closed tiles are skipped)
+constexpr uint64_t kMaxGroupPairs = (1ull << 31);
+
+// thrown by an eager enqueue when a group's pair total exceeds group_pairs()
+struct GroupOverflow {
+  size_t group;
+};
+
 struct Batch {
   Context *ctx = nullptr;
   dl_renderer_opts opts{};
@@ -70,6 +80,18 @@ struct Batch {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   dl_batch_stats stats{};
+  // A group whose pair total exceeds group_pairs() is split at its middle
+  // view (view_break: a group starts there) and the call is planned again;
+  // a view that exceeds it alone is solo: the single-view renderer solo_r,
+  // which cuts its own passes by pairs, renders it after the groups into
+  // the same image array, outside the graph. Both marks hold for the layout
+  // they were found in (break_layout). pair_budget != 0 replaces the limit
+  // (to test the split at small sizes) and is the solo renderer's budget.
+  uint64_t pair_budget = 0;
+  uint64_t group_pairs() const { return pair_budget ? pair_budget : kMaxGroupPairs; }
+  std::vector<uint64_t> break_layout;
+  std::vector<uint8_t> view_break, view_solo;
+  std::unique_ptr<Renderer> solo_r;
 
   ~Batch();
   void render(const std::vector<SceneView> &scenes, const dl_view *views, uint32_t nviews);
@@ -77,6 +99,7 @@ struct Batch {
 
  private:
   void plan(const std::vector<SceneView> &scenes, const dl_view *views, uint32_t nviews);
+  void run_groups(const std::vector<SceneView> &scenes, const dl_view *views, uint32_t nviews);
   void enqueue(bool eager);
   void drop_graph();
 };

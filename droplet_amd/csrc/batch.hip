@@ -29,7 +29,6 @@ namespace {
 
 constexpr uint32_t kItemBlock = 256;
 static_assert(kItemBlock == (uint32_t)kB1, "level-1 blocks are counted in items of the level-1 kernels' block");
-constexpr uint64_t kMaxGroupPairs = (1ull << 31);
 
 // view owning block b (views ordered by first block); views with no blocks
 // never own one because the search takes the last view with blk0 <= b
@@ -163,6 +162,12 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
   }
   for (const auto &sc : scenes)
     if (sc.bands > 3) fail(DL_ERR_INVALID, "shBands must be 0..3");
+  if (lay != break_layout) {
+    // group splits and solo views were found for another layout
+    break_layout = lay;
+    view_break.assign(nviews, 0);
+    view_solo.assign(nviews, 0);
+  }
   if (lay == layout && !groups.empty()) return;
 
   // a new layout: new groups, tables and buffers; the old graph is invalid
@@ -193,12 +198,26 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
     return c;
   };
   for (uint32_t v = 0; v < nviews;) {
+    if (view_solo[v]) {
+      // rendered by the single-view renderer: only its image slot here
+      BView &b = hviews[v];
+      b.scene = views[v].scene;
+      b.n = scenes[views[v].scene].count;
+      b.width = views[v].camera.width;
+      b.height = views[v].camera.height;
+      b.tiles_x = div_up(b.width, kTile);
+      b.tiles_y = div_up(b.height, kTile);
+      b.px0 = px_total;
+      px_total += (uint64_t)b.width * b.height;
+      v++;
+      continue;
+    }
     BatchGroup g;
     g.v0 = v;
     g.seg_off = hseg.size();
     uint64_t items = 0;
     std::vector<std::vector<uint32_t>> cuts;
-    while (v < nviews && (g.nv == 0 || items + scenes[views[v].scene].count <= item_cap)) {
+    while (v < nviews && !view_solo[v] && (g.nv == 0 || (!view_break[v] && items + scenes[views[v].scene].count <= item_cap))) {
       const uint32_t n = scenes[views[v].scene].count;
       const dl_camera &c = views[v].camera;
       BView &b = hviews[v];
@@ -318,10 +337,9 @@ void Batch::enqueue(bool eager) {
       unsigned long long need = 0;
       DL_HIP(hipMemcpyAsync(&need, d_gp + gi, sizeof(need), hipMemcpyDeviceToHost, s));
       DL_HIP(hipStreamSynchronize(s));
-      if (need > kMaxGroupPairs)
-        fail(DL_ERR_CAPACITY, "a batch group needs more than 2^31 (tile, splat) pairs; use fewer or smaller views");
+      if (need > group_pairs()) throw GroupOverflow{gi};
       if (need > g.pair_cap || g.pair_cap == 0)
-        g.pair_cap = (uint32_t)std::min<uint64_t>(kMaxGroupPairs, need + need / 8 + 4096);
+        g.pair_cap = (uint32_t)std::min<uint64_t>(group_pairs(), need + need / 8 + 4096);
       pvals.reserve((size_t)g.pair_cap * 4);
     }
     const bool in_tmp = seg_radix_sort_pairs(s, keys.as<uint32_t>(), vals.as<uint32_t>(), tkeys.as<uint32_t>(),
@@ -372,7 +390,51 @@ void Batch::render(const std::vector<SceneView> &scenes, const dl_view *views, u
   hipStream_t s = ctx->stream;
   if (!ev0) DL_HIP(hipEventCreate(&ev0));
   if (!ev1) DL_HIP(hipEventCreate(&ev1));
-  plan(scenes, views, nviews);
+  for (uint32_t attempt = 0;; attempt++) {
+    plan(scenes, views, nviews);
+    try {
+      run_groups(scenes, views, nviews);
+      break;
+    } catch (const GroupOverflow &o) {
+      // split the group at its middle view, or render its one view solo,
+      // and plan again (the launches already queued finish first: the new
+      // plan reallocates their buffers)
+      DL_HIP(hipStreamSynchronize(s));
+      const BatchGroup &g = groups[o.group];
+      if (g.nv > 1)
+        view_break[g.v0 + g.nv / 2] = 1;
+      else
+        view_solo[g.v0] = 1;
+      layout.clear();
+      drop_graph();
+      if (attempt > 2 * nviews) fail(DL_ERR_INTERNAL, "batch groups could not be split to fit their pairs");
+    }
+  }
+  // solo views: the single-view renderer, into their image slots
+  for (uint32_t v = 0; v < nviews; v++) {
+    if (!view_solo[v]) continue;
+    if (!solo_r) {
+      solo_r.reset(new Renderer());
+      solo_r->ctx = ctx;
+    }
+    solo_r->opts = opts;
+    solo_r->opts.chunk_cap = 0;
+    solo_r->max_cov = max_cov;
+    solo_r->pair_budget = pair_budget;
+    solo_r->render(scenes[views[v].scene], views[v].camera, views[v].background);
+    const BView &b = hviews[v];
+    DL_HIP(hipMemcpyAsync(images.as<uint32_t>() + b.px0, solo_r->rgba.get(), (size_t)b.width * b.height * 4,
+                          hipMemcpyDeviceToDevice, s));
+    stats.num_pairs += solo_r->stats.num_pairs;
+  }
+  DL_HIP(hipStreamSynchronize(s));
+}
+
+// The groups' launches of one call: the cameras and backgrounds go into the
+// view table, then the captured graph replays (or the launches run
+// directly and are captured). Throws GroupOverflow from a direct run.
+void Batch::run_groups(const std::vector<SceneView> &scenes, const dl_view *views, uint32_t nviews) {
+  hipStream_t s = ctx->stream;
   // cameras and backgrounds of this call
   for (uint32_t v = 0; v < nviews; v++) {
     BView &b = hviews[v];
@@ -418,7 +480,7 @@ void Batch::render(const std::vector<SceneView> &scenes, const dl_view *views, u
   stats.device_ms = ms;
   stats.num_pairs = 0;
   for (auto p : hpairs) stats.num_pairs += p;
-  if (use_graph && !exec) {
+  if (use_graph && !exec && !groups.empty()) {
     DL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
       enqueue(false);

@@ -3,11 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "dl_common.h"
@@ -69,6 +72,28 @@ void DeviceBuffer::reserve(size_t bytes) {
 }
 
 void Context::make_current() const { DL_HIP(hipSetDevice(device)); }
+
+// ---- scene registry ---------------------------------------------------------
+namespace {
+std::mutex g_scene_mu;
+std::unordered_set<uint64_t> g_live_scenes;
+std::atomic<uint64_t> g_scene_serial{0};
+}  // namespace
+
+Scene::Scene() : serial(++g_scene_serial) {
+  std::lock_guard<std::mutex> lk(g_scene_mu);
+  g_live_scenes.insert(serial);
+}
+
+Scene::~Scene() {
+  std::lock_guard<std::mutex> lk(g_scene_mu);
+  g_live_scenes.erase(serial);
+}
+
+bool scene_alive(uint64_t serial) {
+  std::lock_guard<std::mutex> lk(g_scene_mu);
+  return g_live_scenes.count(serial) != 0;
+}
 
 static void upload_array(DeviceBuffer &b, const float *src, size_t count, const char *name, hipStream_t s) {
   if (count == 0) {
@@ -370,7 +395,7 @@ int dl_render_forward(dl_renderer *r, const dl_scene *scene, const dl_camera *ca
   return guard([&] {
     if (!r || !scene || !cam) fail(DL_ERR_INVALID, "null argument");
     if (scene->s.ctx != r->r.ctx) fail(DL_ERR_INVALID, "scene and renderer belong to different contexts");
-    r->r.forward_train(scene->s.view(), *cam, bg_or_default(bg), out_rgb);
+    r->r.forward_train(scene->s.view(), scene->s.serial, *cam, bg_or_default(bg), out_rgb);
   });
 }
 
@@ -399,6 +424,28 @@ int dl_renderer_set_counting(dl_renderer *r, int enabled) {
   return guard([&] {
     if (!r) fail(DL_ERR_INVALID, "null renderer");
     r->r.counting = enabled != 0;
+  });
+}
+
+int dl_renderer_set_pair_budget(dl_renderer *r, uint64_t pairs) {
+  return guard([&] {
+    if (!r) fail(DL_ERR_INVALID, "null renderer");
+    if (pairs > 0xFFFFFFF0ull) fail(DL_ERR_INVALID, "the pair budget must be below 2^32 - 16");
+    r->r.pair_budget = pairs;
+    r->r.spec_cap = 0;
+  });
+}
+
+int dl_renderer_row_iters(dl_renderer *r, uint64_t *out, uint32_t cap, uint32_t *rows) {
+  return guard([&] {
+    if (!r || !rows) fail(DL_ERR_INVALID, "null argument");
+    Renderer &R = r->r;
+    *rows = R.row_iters_rows;
+    if (!out || R.row_iters_rows == 0) return;
+    if (cap < R.row_iters_rows) fail(DL_ERR_CAPACITY, "row_iters buffer too small");
+    R.ctx->make_current();
+    DL_HIP(hipMemcpyAsync(out, R.row_iters.get(), (size_t)R.row_iters_rows * 8, hipMemcpyDeviceToHost, R.ctx->stream));
+    DL_HIP(hipStreamSynchronize(R.ctx->stream));
   });
 }
 
@@ -435,6 +482,19 @@ int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera 
     if (scene->s.ctx != r->r.ctx) fail(DL_ERR_INVALID, "scene and renderer belong to different contexts");
     Renderer &R = r->r;
     const dl_renderer_opts before = R.opts;
+    // restores the caller's options and switches if a candidate throws
+    struct Restore {
+      Renderer &R;
+      dl_renderer_opts opts;
+      bool keep_taps, timing, counting, done = false;
+      ~Restore() {
+        if (done) return;
+        R.opts = opts;
+        R.keep_taps = keep_taps;
+        R.timing = timing;
+        R.counting = counting;
+      }
+    } restore{R, before, R.keep_taps, R.timing, R.counting};
     const bool keep_taps = R.keep_taps, timing = R.timing, counting = R.counting;
     R.keep_taps = R.timing = R.counting = false;
     const dl_background bg{0.0f, 0.0f, 0.0f, 1.0f};
@@ -457,6 +517,7 @@ int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera 
     R.keep_taps = keep_taps;
     R.timing = timing;
     R.counting = counting;
+    restore.done = true;
     *out = res;
   });
 }
@@ -475,6 +536,16 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
     }
     Batch &B = b->b;
     const dl_renderer_opts before = B.opts;
+    struct Restore {
+      Batch &B;
+      dl_renderer_opts opts;
+      bool done = false;
+      ~Restore() {
+        if (done) return;
+        B.opts = opts;
+        B.layout.clear();
+      }
+    } restore{B, before};
     auto call = [&] { B.render(sv, views, num_views); };
     for (int k = 0; k < 2; k++) call();
     dl_tune_result res{};
@@ -494,6 +565,7 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
     }
     B.opts = res.chosen;
     B.layout.clear();
+    restore.done = true;
     *out = res;
   });
 }
@@ -535,6 +607,16 @@ int dl_batch_render(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, c
       sv[k] = scenes[k]->s.view();
     }
     b->b.render(sv, views, num_views);
+  });
+}
+
+int dl_batch_set_pair_budget(dl_batch *b, uint64_t pairs) {
+  return guard([&] {
+    if (!b) fail(DL_ERR_INVALID, "null batch");
+    if (pairs > 0xFFFFFFF0ull) fail(DL_ERR_INVALID, "the pair budget must be below 2^32 - 16");
+    b->b.pair_budget = pairs;
+    b->b.layout.clear();
+    b->b.break_layout.clear();
   });
 }
 

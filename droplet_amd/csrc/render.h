@@ -67,9 +67,21 @@ struct RastParams {
   // tile records the sort key of its work (null: not recorded)
   const uint32_t *order;
   uint32_t *cost_key;
+  // counting frames: loop iterations per tile row (null: not recorded)
+  unsigned long long *row_iters;
 };
 
+// True while the scene with this serial number exists (scenes register
+// themselves on construction and leave on destruction).
+bool scene_alive(uint64_t serial);
+
 struct Scene {
+  Scene();
+  ~Scene();
+  Scene(const Scene &) = delete;
+  Scene &operator=(const Scene &) = delete;
+  // unique per scene in the process, never reused
+  const uint64_t serial;
   Context *ctx = nullptr;
   uint32_t count = 0, bands = 0;
   // SH coefficients per channel held per splat: M(bands) for uploaded
@@ -105,6 +117,9 @@ struct Renderer {
   BinWorkspace bin_ws;
   // image; closed-tile flags and open-tile map of front-to-back passes
   DeviceBuffer accum, rgba, small, closed, open_pre;
+  // counting frames: rasterize loop iterations per tile row (u64[tiles_y])
+  DeviceBuffer row_iters;
+  uint32_t row_iters_rows = 0;
   uint32_t width = 0, height = 0;
   dl_render_stats stats{};
   bool timing = false;
@@ -128,6 +143,14 @@ struct Renderer {
   // outgrew the bits (up to 2), so a growing range stops re-rendering
   uint32_t key_headroom = 0;
   bool force_exact = false;
+  // (tile, splat) pair totals above budget() in one frame do not fit the
+  // u32 list offsets (2^32 - 16): such a frame is cut into passes of at
+  // most cut_pairs() pairs each, 2^30 (about 20 GB of lists and spans per
+  // pass), as the reference renders any total in 200000-splat chunks.
+  // pair_budget != 0 sets both, to test the cut at small sizes.
+  uint64_t pair_budget = 0;
+  uint64_t budget() const { return pair_budget ? pair_budget : 0xFFFFFFF0ull; }
+  uint64_t cut_pairs() const { return pair_budget ? pair_budget : (1ull << 30); }
   // Launch order of the first pass's tiles in large frames: decreasing work
   // of the same tiles in the last frame, in bands of about 1.4x (a 6-bit key
   // the rasterizer writes), and tile order inside a band, which keeps the
@@ -150,7 +173,11 @@ struct Renderer {
 
   // differentiable render (backward.hip): the forward keeps one pass of full
   // lists, the accumulator and the projected records for the backward
+  // train_valid holds from a forward until the next render of any kind;
+  // train_scene is the serial of the forward's scene, which the backward
+  // checks is still alive (its arrays are read through train_sc)
   bool train_mode = false, train_valid = false;
+  uint64_t train_scene = 0;
   SceneView train_sc{};
   ProjUniforms train_u{};
   dl_background train_bg{};
@@ -159,7 +186,8 @@ struct Renderer {
 
   ~Renderer();
   void render(const SceneView &sc, const dl_camera &cam, const dl_background &bg);
-  void forward_train(const SceneView &sc, const dl_camera &cam, const dl_background &bg, float *out_rgb);
+  void forward_train(const SceneView &sc, uint64_t scene_serial, const dl_camera &cam, const dl_background &bg,
+                     float *out_rgb);
   void backward(const float *grad_rgb, const dl_grads &out);
   int digit_bits() const { return opts.sort_digit_bits ? (int)std::min<uint32_t>(opts.sort_digit_bits, 8) : 8; }
   void read_image(uint8_t *out);

@@ -332,6 +332,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   if (cam.width == 0 || cam.height == 0) fail(DL_ERR_INVALID, "camera width and height must be > 0");
   if (sc.bands > 3) fail(DL_ERR_INVALID, "shBands must be 0..3");
   if (cam.projection != DL_PINHOLE && cam.projection != DL_EQUIRECT) fail(DL_ERR_INVALID, "unknown projection");
+  // any other render replaces the lists, records and accumulator that a
+  // backward would read
+  if (!train_mode) train_valid = false;
   hipStream_t s = ctx->stream;
   ctx->make_current();
   ev_used = 0;
@@ -364,6 +367,12 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   if (counting) DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
   k_zero_u64<<<1, 64, 0, s>>>(d_entries);
   DL_HIP(hipGetLastError());
+  row_iters_rows = 0;
+  if (counting) {
+    row_iters.reserve((size_t)tiles_y * 8);
+    DL_HIP(hipMemsetAsync(row_iters.get(), 0, (size_t)tiles_y * 8, s));
+    row_iters_rows = tiles_y;
+  }
 
   RastParams rp{};
   rp.tiles_x = tiles_x;
@@ -373,6 +382,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rp.bg_g = bg.g;
   rp.bg_b = bg.b;
   rp.bg_a = bg.a;
+  rp.row_iters = counting ? row_iters.as<unsigned long long>() : nullptr;
 
   const ProjUniforms u = make_uniforms(cam, max_cov);
 
@@ -456,8 +466,9 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
       DL_HIP(hipStreamSynchronize(s));
     }
-    // every per-chunk u32 scan below sums a subset of these pairs
-    if (pairs_all > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (tile, splat) pairs in one frame");
+    // a frame above the pair budget is cut into passes that each fit (below)
+    if (pairs_all > budget() && train_mode)
+      fail(DL_ERR_CAPACITY, "the differentiable render needs the frame's (tile, splat) pairs in one list (< 2^32)");
     if (keep_taps) {
       auto sid = to_host(sorted_ids, nvis, s);
       tap_append(taps[DL_TAP_SORTED_INDICES], sid.data(), nvis);
@@ -510,10 +521,41 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       cuts.push_back(0);
     }
     cuts.push_back(nvis);
+    // Over the pair budget (exact frames only: a speculative frame that
+    // outgrows its capacity renders again exactly): every pass whose splats
+    // hold more than cut_pairs() pairs is cut further, greedily in depth
+    // order, from the per-splat tile counts. A pass's list never exceeds
+    // its splats' counts (closed tiles only remove entries), so each fits.
+    std::vector<uint64_t> cut_bound;  // pair bound of each pass (over budget only)
+    if (!speculative && pairs_all > budget()) {
+      const uint64_t cap = cut_pairs();
+      chunk_counts.reserve((size_t)nvis * 4);
+      k_gather_counts<<<div_up(nvis, 256), 256, 0, s>>>(counts.as<uint32_t>(), sorted_ids, nvis,
+                                                        chunk_counts.as<uint32_t>());
+      DL_HIP(hipGetLastError());
+      const std::vector<uint32_t> hc = to_host(chunk_counts.as<uint32_t>(), nvis, s);
+      std::vector<uint32_t> fine{0};
+      for (size_t c = 0; c + 1 < cuts.size(); c++) {
+        uint64_t run = 0;
+        for (uint32_t j = cuts[c]; j < cuts[c + 1]; j++) {
+          if (run > 0 && run + hc[j] > cap) {
+            fine.push_back(j);
+            cut_bound.push_back(run);
+            run = 0;
+          }
+          run += hc[j];
+        }
+        fine.push_back(cuts[c + 1]);
+        cut_bound.push_back(run);
+      }
+      cuts.swap(fine);
+    }
     const uint32_t chunks = (uint32_t)cuts.size() - 1;
     // exact per-chunk pair totals (host sync) only where a pass needs them
     const bool exact_counts = !span || (chunks > 1 && !early_stop);
-    const bool track_closed = early_stop && chunks > 1;
+    // closed tiles are tracked across passes (they shrink later lists) also
+    // when the passes come from the pair budget alone
+    const bool track_closed = (early_stop || (!cut_bound.empty() && span && !keep_taps && !train_mode)) && chunks > 1;
     ranges.reserve(((size_t)num_tiles + 1) * 4);
     if (chunks > 1 || need_accum) accum.reserve(npx * 16);
     if (track_closed) {
@@ -533,7 +575,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       stats.num_chunks++;
       // ---- pair count of this chunk (and scanned offsets for emit) ----
       if (exact_counts) mark(DL_STAGE_SCAN);
-      uint64_t npairs64 = pairs_all;  // bound of every chunk
+      uint64_t npairs64 = cut_bound.empty() ? pairs_all : cut_bound[c];  // bound of this chunk
       if (exact_counts) {
         chunk_counts.reserve((size_t)cn * 4);
         offsets.reserve((size_t)cn * 4);
@@ -681,8 +723,12 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     if (n > 0) {
       const unsigned long long all = u64_at(8);
       last_pairs_all = all;
-      if (all <= 0xFFFFFFF0ull)
-        spec_cap = std::max<uint64_t>(spec_cap, std::min<uint64_t>(all + all / 8 + 4096, 0xFFFFFFF0ull));
+      // frames over the pair budget stay exact (their passes are cut from
+      // the per-splat counts, which a speculative frame does not write)
+      if (all <= budget())
+        spec_cap = std::max<uint64_t>(spec_cap, std::min<uint64_t>(all + all / 8 + 4096, budget()));
+      else
+        spec_cap = 0;
       // span of the visible depth keys: (kmax - kmin + 1) and the sentinel
       // must fit the sort's bits
       bool keys_fit = true;

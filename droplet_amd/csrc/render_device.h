@@ -568,6 +568,7 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
     if (lane == 0) {
       atomicAdd(d_iters, (unsigned long long)s);
       atomicAdd(d_iters + 1, (unsigned long long)consumed);
+      if (p.row_iters) atomicAdd(p.row_iters + ty, (unsigned long long)s);
     }
   }
 }

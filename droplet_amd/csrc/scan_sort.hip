@@ -363,6 +363,11 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v) {
 
 // Segment tables: units[s] = ceil(len(s) / unit), then (after a scan into
 // first[]) owner[u] = s for every unit u of segment s.
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t *__restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
 __global__ __launch_bounds__(256) void k_seg_units(const uint32_t *__restrict__ seg_start, uint32_t nseg, uint32_t unit,
                                                    uint32_t *__restrict__ units) {
   const uint32_t sg = blockIdx.x * 256 + threadIdx.x;
@@ -483,7 +488,14 @@ size_t radix_workspace_bytes(size_t n, int key_bits) {
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
                       size_t n, int key_bits, void *ws, int max_digit_bits, const RadixKeyMap &map,
                       bool index_vals) {
-  if (n <= 1 || key_bits <= 0) return false;
+  if (n <= 1 || key_bits <= 0) {
+    // nothing to reorder; index values are still owed: value i is i
+    if (index_vals && n > 0) {
+      k_iota_u32<<<div_up(n, 256), 256, 0, s>>>(vals, n);
+      DL_HIP(hipGetLastError());
+    }
+    return false;
+  }
   if (key_bits > 32) key_bits = 32;
   const uint32_t nb = div_up(n, kTileN);
   uint8_t *w = static_cast<uint8_t *>(ws);

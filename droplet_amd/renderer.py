@@ -203,6 +203,23 @@ class Renderer:
     def set_counting(self, enabled: bool) -> None:
         L.check(L.load().dl_renderer_set_counting(self.handle, int(enabled)))
 
+    def set_pair_budget(self, pairs: int) -> None:
+        """Frames with more (tile, splat) pairs than the budget render in
+        passes of at most that many pairs (default: 2^32 - 16 triggers,
+        2^30 per pass); 0 restores the default (dl_renderer_set_pair_budget)."""
+        L.check(L.load().dl_renderer_set_pair_budget(self.handle, int(pairs)))
+
+    def row_iters(self) -> np.ndarray:
+        """Rasterize loop iterations per tile row of the last frame rendered
+        with counting enabled (empty if it did not count)."""
+        rows = C.c_uint32()
+        L.check(L.load().dl_renderer_row_iters(self.handle, None, 0, C.byref(rows)))
+        out = np.zeros(rows.value, np.uint64)
+        if rows.value:
+            L.check(L.load().dl_renderer_row_iters(self.handle, out.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                   rows.value, C.byref(rows)))
+        return out
+
     def set_taps(self, enabled: bool) -> None:
         L.check(L.load().dl_renderer_set_taps(self.handle, int(enabled)))
 

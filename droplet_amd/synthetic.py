@@ -100,6 +100,17 @@ def orbit_cameras(count: int = 64, radius: float = 4.0) -> List[RenderCamera]:
     return cams
 
 
+def batch_workload(cameras: int = 64):
+    """Batched-views workload: 8 scenes with the render distributions sized
+    BATCH_SCENE_SIZES (seeds 100..107) and `cameras` orbit cameras shared by
+    all scenes. Returns (scenes' SplatData, views), views scene-major: view
+    si * cameras + k is scene si seen by camera k."""
+    from .batch import View
+    datas = [gaussian_cloud(n, seed=100 + i) for i, n in enumerate(BATCH_SCENE_SIZES)]
+    cams = orbit_cameras(cameras)
+    return datas, [View(si, cam) for si in range(len(datas)) for cam in cams]
+
+
 # ---------------------------------------------------------------------------
 # Browser test page fixtures (examples/browser/test.ts:45-103)
 # ---------------------------------------------------------------------------

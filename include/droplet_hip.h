@@ -179,6 +179,19 @@ int dl_renderer_set_timing(dl_renderer *r, int enabled);
  * with an instrumented kernel variant; meant for one untimed frame. */
 int dl_renderer_set_counting(dl_renderer *r, int enabled);
 int dl_renderer_stats(dl_renderer *r, dl_render_stats *out);
+/* A frame whose (tile, splat) pairs exceed 2^32 - 16 renders in passes of at
+ * most 2^30 pairs each, cut in depth order from the per-splat tile counts
+ * (the image is unchanged: passes resume like the reference's 200000-splat
+ * chunks, pipeline.ts:221-423). pairs != 0 lowers both figures to `pairs`,
+ * so the cut can be tested at small sizes; 0 restores the defaults. No
+ * reference counterpart (the reference's chunks bound each pass by splats). */
+int dl_renderer_set_pair_budget(dl_renderer *r, uint64_t pairs);
+/* Rasterize loop iterations per tile row of the last frame rendered with
+ * counting enabled (rasterize.ts:62-79 iterations of the row's pixels, the
+ * per-row split of dl_render_stats.pixel_iters): *rows = tile rows
+ * recorded (0 when the last frame did not count); out (may be NULL) takes
+ * up to cap values. No reference counterpart: a parity aid. */
+int dl_renderer_row_iters(dl_renderer *r, uint64_t *out, uint32_t cap, uint32_t *rows);
 
 /* Debug taps for parity tests: when enabled, the next render keeps copies of
  * its intermediate arrays (concatenated over chunks). */
@@ -257,6 +270,14 @@ void dl_batch_destroy(dl_batch *b);
 int dl_batch_set_graph(dl_batch *b, int enabled);
 int dl_batch_render(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes, const dl_view *views,
                     uint32_t num_views);
+/* A group whose views hold more than 2^31 (tile, splat) pairs is split in
+ * two by views, and planned again; a view that alone holds more is rendered
+ * by the single-view path, which cuts its passes by pairs
+ * (dl_renderer_set_pair_budget), into the same image array after the
+ * groups. pairs != 0 lowers the 2^31 and is that renderer's budget, so the
+ * split can be tested at small sizes; 0 restores the defaults. No reference
+ * counterpart. */
+int dl_batch_set_pair_budget(dl_batch *b, uint64_t pairs);
 /* RGBA8 of view `view` of the last call (width*height*4 bytes) */
 int dl_batch_read_image(dl_batch *b, uint32_t view, uint8_t *out_rgba);
 int dl_batch_get_stats(dl_batch *b, dl_batch_stats *out);

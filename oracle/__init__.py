@@ -86,6 +86,10 @@ def lib():
         L.orc_render.restype = C.c_int
         L.orc_render.argtypes = [C.POINTER(OrcSplats), C.POINTER(OrcCamera), f32p, C.c_uint32, C.c_uint32,
                                  u8p, C.POINTER(OrcTaps), C.c_int]
+        L.orc_render_rows.restype = C.c_int
+        L.orc_render_rows.argtypes = [C.POINTER(OrcSplats), C.POINTER(OrcCamera), f32p, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, u8p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64), C.c_int]
         L.orc_render_timed.restype = C.c_int
         L.orc_render_timed.argtypes = [C.POINTER(OrcSplats), C.POINTER(OrcCamera), C.c_uint32, C.c_int,
                                        C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
@@ -272,6 +276,35 @@ def render(splats, cam, background=None, chunk_cap: int = 0, max_coverage: int =
         out["chunk_pairs"] = arrays["chunk_pairs"][:t.num_chunks]
         out["accum"] = arrays["accum"].reshape(cam.height, cam.width, 4)
     return out
+
+
+def render_rows(splats, cam, background=None, row_step: int = 16, row_phase: int = 0, chunk_cap: int = 0,
+                max_coverage: int = 0, threads: Optional[int] = None) -> dict:
+    """The reference pipeline on the tile rows ty % row_step == row_phase only
+    (their pairs are emitted, sorted and rasterized; a tile's list is the
+    same as in the full emit). Returns rgba (H*W*4, other rows hold the
+    background), the kept tile rows, num_visible, num_pairs (every pair of
+    the frame) and row_iters (loop iterations per tile row, 0 for rows not
+    kept). chunk_cap 0 is the reference's 200000."""
+    L = lib()
+    threads = threads or cpu_threads()
+    s = _splats(splats)
+    c = camera_struct(cam)
+    bg = None
+    if background is not None:
+        bg = np.array([background.r, background.g, background.b, background.a], np.float32)
+    rgba = np.zeros(cam.width * cam.height * 4, np.uint8)
+    tiles_y = (cam.height + 15) // 16
+    row_iters = np.zeros(tiles_y, np.uint64)
+    nvis, npairs = C.c_uint32(), C.c_uint64()
+    rc = L.orc_render_rows(C.byref(s), C.byref(c), _p(bg, f32p), chunk_cap, max_coverage, row_step, row_phase,
+                           _p(rgba, u8p), C.byref(nvis), C.byref(npairs),
+                           row_iters.ctypes.data_as(C.POINTER(C.c_uint64)), threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_render_rows failed ({rc})")
+    rows = np.arange(row_phase % row_step, tiles_y, row_step)
+    return {"rgba": rgba, "tile_rows": rows, "num_visible": int(nvis.value), "num_pairs": int(npairs.value),
+            "row_iters": row_iters}
 
 
 def render_timed(splats, cam, row_step: int = 1, threads: Optional[int] = None):

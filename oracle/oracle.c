@@ -429,6 +429,42 @@ static void pack_one(const orc_splats *s, uint32_t g, float *o) {
 /* Emit / pair sort / ranges / rasterize / finalize                         */
 /* ------------------------------------------------------------------------ */
 /* emitPairsShader (emit-pairs.ts:27-56) for one splat */
+/* Row filter of the sampled-row mode: tile row ty is kept when
+ * ty % step == phase (step 1 keeps every row). */
+typedef struct { uint32_t step, phase; } rows_t;
+static const rows_t ALL_ROWS = {1u, 0u};
+static inline int row_kept(const rows_t *rows, uint32_t ty) { return ty % rows->step == rows->phase; }
+
+/* emitPairsShader (emit-pairs.ts:27-56) for one splat, keeping only the pairs
+ * of kept tile rows: maxCoverage counts every pair (emit-pairs.ts:48), so the
+ * kept pairs are exactly those the full emit writes into kept rows, in the
+ * same order. Returns the number written. With pt == NULL it only counts. */
+static uint32_t emit_rows(const float *v0, uint32_t idx, uint32_t offset, const unif_t *u, const rows_t *rows,
+                          uint32_t *pt, uint32_t *ps) {
+    float sx = v0[0], sy = v0[1], radius = v0[2];
+    if (radius <= 0.0f) return 0;
+    uint32_t min_tx = orc_u32sat(wmax(0.0f, floorf((sx - radius) / 16.0f)));
+    uint32_t min_ty = orc_u32sat(wmax(0.0f, floorf((sy - radius) / 16.0f)));
+    uint32_t max_tx = orc_u32sat(wmin((float)u->tiles_x - 1.0f, floorf((sx + radius) / 16.0f)));
+    uint32_t max_ty = orc_u32sat(wmin((float)u->tiles_y - 1.0f, floorf((sy + radius) / 16.0f)));
+    if (max_tx < min_tx || max_ty < min_ty) return 0;
+    uint32_t written = 0, kept = 0;
+    for (uint32_t ty = min_ty; ty <= max_ty; ty++) {
+        for (uint32_t tx = min_tx; tx <= max_tx; tx++) {
+            if (written >= u->max_cov) return kept;
+            if (row_kept(rows, ty)) {
+                if (pt) {
+                    pt[offset + kept] = ty * u->tiles_x + tx;
+                    ps[offset + kept] = idx;
+                }
+                kept++;
+            }
+            written++;
+        }
+    }
+    return kept;
+}
+
 static void emit_one(const float *v0, uint32_t idx, uint32_t offset, const unif_t *u,
                      uint32_t *pt, uint32_t *ps) {
     float sx = v0[0], sy = v0[1], radius = v0[2];
@@ -545,16 +581,19 @@ static void raster_tile(uint32_t tx, uint32_t ty, const unif_t *u, const float *
     st->tile_iters += tmax;
 }
 
+/* rasterizes the kept tile rows; row_iters (may be NULL) gets each tile
+ * row's loop iterations added */
 static void rasterize(const unif_t *u, const float *proj, const uint32_t *off, const uint32_t *ps,
-                      float *accum, rstats_t *st, uint32_t row_step, int nthreads) {
+                      float *accum, rstats_t *st, const rows_t *rows, uint64_t *row_iters, int nthreads) {
     uint64_t it = 0, ev = 0, bl = 0, ti = 0;
     int ty_n = (int)u->tiles_y;
     (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : it, ev, bl, ti) num_threads(nthreads)
     for (int ty = 0; ty < ty_n; ty++) {
-        if ((uint32_t)ty % row_step) continue;
+        if (!row_kept(rows, (uint32_t)ty)) continue;
         rstats_t s = {0, 0, 0, 0};
         for (uint32_t tx = 0; tx < u->tiles_x; tx++) raster_tile(tx, (uint32_t)ty, u, proj, off, ps, accum, &s);
+        if (row_iters) row_iters[ty] += s.iters;
         it += s.iters; ev += s.evals; bl += s.blends; ti += s.tile_iters;
     }
     st->iters += it; st->evals += ev; st->blends += bl; st->tile_iters += ti;
@@ -580,9 +619,17 @@ static void finalize(uint32_t W, uint32_t H, const float *accum, const float bg[
 /* ------------------------------------------------------------------------ */
 /* Render orchestration (pipeline.ts:181-473)                               */
 /* ------------------------------------------------------------------------ */
+/* The reference pipeline. rows selects the tile rows that are rasterized;
+ * with emit_all_rows == 0 the emit keeps only the pairs of those rows too
+ * (a tile's list is the same either way: the pair sort is stable and keeps
+ * no pair of another tile between two of its own), so a frame whose full
+ * lists would not fit in memory can be checked on a sample of its rows.
+ * num_pairs always counts every pair. Returns -2 when a chunk would hold
+ * 2^32 or more pairs (the u32 pair index of the reference's buffers). */
 static int render_impl(const orc_splats *s, const orc_camera *cam, const float *bg_in,
                        uint32_t chunk_cap_opt, uint32_t max_cov, uint8_t *out_rgba,
-                       orc_render_taps *taps, int nthreads, uint32_t row_step, double *stage_s) {
+                       orc_render_taps *taps, int nthreads, const rows_t *rows, int emit_all_rows,
+                       uint64_t *row_iters, double *stage_s) {
     if (!s || !cam) return -1;
     if (s->sh_bands > 3) return -1;
     float bg[4] = {0.0f, 0.0f, 0.0f, 1.0f};
@@ -618,6 +665,7 @@ static int render_impl(const orc_splats *s, const orc_camera *cam, const float *
     rstats_t st = {0, 0, 0, 0};
     uint64_t pairs_total = 0, pairs_written = 0;
     uint32_t nchunks = 0;
+    int rc = 0;
     if (nvis > 0) {
         uint32_t chunk_cap = chunk_cap_opt < nvis ? chunk_cap_opt : nvis;
         float *stage = (float *)malloc(sizeof(float) * (size_t)stride * chunk_cap);
@@ -641,10 +689,23 @@ static int render_impl(const orc_splats *s, const orc_camera *cam, const float *
                     memcpy(taps->tile_counts + c0, counts, sizeof(uint32_t) * cn);
             }
             uint64_t sum = 0;
-            for (uint32_t i = 0; i < cn; i++) { offs[i] = (uint32_t)sum; sum += counts[i]; }
-            uint32_t npairs = (uint32_t)sum;
-            if (taps && taps->chunk_pairs && chunk_id < taps->cap_chunks) taps->chunk_pairs[chunk_id] = npairs;
+            for (uint32_t i = 0; i < cn; i++) sum += counts[i];
+            if (sum > 0xFFFFFFFFull) { rc = -2; free(stage); free(proj); free(counts); free(offs); free(toff); goto done; }
             pairs_total += sum;
+            if (taps && taps->chunk_pairs && chunk_id < taps->cap_chunks) taps->chunk_pairs[chunk_id] = (uint32_t)sum;
+            if (!emit_all_rows) {
+                /* counts of the kept rows only */
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+                for (int64_t i = 0; i < (int64_t)cn; i++)
+                    counts[i] = emit_rows(proj + 12 * (size_t)i, (uint32_t)i, 0, &u, rows, NULL, NULL);
+                sum = 0;
+                for (uint32_t i = 0; i < cn; i++) sum += counts[i];
+            }
+            {
+                uint64_t run = 0;
+                for (uint32_t i = 0; i < cn; i++) { offs[i] = (uint32_t)run; run += counts[i]; }
+            }
+            uint32_t npairs = (uint32_t)sum;
             if (npairs == 0) {
                 if (taps && taps->tile_offsets && chunk_id < taps->cap_chunks)
                     for (uint32_t t = 0; t <= num_tiles; t++) taps->tile_offsets[(size_t)chunk_id * (num_tiles + 1) + t] = 0;
@@ -652,8 +713,14 @@ static int render_impl(const orc_splats *s, const orc_camera *cam, const float *
             }
             uint32_t *pt = (uint32_t *)malloc(sizeof(uint32_t) * npairs);
             uint32_t *ps = (uint32_t *)malloc(sizeof(uint32_t) * npairs);
+            if (emit_all_rows) {
 #pragma omp parallel for schedule(static) num_threads(nthreads)
-            for (int64_t i = 0; i < (int64_t)cn; i++) emit_one(proj + 12 * (size_t)i, (uint32_t)i, offs[i], &u, pt, ps);
+                for (int64_t i = 0; i < (int64_t)cn; i++) emit_one(proj + 12 * (size_t)i, (uint32_t)i, offs[i], &u, pt, ps);
+            } else {
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+                for (int64_t i = 0; i < (int64_t)cn; i++)
+                    (void)emit_rows(proj + 12 * (size_t)i, (uint32_t)i, offs[i], &u, rows, pt, ps);
+            }
             t1 = now_s(); if (stage_s) stage_s[3] += t1 - t0; t0 = t1;
             orc_sort_pairs(pt, ps, npairs);
             t1 = now_s(); if (stage_s) stage_s[4] += t1 - t0; t0 = t1;
@@ -668,7 +735,7 @@ static int render_impl(const orc_splats *s, const orc_camera *cam, const float *
                     memcpy(taps->tile_offsets + (size_t)chunk_id * (num_tiles + 1), toff, sizeof(uint32_t) * (num_tiles + 1));
             }
             pairs_written += npairs;
-            rasterize(&u, proj, toff, ps, accum, &st, row_step, nthreads);
+            rasterize(&u, proj, toff, ps, accum, &st, rows, row_iters, nthreads);
             t1 = now_s(); if (stage_s) stage_s[6] += t1 - t0; t0 = t1;
             free(pt); free(ps);
         }
@@ -690,15 +757,30 @@ static int render_impl(const orc_splats *s, const orc_camera *cam, const float *
             for (uint32_t j = 0; j < nvis; j++) taps->depth_keys[j] = keys_unsorted[idx[j]];
         if (taps->accum) memcpy(taps->accum, accum, sizeof(float) * 4 * npx);
     }
+done:
     free(keys_unsorted);
     free(idx); free(dep); free(accum);
-    return 0;
+    return rc;
 }
 
 int orc_render(const orc_splats *s, const orc_camera *cam, const float *bg, uint32_t chunk_cap,
                uint32_t max_coverage, uint8_t *out_rgba, orc_render_taps *taps, int num_threads) {
     if (num_threads <= 0) num_threads = 1;
-    return render_impl(s, cam, bg, chunk_cap, max_coverage, out_rgba, taps, num_threads, 1, NULL);
+    return render_impl(s, cam, bg, chunk_cap, max_coverage, out_rgba, taps, num_threads, &ALL_ROWS, 1, NULL, NULL);
+}
+
+int orc_render_rows(const orc_splats *s, const orc_camera *cam, const float *bg, uint32_t chunk_cap,
+                    uint32_t max_coverage, uint32_t row_step, uint32_t row_phase, uint8_t *out_rgba,
+                    uint32_t *num_visible, uint64_t *num_pairs, uint64_t *row_iters, int num_threads) {
+    if (num_threads <= 0) num_threads = 1;
+    if (row_step == 0) return -1;
+    const rows_t rows = {row_step, row_phase % row_step};
+    orc_render_taps taps;
+    memset(&taps, 0, sizeof(taps));
+    int rc = render_impl(s, cam, bg, chunk_cap, max_coverage, out_rgba, &taps, num_threads, &rows, 0, row_iters, NULL);
+    if (num_visible) *num_visible = taps.num_visible;
+    if (num_pairs) *num_pairs = taps.num_pairs;
+    return rc;
 }
 
 int orc_render_timed(const orc_splats *s, const orc_camera *cam, uint32_t row_step, int num_threads,
@@ -708,8 +790,10 @@ int orc_render_timed(const orc_splats *s, const orc_camera *cam, uint32_t row_st
     orc_render_taps taps;
     memset(&taps, 0, sizeof(taps));
     for (int k = 0; k < 7; k++) stage_seconds[k] = 0.0;
-    /* one chunk, as the device path runs */
-    int rc = render_impl(s, cam, NULL, 0xFFFFFFFFu, 0, NULL, &taps, num_threads, row_step, stage_seconds);
+    /* one chunk, as the device path runs; every pair is emitted and sorted,
+     * only every row_step-th tile row is rasterized */
+    const rows_t rows = {row_step, 0u};
+    int rc = render_impl(s, cam, NULL, 0xFFFFFFFFu, 0, NULL, &taps, num_threads, &rows, 1, NULL, stage_seconds);
     if (num_pairs) *num_pairs = taps.num_pairs;
     return rc;
 }

@@ -143,6 +143,14 @@ void orc_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n);
  * scan+emit, pair sort, ranges, rasterize(sampled rows)+finalize. */
 int orc_render_timed(const orc_splats *s, const orc_camera *cam, uint32_t row_step,
                      int num_threads, double *stage_seconds, uint64_t *num_pairs);
+/* The reference pipeline rasterizing only tile rows ty with ty % row_step ==
+ * row_phase, and emitting only their pairs (their lists equal the full
+ * emit's): out_rgba holds those rows (others show the background),
+ * num_pairs counts every pair, row_iters[tiles_y] (may be NULL) gets each
+ * kept row's loop iterations. -2: a chunk would hold 2^32 pairs or more. */
+int orc_render_rows(const orc_splats *s, const orc_camera *cam, const float *bg, uint32_t chunk_cap,
+                    uint32_t max_coverage, uint32_t row_step, uint32_t row_phase, uint8_t *out_rgba,
+                    uint32_t *num_visible, uint64_t *num_pairs, uint64_t *row_iters, int num_threads);
 
 /* --- voxelizer (voxelize.ts:59-306) --- */
 typedef struct {

@@ -162,3 +162,30 @@ def test_batch_autotune_keeps_images(device, scenes):
         _check_against_plain(device, datas, b, views)
     finally:
         b.destroy()
+
+
+def test_batch_splits_groups_over_the_pair_budget(device, scenes):
+    """Groups above the pair budget split by views; a view above it alone
+    renders through the single-view path (which cuts its own passes by the
+    same budget) into its image slot. A budget at the median view's pair
+    count forces both; every view still equals the single-view render, on
+    the direct call, the graph replay and a call with new cameras."""
+    datas, sc = scenes
+    views = _views()
+    pairs = [oracle.render(datas[v.scene], v.camera, taps=False)["num_pairs"] for v in views]
+    budget = int(sorted(p for p in pairs if p)[len([p for p in pairs if p]) // 2])
+    print(f"case: budget {budget} pairs; views over it: {sum(p > budget for p in pairs)} of {len(views)}", flush=True)
+    assert sum(p > budget for p in pairs) >= 3
+    b = BatchRenderer(device, use_graph=True)
+    try:
+        b.set_pair_budget(budget)
+        for call, vs in enumerate((views, views, _views(phase=0.4))):
+            b.render(sc, vs)
+            st = b.stats()
+            print(f"case: call {call}: {st['num_groups']} groups, {st['graph_replays']} replays", flush=True)
+            assert st["num_groups"] >= 2
+            _check_against_plain(device, datas, b, vs)
+            assert st["num_pairs"] == sum(oracle.render(datas[v.scene], v.camera, taps=False)["num_pairs"]
+                                          for v in vs)
+    finally:
+        b.destroy()

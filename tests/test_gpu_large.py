@@ -1,0 +1,188 @@
+"""GPU parity at the workloads' own sizes, through the C ABI.
+
+* Large forward render: 6M Gaussians, shBands 3, 3840x2160 pinhole and
+  4096x2048 equirect from the origin. Several consecutive frames on one
+  renderer, so the later ones run the production path (no host read after
+  the cull, depth sort on the key span, first-pass tiles in last frame's
+  work order, front-to-back passes with live-splat compaction). The pinhole
+  image is compared whole; the equirect frame (22.4e9 tile pairs, beyond
+  the u32 lists of one pass) on every 16th tile row, whose lists the oracle
+  builds alone (oracle.render_rows). Visible splats, pairs and the
+  rasterize loop count per tile row are exact.
+* Voxelizer: 1M Gaussians over the whole grid; masks bit-exact apart from
+  near-cutoff voxels, which are counted.
+* Batched views: 8 scenes (50k-4M) x 64 orbit cameras, every view against
+  the single-view renderer, two against the oracle.
+* The pass cut of frames over the pair budget, at small sizes (a low
+  budget), against the oracle.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from droplet_amd import synthetic as S
+from droplet_amd.renderer import Renderer, RendererOptions, Scene
+from droplet_amd.types import BackgroundRGBA
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def large(device):
+    t0 = time.time()
+    data, cam = S.large_render_workload()
+    sc = Scene(device, data)
+    print(f"case: large scene 6M bands 3 ready in {time.time() - t0:.1f} s", flush=True)
+    yield data, cam, sc
+    sc.destroy()
+
+
+def _rows_equal(got, ref, cam, tile_rows):
+    a = got.reshape(cam.height, cam.width, 4)
+    b = ref.reshape(cam.height, cam.width, 4)
+    for ty in tile_rows:
+        y0, y1 = 16 * int(ty), min(16 * int(ty) + 16, cam.height)
+        if not np.array_equal(a[y0:y1], b[y0:y1]):
+            bad = np.argwhere((a[y0:y1] != b[y0:y1]).any(-1))
+            raise AssertionError(f"tile row {ty}: {len(bad)} pixels differ, first at y={y0 + bad[0][0]} "
+                                 f"x={bad[0][1]}: {a[y0 + bad[0][0], bad[0][1]]} vs {b[y0 + bad[0][0], bad[0][1]]}")
+
+
+def _consecutive_frames(device, sc, cam, ref, tile_rows, frames=3):
+    r = Renderer(device)
+    try:
+        for f in range(frames):
+            t0 = time.time()
+            img = r.render_scene(sc, cam)
+            st = r.stats()
+            print(f"case: {cam.projection} {cam.width}x{cam.height} frame {f}: {time.time() - t0:.2f} s, "
+                  f"{st['num_chunks']} passes, {st['num_pairs']} pairs, {st['list_entries']} list entries", flush=True)
+            assert st["num_visible"] == ref["num_visible"]
+            assert st["num_pairs"] == ref["num_pairs"]
+            _rows_equal(img, ref["rgba"], cam, tile_rows)
+        r.set_counting(True)
+        img = r.render_scene(sc, cam)
+        r.set_counting(False)
+        _rows_equal(img, ref["rgba"], cam, tile_rows)
+        rows = r.row_iters()
+        np.testing.assert_array_equal(rows[tile_rows], ref["row_iters"][tile_rows])
+        print(f"case: loop iterations over the checked rows {int(rows[tile_rows].sum())} equal the oracle's",
+              flush=True)
+        if len(tile_rows) == rows.size:
+            assert r.stats()["pixel_iters"] == int(ref["row_iters"].sum())
+    finally:
+        r.destroy()
+
+
+def test_large_pinhole_frames_match_oracle(device, large):
+    data, cam, sc = large
+    t0 = time.time()
+    ref = oracle.render_rows(data, cam, row_step=1)
+    print(f"case: oracle 6M 3840x2160 pinhole, whole image, {time.time() - t0:.1f} s, "
+          f"{ref['num_pairs']} pairs", flush=True)
+    _consecutive_frames(device, sc, cam, ref, ref["tile_rows"])
+
+
+def test_large_equirect_frames_match_oracle(device, large):
+    data, _, sc = large
+    cam = S.equirect_camera(4096, 2048)
+    t0 = time.time()
+    ref = oracle.render_rows(data, cam, row_step=16, row_phase=0)
+    print(f"case: oracle 6M 4096x2048 equirect, tile rows {ref['tile_rows'].tolist()}, {time.time() - t0:.1f} s, "
+          f"{ref['num_pairs']} pairs", flush=True)
+    assert ref["num_pairs"] > 1 << 32  # beyond one pass's u32 lists
+    _consecutive_frames(device, sc, cam, ref, ref["tile_rows"], frames=2)
+
+
+@pytest.mark.parametrize("budget,opts", [
+    (3000, None),
+    (20_000, RendererOptions(binning=1)),
+    (9000, RendererOptions(full_lists=1)),
+    (5000, RendererOptions(chunk_cap=4000, max_coverage_per_splat=40)),
+])
+def test_pair_budget_cuts_passes(device, budget, opts):
+    """A frame above the pair budget renders in passes cut from the per-splat
+    tile counts; a low budget forces that at a small size. Image, visible
+    splats, pair total and loop count equal the oracle's, on consecutive
+    frames (they stay exact: no speculation over the budget)."""
+    data = S.gaussian_cloud(40_000, seed=71, log_scale_range=(np.log(0.005), np.log(0.08)))
+    cam = S.default_camera(480, 320)
+    bg = BackgroundRGBA(0.3, 0.2, 0.1, 1.0)
+    mc = opts.max_coverage_per_splat if opts else 0
+    ref = oracle.render(data, cam, background=bg, max_coverage=mc, taps=True)
+    assert ref["num_pairs"] > 8 * budget
+    r = Renderer(device, opts)
+    try:
+        r.set_pair_budget(budget)
+        for f in range(2):
+            img = r.render(data, cam, bg)
+            st = r.stats()
+            print(f"case: budget {budget} frame {f}: {st['num_chunks']} passes for {st['num_pairs']} pairs", flush=True)
+            assert st["num_chunks"] >= ref["num_pairs"] // budget
+            np.testing.assert_array_equal(img, ref["rgba"])
+            assert st["num_visible"] == ref["num_visible"] and st["num_pairs"] == ref["num_pairs"]
+        r.set_counting(True)
+        r.render(data, cam, bg)
+        r.set_counting(False)
+        assert r.stats()["pixel_iters"] == ref["pixel_iters"]
+        # the accumulator of a frame with taps (its passes all keep their lists)
+        r.set_taps(True)
+        r.render(data, cam, bg)
+        acc = r.tap("accum")
+        r.set_taps(False)
+        np.testing.assert_array_equal(acc.view(np.uint32), ref["accum"].reshape(-1).view(np.uint32))
+        # back to the default budget: one pass again, same image
+        r.set_pair_budget(0)
+        np.testing.assert_array_equal(r.render(data, cam, bg), ref["rgba"])
+    finally:
+        r.destroy()
+
+
+def test_voxelizer_workload_whole_grid(device):
+    """The voxelizer workload (1M Gaussians over a 10 x 3 x 10 m box, res
+    0.02, cutoff 0.1) against the oracle on the whole grid."""
+    from test_gpu_voxel import assert_voxel_parity
+    data, o = S.voxel_workload()
+    t0 = time.time()
+    got, ref = assert_voxel_parity(device, data, o["voxel_resolution"], o["opacity_cutoff"])
+    print(f"case: voxelizer 1M whole grid, {ref['block_idx'].size} blocks, {ref['num_near']} near-cutoff voxels, "
+          f"{time.time() - t0:.1f} s", flush=True)
+    assert ref["block_idx"].size > 500_000
+
+
+def test_batched_views_workload(device):
+    """The batched-views workload: 8 scenes of 50k-4M Gaussians x 64 orbit
+    cameras at 640x360 / 1280x720 / 1920x1080 (512 views) in one call per
+    stage, replayed from its graph: every view equals the single-view
+    render, two views the oracle's."""
+    from droplet_amd.batch import BatchRenderer
+    datas, views = S.batch_workload()
+    scenes = [Scene(device, d) for d in datas]
+    b = BatchRenderer(device)
+    r = Renderer(device)
+    try:
+        b.render(scenes, views)   # direct, captures the graph
+        b.render(scenes, views)   # replay
+        st = b.stats()
+        print(f"case: batch {st['num_views']} views, {st['num_groups']} groups, {st['graph_replays']} replays, "
+              f"{st['num_pairs']} pairs", flush=True)
+        assert st["graph_replays"] >= 1
+        for k, v in enumerate(views):
+            ref = r.render_scene(scenes[v.scene], v.camera)
+            got = b.image(k)
+            if not np.array_equal(got, ref):
+                raise AssertionError(f"view {k} (scene {v.scene}, {v.camera.width}x{v.camera.height}) differs "
+                                     f"from the single-view render in {(got != ref).sum()} bytes")
+        for k in (3 * 64 + 4, 7 * 64 + 2):  # the 1M-splat scene at 1280x720, the 4M one at 1920x1080
+            v = views[k]
+            ref = oracle.render(datas[v.scene], v.camera, taps=False)
+            np.testing.assert_array_equal(b.image(k), ref["rgba"])
+            print(f"case: batch view {k} (scene {v.scene}, {v.camera.width}x{v.camera.height}) equals the oracle",
+                  flush=True)
+    finally:
+        b.destroy()
+        r.destroy()
+        for s in scenes:
+            s.destroy()

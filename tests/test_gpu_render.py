@@ -137,6 +137,24 @@ def test_render_equirect_parity(renderer):
     assert_render_parity(renderer, data, S.equirect_camera(512, 256))
 
 
+def test_one_splat_after_a_large_frame_without_taps(device):
+    """Frames without taps after the first run speculatively: the cull writes
+    no ids and the depth sort's first pass makes them. A one-splat scene has
+    nothing to sort, so the sort must still write its id, or the frame
+    would draw the previous frame's splat 0 (regression)."""
+    r = Renderer(device)
+    try:
+        cam = S.default_camera(256, 160)
+        big = S.gaussian_cloud(50_000, seed=21)
+        np.testing.assert_array_equal(r.render(big, cam), oracle.render(big, cam, taps=False)["rgba"])
+        for data in (S.one_splat((0.2, -0.1, 0.0), (0.1, 0.9, 0.3)), S.one_splat((0, 0, 0)),
+                     S.gaussian_cloud(2, seed=22), S.empty_scene()):
+            got = r.render(data, cam)
+            np.testing.assert_array_equal(got, oracle.render(data, cam, taps=False)["rgba"])
+    finally:
+        r.destroy()
+
+
 def test_render_empty_scene(renderer):
     rgba = renderer.render(S.empty_scene(), S.default_camera(128, 128))
     img = rgba.reshape(128, 128, 4)

@@ -15,6 +15,9 @@ What each test fixes (a plausible slip in the oracle fails at least one):
   rectangle (including the row 0 / column 0 clamp quirk and maxCoverage
   truncation), and tile ranges against searchsorted;
 * chunked rendering is bit-identical to one chunk;
+* the sampled-row mode (only some tile rows emitted and rasterized) gives
+  those rows bit-identical to the full render, the full pair count, and
+  per-row loop counts that add up to the full render's;
 * the equirect screen mapping against its closed form in every direction
   (ahead, +-x, up, down, both sides of the seam), and the equirect path's
   reuse of the pinhole Jacobian (records equal to the pinhole projection's
@@ -380,3 +383,52 @@ def test_browser_chunk_boundary():
     a = oracle.render(data, cam, taps=False)["rgba"]
     b = oracle.render(data, cam, chunk_cap=50_000, taps=False)["rgba"]
     assert np.abs(a.astype(int) - b.astype(int)).max() <= 2
+
+
+@pytest.mark.parametrize("step,phase", [(1, 0), (3, 1), (5, 4), (16, 0)])
+def test_sampled_rows_equal_the_full_render(step, phase):
+    """orc_render_rows keeps only the pairs of every step-th tile row: the
+    stable pair sort keeps each tile's list order, and maxCoverage counts
+    the dropped pairs too, so the kept rows match the full render bit for
+    bit. Covers maxCoverage truncation and several reference chunks."""
+    data = S.gaussian_cloud(30_000, seed=9, pos_std=0.6, log_scale_range=(math.log(0.005), math.log(0.2)))
+    cam = S.default_camera(400, 300)
+    bg = BackgroundRGBA(0.2, 0.1, 0.3, 0.8)
+    for chunk, cov in ((0, 0), (7_000, 23)):
+        full = oracle.render(data, cam, background=bg, chunk_cap=chunk, max_coverage=cov, taps=True)
+        part = oracle.render_rows(data, cam, background=bg, row_step=step, row_phase=phase, chunk_cap=chunk,
+                                  max_coverage=cov)
+        assert part["num_pairs"] == full["num_pairs"] and part["num_visible"] == full["num_visible"]
+        a = full["rgba"].reshape(cam.height, cam.width, 4)
+        b = part["rgba"].reshape(cam.height, cam.width, 4)
+        ty = np.arange(cam.height) // 16
+        kept = (ty % step) == (phase % step)
+        np.testing.assert_array_equal(b[kept], a[kept])
+        # rows not kept show the background only
+        f = np.float32
+        bgv = np.array([0.2, 0.1, 0.3, (f(1) - f(1)) + f(1) * f(0.8)], np.float32)
+        bgpx = (bgv * f(255) + f(0.5)).astype(np.uint32).astype(np.uint8)  # finalize.ts:35-40
+        assert (b[~kept] == bgpx).all()
+        if step == 1:
+            assert int(part["row_iters"].sum()) == full["pixel_iters"]
+        else:
+            one = oracle.render_rows(data, cam, background=bg, row_step=1, chunk_cap=chunk, max_coverage=cov)
+            np.testing.assert_array_equal(part["row_iters"][part["tile_rows"]], one["row_iters"][part["tile_rows"]])
+            assert (np.delete(part["row_iters"], part["tile_rows"]) == 0).all()
+
+
+def test_chunk_over_u32_pairs_is_an_error_not_a_wrap():
+    """A chunk whose pair total needs more than 32 bits is refused, not
+    wrapped (the reference's pair buffers are indexed by u32): 270k splats
+    that each cover the whole 2048x2048 frame are 270k x 16384 > 2^32 pairs
+    in one chunk."""
+    n = 270_000
+    rng = np.random.default_rng(3)
+    pos = np.zeros((n, 3), np.float32)
+    pos[:, :2] = rng.uniform(-0.05, 0.05, (n, 2))
+    pos[:, 2] = 3.0
+    data = SplatData(n, pos.reshape(-1), np.tile([1, 0, 0, 0], n).astype(np.float32),
+                     np.full(3 * n, -0.5, np.float32), np.full(n, -6.0, np.float32), np.zeros(3 * n, np.float32))
+    cam = S.default_camera(2048, 2048)
+    with pytest.raises(RuntimeError, match=r"\(-2\)"):
+        oracle.render_rows(data, cam, row_step=64, chunk_cap=0xFFFFFFFF)
