@@ -98,8 +98,8 @@ def test_large_equirect_frames_match_oracle(device, large):
 
 @pytest.mark.parametrize("budget,opts", [
     (3000, None),
-    (20_000, RendererOptions(binning=1)),
-    (9000, RendererOptions(full_lists=1)),
+    (20_000, RendererOptions(binning="sort")),
+    (9000, RendererOptions(full_lists=True)),
     (5000, RendererOptions(chunk_cap=4000, max_coverage_per_splat=40)),
 ])
 def test_pair_budget_cuts_passes(device, budget, opts):
@@ -110,7 +110,7 @@ def test_pair_budget_cuts_passes(device, budget, opts):
     data = S.gaussian_cloud(40_000, seed=71, log_scale_range=(np.log(0.005), np.log(0.08)))
     cam = S.default_camera(480, 320)
     bg = BackgroundRGBA(0.3, 0.2, 0.1, 1.0)
-    mc = opts.max_coverage_per_splat if opts else 0
+    mc = (opts.max_coverage_per_splat or 0) if opts else 0
     ref = oracle.render(data, cam, background=bg, max_coverage=mc, taps=True)
     assert ref["num_pairs"] > 8 * budget
     r = Renderer(device, opts)
