@@ -133,6 +133,8 @@ SIGNATURES = [
     ("dl_batch_read_image", C.c_int, [C.c_void_p, C.c_uint32, u8p]),
     ("dl_batch_get_stats", C.c_int, [C.c_void_p, C.POINTER(DlBatchStats)]),
     ("dl_sort_pairs_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint32, C.c_uint32]),
+    ("dl_sort_pairs_u32_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("dl_sort_pairs_u32_noalloc_max", C.c_uint64, []),
     ("dl_scan_u32", C.c_int, [C.c_void_p, u32p, u32p, C.c_uint64, C.c_uint32, u32p]),
     ("dl_voxelize", C.c_int, [C.c_void_p, C.POINTER(DlSplats), C.POINTER(DlVoxelOpts), C.POINTER(C.c_void_p)]),
     ("dl_voxel_result_count", C.c_uint64, [C.c_void_p]),

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdroplet_hip.so")
-SOURCES = ["capi.hip", "scan_sort.hip", "render.hip", "binning.hip", "batch.hip", "backward.hip", "voxel.hip", "collision.hip", "ply.hip"]
+SOURCES = ["capi.hip", "scan_sort.hip", "render.hip", "binning.hip", "batch.hip", "backward.hip", "voxel.hip",
+           "collision.hip", "ply.hip", "noalloc.hip"]
 ARCH = os.environ.get("DROPLET_HIP_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -257,6 +257,8 @@ int dl_context_create(const dl_device_opts *opts, dl_context **out) {
     c->c.num_cus = prop.multiProcessorCount;
     c->c.arch = prop.gcnArchName;
     DL_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+    // the device address of the no-alloc sort's scratch (once per device)
+    noalloc_setup(dev);
     *out = c.release();
   });
 }
@@ -659,6 +661,18 @@ int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t 
     DL_HIP(hipStreamSynchronize(s));
   });
 }
+
+int dl_sort_pairs_u32_noalloc(dl_context *ctx, uint32_t *d_keys, uint32_t *d_vals, uint32_t n, uint32_t key_bits) {
+  return guard([&] {
+    if (!ctx) fail(DL_ERR_INVALID, "null context");
+    if (n == 0) return;
+    if (!d_keys || !d_vals) fail(DL_ERR_INVALID, "null keys/vals");
+    if (key_bits == 0 || key_bits > 32) fail(DL_ERR_INVALID, "key_bits must be 1..32");
+    noalloc_sort_pairs(ctx->c, d_keys, d_vals, n, (int)key_bits);
+  });
+}
+
+uint64_t dl_sort_pairs_u32_noalloc_max(void) { return noalloc_max_pairs(); }
 
 int dl_scan_u32(dl_context *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint32_t repeat, uint32_t *total) {
   return guard([&] {

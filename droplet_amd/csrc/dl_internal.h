@@ -131,6 +131,14 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
                           const uint32_t *d_seg_start, uint32_t nseg, size_t max_elems, int key_bits,
                           int max_digit_bits, void *ws);
 
+// The no-alloc sort (noalloc.hip): noalloc_setup looks up the device
+// address of its scratch array for `device` (current) once; the sort runs
+// in that scratch, in place on device arrays, and fails with
+// DL_ERR_CAPACITY above noalloc_max_pairs() pairs.
+void noalloc_setup(int device);
+uint64_t noalloc_max_pairs();
+void noalloc_sort_pairs(Context &c, uint32_t *d_keys, uint32_t *d_vals, size_t n, int key_bits);
+
 // off[b] = lower_bound(sorted keys, b) for b < num_bins and off[num_bins] = n
 // (all zeros when n == 0): the result of find-boundaries.ts:17-99.
 void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_bins, uint32_t *d_off);

@@ -289,6 +289,20 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
  * Host arrays in/out; only the low key_bits bits of each key are compared. */
 int dl_sort_pairs_u32(dl_context *ctx, uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t key_bits);
 
+/* ---- the same stable sort without allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). keys and vals are DEVICE arrays of n values, sorted in place
+ * on the context's stream; the call returns when they are sorted. Its
+ * temporary arrays live in a fixed scratch array of the library (about 10
+ * MB per GPU, held until the process ends), so n is at most
+ * dl_sort_pairs_u32_noalloc_max() (2^20); larger n fails with
+ * DL_ERR_CAPACITY. All calls on one GPU share that scratch: calls that
+ * overlap on one GPU (several threads, or contexts on several streams) race
+ * and must be serialised by the caller. */
+int dl_sort_pairs_u32_noalloc(dl_context *ctx, uint32_t *d_keys, uint32_t *d_vals, uint32_t n, uint32_t key_bits);
+uint64_t dl_sort_pairs_u32_noalloc_max(void);
+
 /* ---- exclusive prefix sum of u32 (wrapping) values: the device-wide
  * replacement for the CPU scan of the sort's histogram
  * (src/gpu/radix-sort.ts:172-187), and the scan under the renderer's
