@@ -61,7 +61,9 @@ class DlRenderStats(C.Structure):
     _fields_ = [("num_visible", C.c_uint32), ("num_chunks", C.c_uint32), ("num_tiles", C.c_uint32),
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("pair_key_bits", C.c_uint32),
                 ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
-                ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64)]
+                ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64),
+                ("binned_splats", C.c_uint64), ("row_spans", C.c_uint64), ("group_spans", C.c_uint64),
+                ("depth_key_bits", C.c_uint32)]
 
 
 class DlGrads(C.Structure):
@@ -92,6 +94,11 @@ class DlVoxelDeviceView(C.Structure):
     _fields_ = [("masks", C.c_void_p), ("num_blocks", C.c_uint64), ("nb", C.c_int32 * 3),
                 ("bounds_min", C.c_double * 3), ("voxel_resolution", C.c_double), ("block_idx", C.c_void_p),
                 ("mask_lo", C.c_void_p), ("mask_hi", C.c_void_p), ("count", C.c_uint64), ("device", C.c_int)]
+
+
+class DlVoxelStats(C.Structure):
+    _fields_ = [("device_ms", C.c_float), ("eval_ms", C.c_float), ("block_pairs", C.c_uint64),
+                ("grid_blocks", C.c_uint64)]
 
 
 f64p = C.POINTER(C.c_double)
@@ -141,6 +148,7 @@ SIGNATURES = [
     ("dl_voxel_result_blocks", C.c_int, [C.c_void_p, u32p, u32p, u32p]),
     ("dl_voxel_result_bounds", C.c_int, [C.c_void_p, C.c_double * 3, C.c_double * 3, C.c_double * 3]),
     ("dl_voxel_result_device_ms", C.c_float, [C.c_void_p]),
+    ("dl_voxel_result_stats", C.c_int, [C.c_void_p, C.POINTER(DlVoxelStats)]),
     ("dl_voxel_result_free", None, [C.c_void_p]),
     ("dl_voxel_result_device", C.c_int, [C.c_void_p, C.POINTER(DlVoxelDeviceView)]),
     ("dl_voxel_result_create", C.c_int, [C.c_void_p, C.c_double * 3, C.c_double * 3, C.c_double * 3, C.c_double,
@@ -154,6 +162,8 @@ SIGNATURES = [
     ("dl_voxel_mesh_device_ms", C.c_float, [C.c_void_p]),
     ("dl_voxel_mesh_free", None, [C.c_void_p]),
     ("dl_debug_math", C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, C.c_uint32]),
+    ("dl_debug_valu_probe", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
+    ("dl_context_synchronize", C.c_int, [C.c_void_p]),
 ]
 
 _lib = None

@@ -162,6 +162,27 @@ SceneView Scene::view() const {
 // ---------------------------------------------------------------------------
 // numerics self-check kernel
 // ---------------------------------------------------------------------------
+// A kernel of known VALU work, to calibrate the SQ instruction counters
+// against: each lane runs `iters` iterations of 16 f32 FMAs (8 independent
+// chains of 2), so a wave issues 16 * iters v_fma_f32 plus a few setup
+// instructions, and 2 * 16 * iters * 64 FP32 operations; the loop counter
+// is uniform, so the loop control is scalar.
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float *__restrict__ out) {
+  float a[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] = (float)(threadIdx.x + 7 * k) * 1e-3f;
+  const float m = 0.999f, c = 1e-4f;
+#pragma unroll 1
+  for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      a[k] = __builtin_fmaf(a[k], m, c);
+      a[k] = __builtin_fmaf(a[k], m, c);
+    }
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __global__ void k_debug_math(int op, const float *a, const float *b, float *out, uint32_t n) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -694,6 +715,33 @@ int dl_scan_u32(dl_context *ctx, const uint32_t *in, uint32_t *out, uint64_t n, 
     if (n) DL_HIP(hipMemcpyAsync(out, d_out.get(), bytes, hipMemcpyDeviceToHost, s));
     DL_HIP(hipMemcpyAsync(total, d_total.get(), 4, hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dl_debug_valu_probe(dl_context *ctx, uint32_t blocks, uint32_t iters, double *checksum) {
+  return guard([&] {
+    if (!ctx || blocks == 0) fail(DL_ERR_INVALID, "null context or no blocks");
+    Context &c = ctx->c;
+    c.make_current();
+    hipStream_t s = c.stream;
+    DeviceBuffer out;
+    out.reserve((size_t)blocks * 256 * 4);
+    k_valu_probe<<<blocks, 256, 0, s>>>(iters, out.as<float>());
+    DL_HIP(hipGetLastError());
+    std::vector<float> h((size_t)blocks * 256);
+    DL_HIP(hipMemcpyAsync(h.data(), out.get(), h.size() * 4, hipMemcpyDeviceToHost, s));
+    DL_HIP(hipStreamSynchronize(s));
+    double sum = 0.0;
+    for (float v : h) sum += v;
+    if (checksum) *checksum = sum;
+  });
+}
+
+int dl_context_synchronize(dl_context *ctx) {
+  return guard([&] {
+    if (!ctx) fail(DL_ERR_INVALID, "null context");
+    ctx->c.make_current();
+    DL_HIP(hipDeviceSynchronize());
   });
 }
 

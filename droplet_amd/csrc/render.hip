@@ -206,6 +206,18 @@ __global__ void k_add_const(uint32_t n, unsigned long long *__restrict__ acc) {
   if (threadIdx.x == 0) *acc += n;
 }
 
+// Counting frames: the pass's binned splats (live ones in a compacted pass,
+// else cn), row spans and group spans (span-binning scalars [0], [1]) added
+// to the frame's tallies.
+__global__ void k_bin_tally(const uint32_t *__restrict__ bin, const uint32_t *__restrict__ live_items, uint32_t cn,
+                            unsigned long long *__restrict__ tally) {
+  if (threadIdx.x == 0) {
+    tally[0] += live_items ? min(live_items[0], cn) : cn;
+    tally[1] += bin[0];
+    tally[2] += bin[1];
+  }
+}
+
 
 void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float4 *rec, const uint32_t *off,
                       const uint32_t *pvals, float4 *accum, uint32_t *out_rgba, const RastParams &p,
@@ -357,14 +369,19 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   small.reserve(256);
   // device scalars: [0] visible, [1] chunk pairs, [4..6] span-binning
   // scalars, [8..9] u64 total pairs, [10..15] u64 loop iterations and
-  // consumed list entries (counting frames), list entries built
+  // consumed list entries (counting frames), list entries built, [16..21]
+  // u64 binned splats, row spans and group spans (counting frames)
   uint32_t *d_nvis = small.as<uint32_t>();
   uint32_t *d_total = d_nvis + 1;
   uint32_t *d_bin = d_nvis + 4;
   unsigned long long *d_pairs_all = reinterpret_cast<unsigned long long *>(d_nvis + 8);
   unsigned long long *d_iters = reinterpret_cast<unsigned long long *>(d_nvis + 10);
   unsigned long long *d_entries = d_iters + 2;
-  if (counting) DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
+  unsigned long long *d_tally = reinterpret_cast<unsigned long long *>(d_nvis + 16);
+  if (counting) {
+    DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
+    DL_HIP(hipMemsetAsync(d_tally, 0, 3 * sizeof(unsigned long long), s));
+  }
   k_zero_u64<<<1, 64, 0, s>>>(d_entries);
   DL_HIP(hipGetLastError());
   row_iters_rows = 0;
@@ -449,6 +466,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       }
     }
     key_bits_used = km.kmin ? (uint32_t)sort_bits : 0u;
+    stats.depth_key_bits = (uint32_t)sort_bits;
     // a speculative frame needs no per-splat tile counts (span binning
     // counts its own) and lets the sort's first pass make the ids 0..n-1
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), ids.as<uint32_t>(), tkeys.as<uint32_t>(),
@@ -626,6 +644,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         mark(DL_STAGE_EMIT);
         span_bin(s, rec.as<float4>(), cids, cn, grid, npairs, live, want_rect8 ? rect8.as<uint2>() : nullptr, n,
                  bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin, d_entries, fused);
+        if (counting) {
+          const bool compacted = live.pre && want_rect8;
+          k_bin_tally<<<1, 64, 0, s>>>(d_bin, compacted ? bin_ws.live_small.as<uint32_t>() : nullptr, cn, d_tally);
+          DL_HIP(hipGetLastError());
+        }
         if (defer_color && !fused) {
           mark(DL_STAGE_PROJECT);
           launch_colors(s, sc, nullptr, bin_ws.rects.as<uint4>(), cn, bin_ws.live_small.as<uint32_t>(),
@@ -710,14 +733,17 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   }
   mark(-1);
   {
-    // small: [0] visible, [8..9] all pairs, [10..15] loop counters
-    uint32_t sm[16] = {};
+    // small: [0] visible, [8..9] all pairs, [10..15] loop counters, [16..21] tallies
+    uint32_t sm[22] = {};
     DL_HIP(hipMemcpyAsync(sm, d_nvis, sizeof(sm), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
     auto u64_at = [&](int k) { return (unsigned long long)sm[k] | ((unsigned long long)sm[k + 1] << 32); };
     if (counting) {
       stats.pixel_iters = u64_at(10);
       stats.pairs_consumed = u64_at(12);
+      stats.binned_splats = u64_at(16);
+      stats.row_spans = u64_at(18);
+      stats.group_spans = u64_at(20);
     }
     stats.list_entries = u64_at(14);
     if (n > 0) {

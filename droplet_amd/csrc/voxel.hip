@@ -530,14 +530,23 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     }
   }
   bin_ranges(s, skeys, npairs, total, ranges.as<uint32_t>());
+  hipEvent_t ev_a, ev_b;
+  DL_HIP(hipEventCreate(&ev_a));
+  DL_HIP(hipEventCreate(&ev_b));
+  DL_HIP(hipEventRecord(ev_a, s));
   k_vox_eval<<<total, 64, 0, s>>>(rec.as<VGauss>(), ranges.as<uint32_t>(), svals, G, masks.as<uint2>());
   DL_HIP(hipGetLastError());
+  DL_HIP(hipEventRecord(ev_b, s));
+  out.block_pairs = npairs;
 
   // reference insertion order + compaction of non-empty blocks
   voxel_compact(s, out, e1);
   DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
+  DL_HIP(hipEventElapsedTime(&out.eval_ms, ev_a, ev_b));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(ev_a);
+  (void)hipEventDestroy(ev_b);
 }
 
 }  // namespace dl
@@ -583,6 +592,17 @@ int dl_voxel_result_bounds(const dl_voxel_result *res, double bounds_min[3], dou
 }
 
 float dl_voxel_result_device_ms(const dl_voxel_result *res) { return res ? res->r.device_ms : 0.0f; }
+
+int dl_voxel_result_stats(const dl_voxel_result *res, dl_voxel_stats *out) {
+  return guard([&] {
+    if (!res || !out) fail(DL_ERR_INVALID, "null argument");
+    *out = dl_voxel_stats{};
+    out->device_ms = res->r.device_ms;
+    out->eval_ms = res->r.eval_ms;
+    out->block_pairs = res->r.block_pairs;
+    out->grid_blocks = res->r.total;
+  });
+}
 
 void dl_voxel_result_free(dl_voxel_result *res) { delete res; }
 

@@ -29,6 +29,8 @@ struct VoxelResult {
   mutable std::vector<uint32_t> h_idx, h_lo, h_hi;
   mutable bool host_ready = false;
   float device_ms = 0.0f;
+  float eval_ms = 0.0f;           // the per-voxel Gaussian sums (k_vox_eval)
+  uint64_t block_pairs = 0;       // (block, Gaussian) pairs the sums read
 };
 
 // Builds r.idx / r.lo / r.hi and r.count from r.masks (r.nb and r.total

@@ -63,6 +63,10 @@ class Device:
             raise RuntimeError("device destroyed")
         return self._h
 
+    def synchronize(self) -> None:
+        """Waits until all work of the device is done (dl_context_synchronize)."""
+        L.check(L.load().dl_context_synchronize(self.handle))
+
     def destroy(self) -> None:
         if self._h is not None:
             L.load().dl_context_destroy(self._h)
@@ -233,6 +237,8 @@ class Renderer:
             "stage_ms": {name: float(st.stage_ms[i]) for i, name in enumerate(L.STAGES)},
             "total_ms": float(st.total_ms), "pixel_iters": int(st.pixel_iters),
             "pairs_consumed": int(st.pairs_consumed), "list_entries": int(st.list_entries),
+            "binned_splats": int(st.binned_splats), "row_spans": int(st.row_spans),
+            "group_spans": int(st.group_spans), "depth_key_bits": int(st.depth_key_bits),
         }
 
     def forward(self, scene: "Scene", camera: RenderCamera,

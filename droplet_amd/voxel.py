@@ -178,6 +178,15 @@ class VoxelGrid:
     def device_ms(self) -> float:
         return float(self._require().dl_voxel_result_device_ms(self._h))
 
+    def stats(self) -> dict:
+        """Timing and sizes of the voxelize call that made this grid
+        (dl_voxel_result_stats): device_ms, eval_ms (the per-voxel sums),
+        block_pairs, grid_blocks."""
+        st = L.DlVoxelStats()
+        L.check(self._require().dl_voxel_result_stats(self._h, C.byref(st)))
+        return {"device_ms": float(st.device_ms), "eval_ms": float(st.eval_ms),
+                "block_pairs": int(st.block_pairs), "grid_blocks": int(st.grid_blocks)}
+
     def device_view(self) -> "L.DlVoxelDeviceView":
         v = L.DlVoxelDeviceView()
         L.check(self._require().dl_voxel_result_device(self._h, C.byref(v)))

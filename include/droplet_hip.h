@@ -173,6 +173,12 @@ typedef struct {
     uint64_t pixel_iters;        /* rasterize loop iterations over all pixels (counting enabled) */
     uint64_t pairs_consumed;     /* tile-list entries read before each tile saturated (counting enabled) */
     uint64_t list_entries;       /* tile-list entries built (<= num_pairs; fewer when closed tiles are skipped) */
+    /* span binning, counting enabled: splats binned over all passes (the live
+     * ones of a compacted pass), row spans and 16-column group spans built */
+    uint64_t binned_splats, row_spans, group_spans;
+    /* key bits of the frame's depth sort: 32, or the span bits of a
+     * speculative frame that sorted on key - (least key) */
+    uint32_t depth_key_bits;
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
 /* Counts rasterize loop iterations (the work unit of the rasterize roofline)
@@ -331,6 +337,18 @@ int dl_voxel_result_bounds(const dl_voxel_result *res, double bounds_min[3], dou
                            double num_blocks[3]);
 /* GPU time of the voxelizer's device stages of the last call, in ms */
 float dl_voxel_result_device_ms(const dl_voxel_result *res);
+/* Timing and sizes of the call that made a result (no reference
+ * counterpart): device_ms as above; eval_ms the per-voxel Gaussian sums
+ * alone (the kernel that replaces voxel/shaders/voxelize.ts:86-147);
+ * block_pairs the (block, Gaussian) pairs those sums read; grid_blocks the
+ * blocks of the grid. Zero for results of the collision stages. */
+typedef struct {
+    float device_ms;
+    float eval_ms;
+    uint64_t block_pairs;
+    uint64_t grid_blocks;
+} dl_voxel_stats;
+int dl_voxel_result_stats(const dl_voxel_result *res, dl_voxel_stats *out);
 void dl_voxel_result_free(dl_voxel_result *res);
 
 /* ---- the voxel grid in HBM and the collision stages after it.
@@ -406,6 +424,16 @@ void dl_voxel_mesh_free(dl_voxel_mesh *m);
  * op: 0 exp, 1 sqrt, 2 div (a/b), 3 f64->f32 depth key, 4 sigmoid,
  * 5 the rasterizer's exp for arguments in [-87, 0]. */
 int dl_debug_math(dl_context *ctx, int op, const float *a, const float *b, float *out, uint32_t n);
+
+/* ---- profiling aid: a kernel of known VALU work (k_valu_probe), blocks x
+ * 256 threads, each lane 16 * iters f32 FMAs, so that rocprofv3 counter
+ * runs can calibrate SQ_INSTS_VALU and its kin; *checksum (may be NULL) gets
+ * the sum of the lanes' results. No reference counterpart. */
+int dl_debug_valu_probe(dl_context *ctx, uint32_t blocks, uint32_t iters, double *checksum);
+
+/* Waits until all work of the context's device is done (hipDeviceSynchronize),
+ * including the renderer's second stream. */
+int dl_context_synchronize(dl_context *ctx);
 
 #ifdef __cplusplus
 }

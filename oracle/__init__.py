@@ -51,7 +51,7 @@ class OrcVoxelOpts(C.Structure):
 class OrcVoxelResult(C.Structure):
     _fields_ = [("num_blocks", C.c_uint64), ("bounds_min", C.c_double * 3), ("bounds_max", C.c_double * 3),
                 ("num_blocks_xyz", C.c_double * 3), ("num_batches", C.c_uint64), ("num_candidates", C.c_uint64),
-                ("contrib_pairs", C.c_uint64), ("num_near", C.c_uint64),
+                ("contrib_pairs", C.c_uint64), ("num_near", C.c_uint64), ("ref_tests", C.c_uint64),
                 ("block_idx", u32p), ("mask_lo", u32p), ("mask_hi", u32p), ("cap_blocks", C.c_uint64),
                 ("near_block", u32p), ("near_bit", u32p), ("near_alpha", f32p), ("cap_near", C.c_uint64),
                 ("near_eps", C.c_double)]
@@ -370,6 +370,7 @@ def voxelize(splats, voxel_resolution: float = 0.05, opacity_cutoff: float = 0.1
         "bounds_min": tuple(r.bounds_min), "bounds_max": tuple(r.bounds_max),
         "num_blocks_xyz": tuple(r.num_blocks_xyz), "num_batches": int(r.num_batches),
         "num_candidates": int(r.num_candidates), "contrib_pairs": int(r.contrib_pairs),
+        "ref_tests": int(r.ref_tests),
         "num_near": nn, "near_block": arrs["near_block"][:min(nn, near_cap)].copy(),
         "near_bit": arrs["near_bit"][:min(nn, near_cap)].copy(),
         "near_alpha": arrs["near_alpha"][:min(nn, near_cap)].copy(),

@@ -1028,6 +1028,7 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
     for (int k = 0; k < 3; k++) nbd[k] = js_round((bmx[k] - bmn[k]) / bs);
     for (int k = 0; k < 3; k++) { res->bounds_min[k] = bmn[k]; res->bounds_max[k] = bmx[k]; res->num_blocks_xyz[k] = nbd[k]; }
     res->num_blocks = 0; res->num_batches = 0; res->num_candidates = 0; res->contrib_pairs = 0; res->num_near = 0;
+    res->ref_tests = 0;
 
     vgauss_t *gs = (vgauss_t *)malloc(sizeof(vgauss_t) * (n ? n : 1));
     for (uint32_t i = 0; i < n; i++) {
@@ -1045,7 +1046,7 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
     uint32_t *scratch = (uint32_t *)malloc(sizeof(uint32_t) * MEGA);
     float resf = (float)vres, half = resf * 0.5f, cutf = (float)cutoff;
     int rc = 0;
-    uint64_t contrib_total = 0;
+    uint64_t contrib_total = 0, tests_total = 0;
 
     for (int64_t bz = 0; bz < nbz && rc == 0; bz += 16)
         for (int64_t by = 0; by < nby && rc == 0; by += 16)
@@ -1071,10 +1072,12 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
                 uint32_t total = bt.cx * bt.cy * bt.cz;
                 uint32_t *lo_arr = (uint32_t *)calloc(total, sizeof(uint32_t));
                 uint32_t *hi_arr = (uint32_t *)calloc(total, sizeof(uint32_t));
-                uint64_t contrib = 0;
-#pragma omp parallel num_threads(nthreads) reduction(+ : contrib)
+                uint64_t contrib = 0, tests = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : contrib, tests)
                 {
                     uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cnt);
+                    /* position of each kept candidate in the batch list */
+                    uint32_t *cpos = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)cnt);
 #pragma omp for schedule(dynamic, 4)
                     for (int64_t fb = 0; fb < (int64_t)total; fb++) {
                         uint32_t blk_x = (uint32_t)fb % bt.cx;
@@ -1102,7 +1105,7 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
                                 double margin = 1e-5 * (fabs(lo_c[k]) + fabs((double)g->p[k]) + reach + 1.0);
                                 if ((double)g->p[k] < lo_c[k] - reach - margin || (double)g->p[k] > hi_c[k] + reach + margin) keep = 0;
                             }
-                            if (keep) cand[nc++] = scratch[c];
+                            if (keep) { cpos[nc] = (uint32_t)c; cand[nc++] = scratch[c]; }
                         }
                         uint32_t mlo = 0, mhi = 0;
                         for (uint32_t vi = 0; vi < 64; vi++) {
@@ -1111,12 +1114,18 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
                             float vc[3];
                             for (int k = 0; k < 3; k++) vc[k] = (bminf[k] + bo[k]) + (((float)lp[k] + 0.5f) * resf);
                             float sigma = 0.0f;
+                            /* the reference tests every candidate of the
+                             * batch in list order until sigma >= 7
+                             * (voxel/shaders/voxelize.ts:115-131); the
+                             * dropped ones fail its AABB test */
+                            uint64_t voxel_tests = (uint64_t)cnt;
                             for (uint32_t c = 0; c < nc; c++) {
                                 int passed = 0;
                                 sigma = sigma + eval_gauss(vc, half, &gs[cand[c]], &passed);
                                 contrib += (uint64_t)passed;
-                                if (sigma >= 7.0f) break;
+                                if (sigma >= 7.0f) { voxel_tests = (uint64_t)cpos[c] + 1; break; }
                             }
+                            tests += voxel_tests;
                             float op = 1.0f - orc_expf(-sigma);
                             uint32_t lin = lp[2] * 16u + lp[1] * 4u + lp[0];
                             if (op >= cutf) {
@@ -1136,8 +1145,10 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
                         lo_arr[fb] = mlo; hi_arr[fb] = mhi;
                     }
                     free(cand);
+                    free(cpos);
                 }
                 contrib_total += contrib;
+                tests_total += tests;
                 /* mask decode + addBlock (voxelize.ts:160-179) */
                 for (uint32_t fb = 0; fb < total; fb++) {
                     if (lo_arr[fb] == 0 && hi_arr[fb] == 0) continue;
@@ -1153,6 +1164,7 @@ int orc_voxelize(const orc_splats *s, const orc_voxel_opts *opts, orc_voxel_resu
                 free(lo_arr); free(hi_arr);
             }
     res->contrib_pairs = contrib_total;
+    res->ref_tests = tests_total;
     free(scratch); free(gs); free(ext);
     bvh_free(&bvh);
     return rc;

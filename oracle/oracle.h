@@ -169,6 +169,8 @@ typedef struct {
     uint64_t num_candidates;     /* sum of batch candidate counts */
     uint64_t contrib_pairs;      /* (voxel, Gaussian) evaluations that passed the AABB test */
     uint64_t num_near;           /* voxels with |alpha - cutoff| <= near_eps */
+    uint64_t ref_tests;          /* (voxel, candidate) AABB tests the reference runs: each voxel tests its
+                                    batch's list in order until sigma >= 7 (contrib_pairs of them pass) */
     /* arrays (may be NULL; capacities in elements) */
     uint32_t *block_idx, *mask_lo, *mask_hi;   /* [num_blocks] insertion order */
     uint64_t cap_blocks;

@@ -187,3 +187,31 @@ def test_browser_sphere_voxelizes():
     assert r["block_idx"].size > 0
     # insertion order: block ids inside one batch ascend in (lz, ly, lx) order
     assert len(set(r["block_idx"].tolist())) == r["block_idx"].size
+
+
+def test_reference_test_count_known_answers():
+    """ref_tests counts the (voxel, candidate) AABB tests the reference runs
+    (each voxel walks its batch list until sigma >= 7). One Gaussian never
+    reaches 7, so every voxel of its one batch tests it once; with opaque
+    copies stacked on one point the centre voxels stop early, so the count
+    falls below voxels x candidates but stays above the passing tests."""
+    from droplet_amd.types import Bounds
+    one = SplatData(1, [0.0, 0.0, 0.0], [1, 0, 0, 0], [-3.0] * 3, [2.0], [0, 0, 0])
+    # a grid wider than the Gaussian's AABB: the outer voxels fail the test
+    r = oracle.voxelize(one, voxel_resolution=0.02, opacity_cutoff=0.1, bounds=Bounds((-0.32,) * 3, (0.32,) * 3))
+    nb = [int(v) for v in r["num_blocks_xyz"]]
+    assert nb == [8, 8, 8] and r["num_batches"] == 1
+    assert r["ref_tests"] == 64 * 8 ** 3
+    # the passing tests: voxels whose centre lies within ext + half of the centre per axis
+    ext, half = float(np.float32(3 * math.exp(-3.0))), 0.01
+    c = -0.32 + (np.arange(32) + 0.5) * 0.02
+    inside = int((np.abs(c) <= ext + half).sum())
+    assert r["contrib_pairs"] == inside ** 3 < r["ref_tests"]
+    k = 12
+    stack = SplatData(k, np.zeros(3 * k, np.float32), np.tile([1, 0, 0, 0], k).astype(np.float32),
+                      np.full(3 * k, -3.0, np.float32), np.full(k, 9.0, np.float32), np.zeros(3 * k, np.float32))
+    r = oracle.voxelize(stack, voxel_resolution=0.02, opacity_cutoff=0.1)
+    nb = [int(v) for v in r["num_blocks_xyz"]]
+    full = 64 * nb[0] * nb[1] * nb[2] * k
+    assert r["num_batches"] == 1 and r["num_candidates"] == k
+    assert r["contrib_pairs"] <= r["ref_tests"] < full
