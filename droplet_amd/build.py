@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdroplet_hip.so")
 SOURCES = ["capi.hip", "scan_sort.hip", "render.hip", "binning.hip", "batch.hip", "backward.hip", "voxel.hip",
-           "collision.hip", "ply.hip", "noalloc.hip"]
+           "collision.hip", "ply.hip", "noalloc.hip", "staging.hip"]
 ARCH = os.environ.get("DROPLET_HIP_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

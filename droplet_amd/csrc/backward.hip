@@ -558,7 +558,7 @@ void Renderer::forward_train(const SceneView &sc, uint64_t scene_serial, const d
   img_rgb.reserve(npx * 12);
   k_image_rgb<<<div_up(npx, 256), 256, 0, s>>>(accum.as<float4>(), npx, bg.r, bg.g, bg.b, img_rgb.as<float>());
   DL_HIP(hipGetLastError());
-  if (out_rgb) DL_HIP(hipMemcpyAsync(out_rgb, img_rgb.get(), npx * 12, hipMemcpyDeviceToHost, s));
+  if (out_rgb) ctx->staging.to_host(out_rgb, img_rgb.get(), npx * 12, s);
   DL_HIP(hipStreamSynchronize(s));
 }
 
@@ -573,7 +573,7 @@ void Renderer::backward(const float *grad_rgb, const dl_grads &out) {
   hipStream_t s = ctx->stream;
   ctx->make_current();
   grad_img.reserve(std::max<size_t>(npx, 1) * 12);
-  DL_HIP(hipMemcpyAsync(grad_img.get(), grad_rgb, npx * 12, hipMemcpyHostToDevice, s));
+  ctx->staging.to_device(grad_img.get(), grad_rgb, npx * 12, s);
   const size_t nn = std::max<uint32_t>(n, 1);
   gsplat.reserve(nn * kSplatGrads * 4);
   gparams.reserve(nn * (3 + 4 + 3 + 1 + 3 + 3 * (size_t)M) * 4 + 1024);
@@ -609,7 +609,7 @@ void Renderer::backward(const float *grad_rgb, const dl_grads &out) {
     DL_HIP(hipGetLastError());
   }
   auto down = [&](float *dst, const float *src, size_t count) {
-    if (dst && count) DL_HIP(hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToHost, s));
+    if (dst && count) ctx->staging.to_host(dst, src, count * 4, s);
   };
   down(out.positions, gpos, 3 * (size_t)n);
   down(out.rotations, grot, 4 * (size_t)n);

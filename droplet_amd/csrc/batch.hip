@@ -499,9 +499,8 @@ void Batch::run_groups(const std::vector<SceneView> &scenes, const dl_view *view
 void Batch::read_image(uint32_t view, uint8_t *out) const {
   if (view >= hviews.size()) fail(DL_ERR_INVALID, "view index out of range");
   const BView &b = hviews[view];
-  DL_HIP(hipMemcpyAsync(out, images.as<uint32_t>() + b.px0, (size_t)b.width * b.height * 4, hipMemcpyDeviceToHost,
-                        ctx->stream));
-  DL_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->make_current();
+  ctx->staging.to_host(out, images.as<uint32_t>() + b.px0, (size_t)b.width * b.height * 4, ctx->stream);
 }
 
 }  // namespace dl

@@ -95,14 +95,14 @@ bool scene_alive(uint64_t serial) {
   return g_live_scenes.count(serial) != 0;
 }
 
-static void upload_array(DeviceBuffer &b, const float *src, size_t count, const char *name, hipStream_t s) {
+static void upload_array(Context &c, DeviceBuffer &b, const float *src, size_t count, const char *name) {
   if (count == 0) {
     b.reserve(256);
     return;
   }
   if (!src) fail(DL_ERR_INVALID, std::string(name) + " is null");
   b.reserve(count * sizeof(float));
-  DL_HIP(hipMemcpyAsync(b.get(), src, count * sizeof(float), hipMemcpyHostToDevice, s));
+  c.staging.to_device(b.get(), src, count * sizeof(float), c.stream);
 }
 
 void Scene::upload(const dl_splats &sp) {
@@ -110,13 +110,12 @@ void Scene::upload(const dl_splats &sp) {
   const size_t n = sp.count;
   const uint32_t M = sp.sh_bands == 0 ? 0 : sp.sh_bands == 1 ? 3 : sp.sh_bands == 2 ? 8 : 15;
   ctx->make_current();
-  hipStream_t s = ctx->stream;
-  upload_array(pos, sp.positions, 3 * n, "positions", s);
-  upload_array(rot, sp.rotations, 4 * n, "rotations", s);
-  upload_array(ls, sp.log_scales, 3 * n, "logScales", s);
-  upload_array(op, sp.opacity_logits, n, "opacityLogits", s);
-  upload_array(dc, sp.colors_dc, 3 * n, "colorsDC", s);
-  upload_array(sh, sp.colors_sh, 3 * (size_t)M * n, "colorsSH", s);
+  upload_array(*ctx, pos, sp.positions, 3 * n, "positions");
+  upload_array(*ctx, rot, sp.rotations, 4 * n, "rotations");
+  upload_array(*ctx, ls, sp.log_scales, 3 * n, "logScales");
+  upload_array(*ctx, op, sp.opacity_logits, n, "opacityLogits");
+  upload_array(*ctx, dc, sp.colors_dc, 3 * n, "colorsDC");
+  upload_array(*ctx, sh, sp.colors_sh, 3 * (size_t)M * n, "colorsSH");
   count = sp.count;
   bands = sp.sh_bands;
   sh_coeffs = M;
@@ -384,7 +383,7 @@ int dl_scene_read(dl_scene *scene, float *positions, float *rotations, float *lo
     hipStream_t st = s.ctx->stream;
     const size_t n = s.count;
     auto get = [&](float *dst, const DeviceBuffer &b, size_t floats) {
-      if (dst && floats) DL_HIP(hipMemcpyAsync(dst, b.get(), floats * 4, hipMemcpyDeviceToHost, st));
+      if (dst && floats) s.ctx->staging.to_host(dst, b.get(), floats * 4, st);
     };
     get(positions, s.pos, 3 * n);
     get(rotations, s.rot, 4 * n);

@@ -67,6 +67,26 @@ class DeviceBuffer {
   size_t cap_ = 0;
 };
 
+// Copies between pageable host memory and the device through two pinned
+// chunks, with the host side spread over a few threads (staging.hip). Both
+// return when the caller's memory may be reused: to_device once the last
+// chunk was read by the DMA, to_host once the data is in dst (the stream is
+// synchronised).
+class HostStaging {
+ public:
+  HostStaging() = default;
+  ~HostStaging();
+  HostStaging(const HostStaging &) = delete;
+  HostStaging &operator=(const HostStaging &) = delete;
+  void to_device(void *dst, const void *src, size_t bytes, hipStream_t s);
+  void to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+ private:
+  void ensure();
+  void *buf_[2] = {nullptr, nullptr};
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -74,6 +94,8 @@ struct Context {
   std::string arch;
   // scratch for the standalone sort entry point
   DeviceBuffer sort_keys, sort_vals, sort_tmp_keys, sort_tmp_vals, sort_ws;
+  // pinned staging of large host <-> device copies
+  HostStaging staging;
   void make_current() const;
 };
 

@@ -463,7 +463,7 @@ void Scene::load_ply(const uint8_t *bytes, size_t len, bool flip_y) {
   op.reserve((size_t)n * 4);
   dc.reserve((size_t)n * 12);
   sh.reserve(std::max<size_t>((size_t)n * c.nsh * 4, 256));
-  DL_HIP(hipMemcpyAsync(body.get(), bytes + h.data_offset, body_bytes, hipMemcpyHostToDevice, s));
+  ctx->staging.to_device(body.get(), bytes + h.data_offset, body_bytes, s);
   if (!sh_cols.empty())
     DL_HIP(hipMemcpyAsync(cols.get(), sh_cols.data(), sh_cols.size() * 4, hipMemcpyHostToDevice, s));
   const uint32_t grid = div_up(n, c.spb);

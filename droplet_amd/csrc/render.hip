@@ -800,8 +800,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
 
 void Renderer::read_image(uint8_t *out) {
   if (!width) fail(DL_ERR_INVALID, "no image has been rendered yet");
-  DL_HIP(hipMemcpyAsync(out, rgba.get(), (size_t)width * height * 4, hipMemcpyDeviceToHost, ctx->stream));
-  DL_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->make_current();
+  ctx->staging.to_host(out, rgba.get(), (size_t)width * height * 4, ctx->stream);
 }
 
 Renderer::~Renderer() {

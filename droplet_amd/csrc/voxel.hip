@@ -416,10 +416,10 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     op.reserve((size_t)n * 4);
     rec.reserve((size_t)n * sizeof(VGauss));
     partials.reserve((size_t)nprep * 6 * sizeof(double));
-    DL_HIP(hipMemcpyAsync(pos.get(), sp.positions, (size_t)n * 12, hipMemcpyHostToDevice, s));
-    DL_HIP(hipMemcpyAsync(rot.get(), sp.rotations, (size_t)n * 16, hipMemcpyHostToDevice, s));
-    DL_HIP(hipMemcpyAsync(ls.get(), sp.log_scales, (size_t)n * 12, hipMemcpyHostToDevice, s));
-    DL_HIP(hipMemcpyAsync(op.get(), sp.opacity_logits, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    ctx.staging.to_device(pos.get(), sp.positions, (size_t)n * 12, s);
+    ctx.staging.to_device(rot.get(), sp.rotations, (size_t)n * 16, s);
+    ctx.staging.to_device(ls.get(), sp.log_scales, (size_t)n * 12, s);
+    ctx.staging.to_device(op.get(), sp.opacity_logits, (size_t)n * 4, s);
     DL_HIP(hipEventRecord(e0, s));
     k_vox_prep<<<nprep, kPrepBlock, 0, s>>>(pos.as<float>(), rot.as<float>(), ls.as<float>(), op.as<float>(), n,
                                            rec.as<VGauss>(), partials.as<double>());

@@ -195,9 +195,15 @@ class Renderer:
         self._last_size = (camera.width, camera.height)
         return out
 
-    def read_image(self) -> np.ndarray:
+    def read_image(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """RGBA8 of the last render; into `out` (W*H*4 uint8, contiguous) when
+        given, else a fresh array (the reference returns a fresh copy,
+        src/gpu/device.ts:89)."""
         w, h = self._last_size
-        out = np.empty(w * h * 4, np.uint8)
+        if out is None:
+            out = np.empty(w * h * 4, np.uint8)
+        elif out.dtype != np.uint8 or out.size != w * h * 4 or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous uint8 array of width*height*4 bytes")
         L.check(L.load().dl_renderer_read_image(self.handle, L.ptr(out, L.u8p)))
         return out
 

@@ -75,6 +75,11 @@ class SplatData:
         elif self.colors_sh is not None:
             self.colors_sh = np.ascontiguousarray(np.asarray(self.colors_sh, dtype=np.float32).reshape(-1))
 
+    def nbytes(self) -> int:
+        """Bytes of the arrays the renderer uploads (colorsSH up to 3*M*N)."""
+        m = num_sh_coeffs_per_channel(self.sh_bands)
+        return 4 * (14 * self.count + 3 * m * self.count)
+
 
 @dataclass
 class RenderCamera:
