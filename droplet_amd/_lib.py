@@ -54,7 +54,7 @@ class DlDeviceOpts(C.Structure):
 class DlRendererOpts(C.Structure):
     _fields_ = [("chunk_cap", C.c_uint32), ("max_coverage_per_splat", C.c_uint32), ("binning", C.c_uint32),
                 ("first_pass", C.c_uint32), ("pass_growth", C.c_uint32), ("full_lists", C.c_uint32),
-                ("sort_digit_bits", C.c_uint32)]
+                ("sort_digit_bits", C.c_uint32), ("tile_size", C.c_uint32), ("raster_group", C.c_uint32)]
 
 
 class DlRenderStats(C.Structure):
@@ -63,7 +63,8 @@ class DlRenderStats(C.Structure):
                 ("num_pairs", C.c_uint64), ("stage_ms", C.c_float * 8), ("total_ms", C.c_float),
                 ("pixel_iters", C.c_uint64), ("pairs_consumed", C.c_uint64), ("list_entries", C.c_uint64),
                 ("binned_splats", C.c_uint64), ("row_spans", C.c_uint64), ("group_spans", C.c_uint64),
-                ("depth_key_bits", C.c_uint32)]
+                ("depth_key_bits", C.c_uint32), ("tile_size", C.c_uint32), ("raster_group", C.c_uint32),
+                ("coverage_truncated", C.c_uint32)]
 
 
 class DlGrads(C.Structure):
@@ -130,6 +131,8 @@ SIGNATURES = [
     ("dl_render_forward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlBackground), f32p]),
     ("dl_render_backward", C.c_int, [C.c_void_p, f32p, C.POINTER(DlGrads)]),
     ("dl_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DlCamera), C.POINTER(DlTuneResult)]),
+    ("dl_renderer_tune_candidates", C.c_int, [C.POINTER(DlRendererOpts), C.c_uint64, C.POINTER(DlRendererOpts),
+                                              C.c_uint32, C.POINTER(C.c_uint32)]),
     ("dl_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DlView), C.c_uint32,
                                     C.POINTER(DlTuneResult)]),
     ("dl_batch_create", C.c_int, [C.c_void_p, C.POINTER(DlRendererOpts), C.POINTER(C.c_void_p)]),

@@ -55,12 +55,12 @@ __global__ __launch_bounds__(kItemBlock) void k_batch_cull(const BView *__restri
     const SceneView sc = scenes[V.scene];
     const ProjUniforms u = V.u;
     const size_t o = (size_t)V.item0 + i;
-    bool vis = false;
+    bool vis = false, cov16 = false;
     switch (sc.bands) {
-      case 0: cull_project_one<0>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count); break;
-      case 1: cull_project_one<1>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count); break;
-      case 2: cull_project_one<2>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count); break;
-      default: cull_project_one<3>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count); break;
+      case 0: cull_project_one<0>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
+      case 1: cull_project_one<1>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
+      case 2: cull_project_one<2>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
+      default: cull_project_one<3>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
     }
   }
   unsigned long long pc = count;

@@ -25,16 +25,14 @@ struct Rect {
 };
 
 // tile rectangle and maxCoverage truncation of one projected splat, with the
-// exact arithmetic of project.ts:223-233 / emit-pairs.ts:38-41
-__device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t tiles_y, uint32_t max_cov) {
+// exact arithmetic of project.ts:223-233 / emit-pairs.ts:38-41 at 16-px
+// tiles (tile_range gives the range at the autotuner's 8- and 32-px tiles)
+__device__ inline Rect splat_rect(const float4 v0, const TileGrid &g, uint32_t max_cov) {
   Rect R{0, 0, 0, 0, 0, 0};
   const float sx = v0.x, sy = v0.y, radius = v0.z;
   if (!(radius > 0.0f)) return R;
-  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
-  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
-  const uint32_t max_tx = u32sat(wmin((float)tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
-  const uint32_t max_ty = u32sat(wmin((float)tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
-  if (max_tx < min_tx || max_ty < min_ty) return R;
+  uint32_t min_tx, max_tx, min_ty, max_ty;
+  if (!tile_range(sx, sy, radius, g, min_tx, max_tx, min_ty, max_ty)) return R;
   const uint32_t w = max_tx - min_tx + 1u, h = max_ty - min_ty + 1u;
   R.x0 = min_tx;
   R.x1 = max_tx;
@@ -48,6 +46,11 @@ __device__ inline Rect splat_rect(const float4 v0, uint32_t tiles_x, uint32_t ti
   }
   R.nrows = R.full_rows + (R.rem > 0 ? 1u : 0u);
   return R;
+}
+
+// the grid of a 16-px tiles_x x tiles_y image
+__host__ __device__ inline TileGrid grid16(uint32_t tiles_x, uint32_t tiles_y) {
+  return TileGrid{kTile, tiles_x, tiles_y, tiles_x, tiles_y};
 }
 
 // Rect of a span-binnable grid (x < 256, y < 1024, rem < 256) in 8 bytes:
@@ -67,8 +70,11 @@ __device__ inline Rect unpack_rect8(const uint2 p) {
   return R;
 }
 
+// binning grid: tiles_x x tiles_y tiles of tg.tile pixels (tg's grid);
+// max_cov is maxCoverage in those tiles
 struct BinGrid {
   uint32_t tiles_x, tiles_y, max_cov;
+  TileGrid tg;
 };
 
 // Level 1 reads each splat as one 16-byte record: id, then the rect.

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_rects(const float4 *__restrict__ rec, c
   const uint32_t jj = blockIdx.x * 256 + threadIdx.x;
   if (jj >= cn) return;
   const uint32_t id = ids[jj];
-  rects[jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], g.tiles_x, g.tiles_y, g.max_cov));
+  rects[jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], g.tg, g.max_cov));
 }
 
 // Does row y (virtual) hold a live tile in columns [xa, xb]? Without a map
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ 
   const uint32_t jj = (blockIdx.x - v.blk0) * kB1 + threadIdx.x;
   if (jj >= v.n) return;
   const uint32_t id = ids[v.item0 + jj];
-  rects[v.item0 + jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], v.tiles_x, v.rows, v.max_cov));
+  rects[v.item0 + jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], grid16(v.tiles_x, v.rows), v.max_cov));
 }
 
 // ---------------------------- level 1 -------------------------------------

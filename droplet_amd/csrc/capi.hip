@@ -205,21 +205,25 @@ using namespace dl;
 // ---- autotuner --------------------------------------------------------------
 namespace {
 
-// Candidate options around `base`: radix digit widths 8 / 6 (4-bit digits,
-// eight passes, never won on MI355X) and pass schedules (full lists; first
-// pass n/32 .. n/4 with growth 4, 8 or 16; the default rule), n = the
-// largest scene.
-bool same_schedule(const dl_renderer_opts &a, const dl_renderer_opts &b) {
-  // digit width 0 means 8
-  const uint32_t da = a.sort_digit_bits ? a.sort_digit_bits : 8, db = b.sort_digit_bits ? b.sort_digit_bits : 8;
-  return da == db && a.first_pass == b.first_pass && a.pass_growth == b.pass_growth &&
-         (a.full_lists != 0) == (b.full_lists != 0);
+// The autotuner searches in two stages. Stage 1, around `base`: radix digit
+// widths 8 / 6 / 4 and pass schedules (full lists; first pass n/32 .. n/4
+// with growth 4, 8 or 16; the default rule), n = the largest scene. Stage 2,
+// around the stage-1 winner: tile sides 16 / 8 / 32 px and rasterizer
+// workgroups of one tile or 2x2 tiles (32-px tiles take one 4-wave
+// workgroup each). The batch runs stage 1 only, without 4-bit digits.
+uint32_t digits_of(const dl_renderer_opts &o) { return o.sort_digit_bits ? o.sort_digit_bits : 8; }
+uint32_t tile_of(const dl_renderer_opts &o) { return o.tile_size ? o.tile_size : 16; }
+uint32_t group_of(const dl_renderer_opts &o) { return tile_of(o) == 32 || o.raster_group != 2 ? 1 : 2; }
+
+bool same_options(const dl_renderer_opts &a, const dl_renderer_opts &b) {
+  return digits_of(a) == digits_of(b) && a.first_pass == b.first_pass && a.pass_growth == b.pass_growth &&
+         (a.full_lists != 0) == (b.full_lists != 0) && tile_of(a) == tile_of(b) && group_of(a) == group_of(b);
 }
 
-// Candidates other than base itself, which the tuner times first.
-std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint64_t n) {
+// Stage-1 candidates other than base itself, which the tuner times first.
+std::vector<dl_renderer_opts> schedule_candidates(const dl_renderer_opts &base, uint64_t n) {
   std::vector<dl_renderer_opts> out;
-  const uint32_t digits[] = {8, 6};
+  const uint32_t digits[] = {8, 6, 4};
   struct Sched {
     uint32_t div, growth, full;
   };
@@ -232,8 +236,21 @@ std::vector<dl_renderer_opts> tune_candidates(const dl_renderer_opts &base, uint
       o.full_lists = sc.full;
       o.first_pass = sc.div ? (uint32_t)std::max<uint64_t>(65536, n / sc.div) : 0;
       o.pass_growth = sc.growth;
-      if (!same_schedule(o, base)) out.push_back(o);
+      if (!same_options(o, base)) out.push_back(o);
     }
+  return out;
+}
+
+// Stage-2 candidates around base (base's shape excluded).
+std::vector<dl_renderer_opts> shape_candidates(const dl_renderer_opts &base) {
+  std::vector<dl_renderer_opts> out;
+  const uint32_t shapes[][2] = {{16, 1}, {16, 2}, {8, 1}, {8, 2}, {32, 1}};
+  for (const auto &sh : shapes) {
+    dl_renderer_opts o = base;
+    o.tile_size = sh[0];
+    o.raster_group = sh[1];
+    if (!same_options(o, base)) out.push_back(o);
+  }
   return out;
 }
 
@@ -526,10 +543,22 @@ int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera 
     res.default_ms = wall_ms_per_call(frame, kTuneReps);
     res.chosen = before;
     res.chosen_ms = res.default_ms;
-    for (const auto &o : tune_candidates(before, scene->s.count)) {
+    for (const auto &o : schedule_candidates(before, scene->s.count)) {
       R.opts = o;
       const float ms = wall_ms_per_call(frame, kTuneReps);
       res.candidates++;
+      if (ms < res.chosen_ms) {
+        res.chosen_ms = ms;
+        res.chosen = o;
+      }
+    }
+    for (const auto &o : shape_candidates(res.chosen)) {
+      R.opts = o;
+      const float ms = wall_ms_per_call(frame, kTuneReps);
+      res.candidates++;
+      // a shape that fell back to 16-px tiles (maxCoverage truncates a splat
+      // there) renders at 16 px whatever it asks: not a candidate
+      if (R.stats.tile_size != tile_of(o)) continue;
       if (ms < res.chosen_ms) {
         res.chosen_ms = ms;
         res.chosen = o;
@@ -541,6 +570,18 @@ int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera 
     R.counting = counting;
     restore.done = true;
     *out = res;
+  });
+}
+
+int dl_renderer_tune_candidates(const dl_renderer_opts *base, uint64_t num_splats, dl_renderer_opts *out,
+                                uint32_t cap, uint32_t *count) {
+  return guard([&] {
+    if (!base || !count) fail(DL_ERR_INVALID, "null argument");
+    std::vector<dl_renderer_opts> all = schedule_candidates(*base, num_splats);
+    for (const auto &o : shape_candidates(*base)) all.push_back(o);
+    *count = (uint32_t)all.size();
+    if (out)
+      for (size_t k = 0; k < all.size() && k < cap; k++) out[k] = all[k];
   });
 }
 
@@ -574,7 +615,7 @@ int dl_batch_autotune(dl_batch *b, dl_scene *const *scenes, uint32_t num_scenes,
     res.default_ms = wall_ms_per_call(call, 2);
     res.chosen = before;
     res.chosen_ms = res.default_ms;
-    for (const auto &o : tune_candidates(before, nmax)) {
+    for (const auto &o : schedule_candidates(before, nmax)) {
       if (o.sort_digit_bits == 4) continue;  // 8 passes over every view's keys: never faster here
       B.opts = o;
       B.layout.clear();  // a new schedule is a new plan (and graph)

@@ -103,6 +103,76 @@ __host__ __device__ inline float wmax(float a, float b) { return a > b ? a : b; 
 __host__ __device__ inline float wmin(float a, float b) { return a < b ? a : b; }
 __host__ __device__ inline float wclamp(float e, float lo, float hi) { return wmin(wmax(e, lo), hi); }
 
+// Tile grid of a frame: the tile side in pixels (16 as in the reference,
+// constants.ts:19, or the autotuner's 8 or 32), the grid at that size and
+// the 16-px grid of the same image.
+struct TileGrid {
+  uint32_t tile, tiles_x, tiles_y, t16x, t16y;
+};
+
+__host__ __device__ inline TileGrid tile_grid(uint32_t width, uint32_t height, uint32_t tile) {
+  return TileGrid{tile, (width + tile - 1) / tile, (height + tile - 1) / tile, (width + kTile - 1) / kTile,
+                  (height + kTile - 1) / kTile};
+}
+
+// The reference's 16-px tile range of a splat (centre sx, sy, radius r),
+// project.ts:223-233; empty when x1 < x0 or y1 < y0.
+__host__ __device__ inline void tile_range16(float sx, float sy, float r, uint32_t t16x, uint32_t t16y,
+                                             uint32_t &x0, uint32_t &x1, uint32_t &y0, uint32_t &y1) {
+  x0 = u32sat(wmax(0.0f, __builtin_floorf((sx - r) / 16.0f)));
+  y0 = u32sat(wmax(0.0f, __builtin_floorf((sy - r) / 16.0f)));
+  x1 = u32sat(wmin((float)t16x - 1.0f, __builtin_floorf((sx + r) / 16.0f)));
+  y1 = u32sat(wmin((float)t16y - 1.0f, __builtin_floorf((sy + r) / 16.0f)));
+}
+
+// Tile range of a splat at the grid's tile size (false: empty). Without
+// maxCoverage truncation every size lists the splat in each tile holding a
+// pixel that the reference blends it into:
+//  16 px: the reference's range;
+//  32 px: the 32-px tiles that hold a tile of the 16-px range (the
+//         rasterizer keeps each 16-px quadrant to the splats whose 16-px
+//         range holds it);
+//   8 px: the 8-px tiles of the 16-px range, narrowed to floor((s -+ r) / 8)
+//         when s -+ r and r are below 2^22 in magnitude. A pixel of the
+//         16-px range outside the narrower range has its centre at least
+//         1/2 beyond the rounded s -+ r, so at least 3/8 beyond the exact
+//         one (the rounding error is at most 1/8 there), more than half the
+//         float spacing at r (at most 1/8): the f32 difference px + 0.5 - s
+//         exceeds r in magnitude and the reference's box test
+//         (rasterize.ts:66-69) skips the splat at that pixel.
+__host__ __device__ inline bool tile_range(float sx, float sy, float r, const TileGrid &g, uint32_t &x0,
+                                           uint32_t &x1, uint32_t &y0, uint32_t &y1) {
+  uint32_t a0, a1, b0, b1;
+  tile_range16(sx, sy, r, g.t16x, g.t16y, a0, a1, b0, b1);
+  if (a1 < a0 || b1 < b0) return false;
+  if (g.tile == kTile) {
+    x0 = a0; x1 = a1; y0 = b0; y1 = b1;
+    return true;
+  }
+  if (g.tile == 32u) {
+    x0 = a0 >> 1; x1 = a1 >> 1; y0 = b0 >> 1; y1 = b1 >> 1;
+    return true;
+  }
+  x0 = 2u * a0;
+  y0 = 2u * b0;
+  x1 = 2u * a1 + 1u < g.tiles_x ? 2u * a1 + 1u : g.tiles_x - 1u;
+  y1 = 2u * b1 + 1u < g.tiles_y ? 2u * b1 + 1u : g.tiles_y - 1u;
+  const float ex = sx - r, fx = sx + r, ey = sy - r, fy = sy + r;
+  const float lim = 4194304.0f;  // 2^22
+  if (__builtin_fabsf(ex) < lim && __builtin_fabsf(fx) < lim && __builtin_fabsf(ey) < lim &&
+      __builtin_fabsf(fy) < lim && r < lim) {
+    const uint32_t nx0 = u32sat(wmax(0.0f, __builtin_floorf(ex / 8.0f)));
+    const uint32_t ny0 = u32sat(wmax(0.0f, __builtin_floorf(ey / 8.0f)));
+    const uint32_t nx1 = u32sat(__builtin_floorf(fx / 8.0f));
+    const uint32_t ny1 = u32sat(__builtin_floorf(fy / 8.0f));
+    x0 = x0 > nx0 ? x0 : nx0;
+    y0 = y0 > ny0 ? y0 : ny0;
+    x1 = x1 < nx1 ? x1 : nx1;
+    y1 = y1 < ny1 ? y1 : ny1;
+  }
+  return x0 <= x1 && y0 <= y1;
+}
+
 // floatToSortable (src/spatial/radix-sort.ts:20-24)
 __host__ __device__ inline uint32_t sortable_key(float f) {
   uint32_t u = f32_bits(f);

@@ -34,13 +34,26 @@ struct SceneView {
 
 // Project uniforms (pipeline.ts:475-506): every value as the f32 the reference
 // writes into its uniform buffer.
+// tg is the tile grid the tile counts and rects use (16-px tiles as in the
+// reference, or the autotuner's 8 or 32; tiles_x / tiles_y are its size);
+// max_cov is maxCoverage in those tiles; max_cov16 is maxCoverage itself,
+// against which every splat's 16-px range is checked so that a frame at
+// another tile size can tell whether the reference would truncate a splat.
 struct ProjUniforms {
   float view[16];
   float width, height, focal, cx, cy, near_, far_;
   uint32_t tiles_x, tiles_y, max_cov, proj;
+  TileGrid tg;
+  uint32_t max_cov16;
 };
 
-ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov);
+ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov, uint32_t tile = kTile);
+// maxCoverage in tiles of `tile` pixels that truncates no splat whose 16-px
+// rect holds at most max_cov tiles (8-px rects are at most 4x as large;
+// 32-px rects are never larger than the 16-px ones)
+inline uint32_t max_cov_for_tile(uint32_t max_cov, uint32_t tile) {
+  return tile == 8 ? (uint32_t)std::min<uint64_t>(4ull * max_cov, 0xFFFFFFFFull) : max_cov;
+}
 
 // Default schedule of the front-to-back passes: a first pass of
 // max(kPassMinFirst, visible / kPassFirstDiv) splats, each later pass
@@ -69,6 +82,8 @@ struct RastParams {
   uint32_t *cost_key;
   // counting frames: loop iterations per tile row (null: not recorded)
   unsigned long long *row_iters;
+  // tile rows, and the 16-px grid (the 32-px rasterizer's quadrant filter)
+  uint32_t tiles_y, t16x, t16y;
 };
 
 // True while the scene with this serial number exists (scenes register
@@ -160,8 +175,15 @@ struct Renderer {
   // The sort that turns one frame's keys into the next frame's order runs on
   // a second stream (default priority) while pass 2 bins, and the next
   // frame's first pass waits for it (order_done).
+  // With 2x2 groups or 32-px tiles the order and keys are per launch unit.
   DeviceBuffer tile_key, tile_key_tmp, tile_order, tile_order_tmp, tile_sort_ws;
-  uint32_t tile_key_tiles = 0;  // tile count of the recorded keys (0: none)
+  uint32_t tile_key_tiles = 0;  // launch units of the recorded keys (0: none)
+  uint32_t tile_key_shape = 0;  // their tile size * 4 + group
+  // Tile size and rasterizer group (opts.tile_size, opts.raster_group).
+  // A frame at 8 or 32 px that finds a splat truncated by maxCoverage at
+  // 16-px tiles renders again at 16 px (force_tile16), and last_cov16 starts
+  // the following frames at 16 px until a frame finds none.
+  bool force_tile16 = false, last_cov16 = false;
   hipStream_t order_stream = nullptr;
   hipEvent_t keys_done = nullptr, order_done = nullptr;
   const uint32_t *next_order = nullptr;  // set once an ordering sort was launched
@@ -193,6 +215,7 @@ struct Renderer {
   void read_image(uint8_t *out);
 
  private:
+  void rerender_at16(const SceneView &sc, const dl_camera &cam, const dl_background &bg);
   void mark(int stage);
   void ensure_events(size_t n);
 };

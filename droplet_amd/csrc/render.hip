@@ -37,12 +37,14 @@ constexpr int kProjBlock = 256;
 // One thread per splat in id order, so every scene array is read coalesced.
 // Per-block partial sums of visible splats and tile counts replace any
 // same-address atomics.
-// Per-block partials: visible splats, tile pairs, and the least and greatest
-// depth key of the visible splats (0xFFFFFFFF / 0 when none).
+// Per-block partials: visible splats, tile pairs, the least and greatest
+// depth key of the visible splats (0xFFFFFFFF / 0 when none), and whether a
+// splat's 16-px tile range exceeds maxCoverage (the reference truncates its
+// pairs, so only 16-px tiles give its image).
 struct CullParts {
   uint32_t *vis;
   unsigned long long *pairs;
-  uint32_t *kmin, *kmax;
+  uint32_t *kmin, *kmax, *cov;
 };
 
 template <int BANDS, bool COLOR>
@@ -52,16 +54,18 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
                                                              uint32_t *__restrict__ counts,
                                                              uint2 *__restrict__ rect8, CullParts parts) {
   __shared__ uint32_t s_vis[kProjBlock / 64], s_kmin[kProjBlock / 64], s_kmax[kProjBlock / 64];
+  __shared__ uint32_t s_cov[kProjBlock / 64];
   __shared__ unsigned long long s_pairs[kProjBlock / 64];
   const uint32_t i = blockIdx.x * kProjBlock + threadIdx.x;
-  bool vis = false;
+  bool vis = false, cov16 = false;
   uint32_t count = 0;
   uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
   if (i < sc.count) {
-    cull_project_one<BANDS, COLOR>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count);
+    cull_project_one<BANDS, COLOR>(sc, u, near_plane, far_plane, i, i, i, keys, ids, rec, counts, vis, count,
+                                   cov16);
     if (vis) kmin = kmax = keys[i];
     // packed tile rect for the live-splat compaction of span binning
-    if (rect8) rect8[i] = pack_rect8(splat_rect(rec[3 * (size_t)i], u.tiles_x, u.tiles_y, u.max_cov));
+    if (rect8) rect8[i] = pack_rect8(splat_rect(rec[3 * (size_t)i], u.tg, u.max_cov));
   }
   uint32_t v = vis ? 1u : 0u;
   unsigned long long pc = count;
@@ -72,15 +76,17 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
     kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, d, 64));
     kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, d, 64));
   }
+  const bool any_cov = __ballot(cov16) != 0;
   if (__lane_id() == 0) {
     s_vis[threadIdx.x >> 6] = v;
     s_pairs[threadIdx.x >> 6] = pc;
     s_kmin[threadIdx.x >> 6] = kmin;
     s_kmax[threadIdx.x >> 6] = kmax;
+    s_cov[threadIdx.x >> 6] = any_cov ? 1u : 0u;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t tv = 0, tmin = 0xFFFFFFFFu, tmax = 0u;
+    uint32_t tv = 0, tmin = 0xFFFFFFFFu, tmax = 0u, tcov = 0u;
     unsigned long long tp = 0;
 #pragma unroll
     for (int w = 0; w < kProjBlock / 64; w++) {
@@ -88,23 +94,26 @@ __global__ __launch_bounds__(kProjBlock) void k_cull_project(SceneView sc, ProjU
       tp += s_pairs[w];
       tmin = min(tmin, s_kmin[w]);
       tmax = max(tmax, s_kmax[w]);
+      tcov |= s_cov[w];
     }
     parts.vis[blockIdx.x] = tv;
     parts.pairs[blockIdx.x] = tp;
     parts.kmin[blockIdx.x] = tmin;
     parts.kmax[blockIdx.x] = tmax;
+    parts.cov[blockIdx.x] = tcov;
   }
 }
 
 // Frame totals from the cull's partials: d_nvis[0] visible splats,
-// d_nvis[2] / d_nvis[3] the least / greatest visible depth key, *d_pairs the
-// tile pairs.
+// d_nvis[2] / d_nvis[3] the least / greatest visible depth key, d_nvis[7]
+// whether maxCoverage truncates a splat at 16-px tiles, *d_pairs the tile
+// pairs.
 __global__ __launch_bounds__(1024) void k_reduce_totals(CullParts parts, uint32_t nparts,
                                                         uint32_t *__restrict__ d_nvis,
                                                         unsigned long long *__restrict__ d_pairs) {
-  __shared__ uint32_t s_v[16], s_lo[16], s_hi[16];
+  __shared__ uint32_t s_v[16], s_lo[16], s_hi[16], s_c[16];
   __shared__ unsigned long long s_p[16];
-  uint32_t v = 0, lo = 0xFFFFFFFFu, hi = 0u;
+  uint32_t v = 0, lo = 0xFFFFFFFFu, hi = 0u, c = 0u;
   unsigned long long p = 0;
   // independent iterations, unrolled so that their loads are in flight together
 #pragma unroll 8
@@ -113,6 +122,7 @@ __global__ __launch_bounds__(1024) void k_reduce_totals(CullParts parts, uint32_
     p += parts.pairs[k];
     lo = min(lo, parts.kmin[k]);
     hi = max(hi, parts.kmax[k]);
+    c |= parts.cov[k];
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
@@ -120,26 +130,30 @@ __global__ __launch_bounds__(1024) void k_reduce_totals(CullParts parts, uint32_
     p += __shfl_xor(p, d, 64);
     lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
     hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
+    c |= (uint32_t)__shfl_xor((int)c, d, 64);
   }
   if (__lane_id() == 0) {
     s_v[threadIdx.x >> 6] = v;
     s_p[threadIdx.x >> 6] = p;
     s_lo[threadIdx.x >> 6] = lo;
     s_hi[threadIdx.x >> 6] = hi;
+    s_c[threadIdx.x >> 6] = c;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t tv = 0, tlo = 0xFFFFFFFFu, thi = 0u;
+    uint32_t tv = 0, tlo = 0xFFFFFFFFu, thi = 0u, tc = 0u;
     unsigned long long tp = 0;
     for (int w = 0; w < 16; w++) {
       tv += s_v[w];
       tp += s_p[w];
       tlo = min(tlo, s_lo[w]);
       thi = max(thi, s_hi[w]);
+      tc |= s_c[w];
     }
     d_nvis[0] = tv;
     d_nvis[2] = tlo;
     d_nvis[3] = thi;
+    d_nvis[7] = tc;
     *d_pairs = tp;
   }
 }
@@ -156,8 +170,8 @@ __global__ __launch_bounds__(256) void k_gather_counts(const uint32_t *__restric
 // emit (emit-pairs.ts:27-56); values are splat ids
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
-                                              const uint32_t *__restrict__ offs, uint32_t cn, uint32_t tiles_x,
-                                              uint32_t tiles_y, uint32_t max_cov, uint32_t *__restrict__ pkeys,
+                                              const uint32_t *__restrict__ offs, uint32_t cn, TileGrid tg,
+                                              uint32_t max_cov, uint32_t *__restrict__ pkeys,
                                               uint32_t *__restrict__ pvals) {
   const uint32_t jj = blockIdx.x * 128 + threadIdx.x;
   if (jj >= cn) return;
@@ -165,11 +179,9 @@ __global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, co
   const float4 v0 = rec[3 * (size_t)id];
   const float sx = v0.x, sy = v0.y, radius = v0.z;
   if (radius <= 0.0f) return;
-  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
-  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
-  const uint32_t max_tx = u32sat(wmin((float)tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
-  const uint32_t max_ty = u32sat(wmin((float)tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
-  if (max_tx < min_tx || max_ty < min_ty) return;
+  uint32_t min_tx, max_tx, min_ty, max_ty;
+  if (!tile_range(sx, sy, radius, tg, min_tx, max_tx, min_ty, max_ty)) return;
+  const uint32_t tiles_x = tg.tiles_x;
   uint32_t off = offs[jj];
   uint32_t written = 0;
   for (uint32_t ty = min_ty; ty <= max_ty; ty++) {
@@ -198,6 +210,87 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                         lds);
 }
 
+// The autotuner's other rasterizer shapes (single-view frames without
+// counting): tiles of 8 or 16 px in 2x2 groups per 256-thread workgroup, 8-px
+// tiles one wave each, and 32-px tiles whose four 16-px quadrants take a wave
+// each. A group or a 32-px tile is one launch unit: p.order lists units and
+// p.cost_key records one key per unit.
+template <uint32_t TILE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize_2x2(
+    const float4 *__restrict__ rec, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pvals,
+    float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba, RastParams p) {
+  __shared__ RastLds lds[4];
+  __shared__ uint32_t s_work[4];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t unit = p.order ? p.order[blockIdx.x] : blockIdx.x;
+  const uint32_t ux = (p.tiles_x + 1) / 2;
+  const uint32_t tx = 2 * (unit % ux) + (w & 1), ty = 2 * (unit / ux) + (w >> 1);
+  uint32_t work = 0;
+  if (tx < p.tiles_x && ty < p.tiles_y) {
+    const uint32_t tile = ty * p.tiles_x + tx;
+    const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
+    if constexpr (TILE == 16)
+      work = rasterize_tile16<false, true>(tx, ty, tile, start, end, rec, pvals, accum, out_rgba, p, nullptr, lds[w]);
+    else
+      work = rasterize_tile8<true>(tx, ty, tile, start, end, rec, pvals, accum, out_rgba, p, lds[w]);
+  }
+  if (p.cost_key) {
+    if ((threadIdx.x & 63) == 0) s_work[w] = work;
+    __syncthreads();
+    if (threadIdx.x == 0) p.cost_key[unit] = work_key(s_work[0] + s_work[1] + s_work[2] + s_work[3]);
+  }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize8(
+    const float4 *__restrict__ rec, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pvals,
+    float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba, RastParams p) {
+  __shared__ RastLds lds;
+  const uint32_t tile = p.order ? p.order[blockIdx.x] : blockIdx.x;
+  const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
+  const uint32_t work =
+      rasterize_tile8<false>(tile % p.tiles_x, tile / p.tiles_x, tile, start, end, rec, pvals, accum, out_rgba, p, lds);
+  if (p.cost_key && threadIdx.x == 0) p.cost_key[tile] = work_key(work);
+}
+
+// One 32-px tile per workgroup: wave w rasterizes 16-px quadrant (w & 1,
+// w >> 1) from the tile's list, keeping to the splats whose 16-px range
+// holds the quadrant; the tile closes when all four quadrants have.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize32(
+    const float4 *__restrict__ rec, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pvals,
+    float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba, RastParams p) {
+  __shared__ RastLds lds[4];
+  __shared__ uint32_t s_open[4], s_work[4];
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t tile = p.order ? p.order[blockIdx.x] : blockIdx.x;
+  const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
+  if (tile_skipped(p, tile, start, end)) {  // the same for every wave of the tile
+    if (p.cost_key && threadIdx.x == 0) p.cost_key[tile] = work_key(0);
+    return;
+  }
+  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+  const uint32_t qx = 2 * tx + (w & 1), qy = 2 * ty + (w >> 1);
+  Px16 s;
+  px16_load(s, qx * kTile, qy * kTile, accum, p);
+  uint32_t iters = 0, consumed = 0;
+  const uint32_t work =
+      px16_run<false, true, true>(s, qx * kTile, qy * kTile, start, end, rec, pvals, p, lds[w], qx, qy, iters, consumed);
+  bool closing = false;
+  if (p.closed && !p.finalize) {
+    const bool open = px16_open(s);
+    if (lane == 0) s_open[w] = open ? 1u : 0u;
+    __syncthreads();
+    const bool any = (s_open[0] | s_open[1] | s_open[2] | s_open[3]) != 0u;
+    if (threadIdx.x == 0) p.closed[tile] = any ? 0 : 1;
+    closing = !any && p.final_on_close;
+  }
+  px16_store(s, p, accum, out_rgba, p.write_accum && !closing, p.finalize || closing);
+  if (p.cost_key) {
+    if (lane == 0) s_work[w] = work;
+    __syncthreads();
+    if (threadIdx.x == 0) p.cost_key[tile] = work_key(s_work[0] + s_work[1] + s_work[2] + s_work[3]);
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long *__restrict__ p) {
   if (threadIdx.x == 0) *p = 0;
 }
@@ -219,13 +312,27 @@ __global__ void k_bin_tally(const uint32_t *__restrict__ bin, const uint32_t *__
 }
 
 
-void launch_rasterize(bool count, uint32_t num_tiles, hipStream_t s, const float4 *rec, const uint32_t *off,
-                      const uint32_t *pvals, float4 *accum, uint32_t *out_rgba, const RastParams &p,
-                      unsigned long long *d_iters) {
-  if (count)
-    k_rasterize<true><<<num_tiles, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
-  else
-    k_rasterize<false><<<num_tiles, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
+// Rasterizer launch for tiles of `tile` px in groups of group x group tiles
+// per workgroup (32-px tiles: one per workgroup); units = workgroups.
+void launch_rasterize(bool count, uint32_t tile, uint32_t group, uint32_t units, hipStream_t s, const float4 *rec,
+                      const uint32_t *off, const uint32_t *pvals, float4 *accum, uint32_t *out_rgba,
+                      const RastParams &p, unsigned long long *d_iters) {
+  if (tile == kTile && group == 1) {
+    if (count)
+      k_rasterize<true><<<units, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
+    else
+      k_rasterize<false><<<units, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p, d_iters);
+  } else {
+    if (count) fail(DL_ERR_INTERNAL, "counting frames rasterize 16-px tiles one per workgroup");
+    if (tile == 32)
+      k_rasterize32<<<units, 256, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
+    else if (group == 2 && tile == kTile)
+      k_rasterize_2x2<16><<<units, 256, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
+    else if (group == 2)
+      k_rasterize_2x2<8><<<units, 256, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
+    else
+      k_rasterize8<<<units, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
+  }
   DL_HIP(hipGetLastError());
 }
 
@@ -256,7 +363,7 @@ __global__ __launch_bounds__(256) void k_rects_colors(SceneView sc, const uint32
   const float4 v0 = rec[3 * (size_t)id];
   float *slot = reinterpret_cast<float *>(rec + 3 * (size_t)id + 2);
   const float pcx = slot[0], pcy = slot[1], pcz = slot[2];
-  rects[j] = pack_rect(id, splat_rect(v0, g.tiles_x, g.tiles_y, g.max_cov));
+  rects[j] = pack_rect(id, splat_rect(v0, g.tg, g.max_cov));
   float col[3];
   splat_color<BANDS>(sc, id, pcx, pcy, pcz, col);
   slot[0] = col[0];
@@ -323,7 +430,7 @@ static std::vector<T> to_host(const T *dev, size_t count, hipStream_t s) {
 
 // pipeline.ts:475-506: the project uniforms as the f32 values the
 // reference writes (focal from the f64 fovY)
-ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov) {
+ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov, uint32_t tile) {
   ProjUniforms u{};
   std::memcpy(u.view, cam.view_matrix, sizeof(u.view));
   u.width = (float)cam.width;
@@ -333,10 +440,12 @@ ProjUniforms make_uniforms(const dl_camera &cam, uint32_t max_cov) {
   u.cy = (float)((double)cam.height / 2.0);
   u.near_ = (float)cam.near_plane;
   u.far_ = (float)cam.far_plane;
-  u.tiles_x = div_up(cam.width, kTile);
-  u.tiles_y = div_up(cam.height, kTile);
-  u.max_cov = max_cov;
+  u.tg = tile_grid(cam.width, cam.height, tile);
+  u.tiles_x = u.tg.tiles_x;
+  u.tiles_y = u.tg.tiles_y;
+  u.max_cov = max_cov_for_tile(max_cov, tile);
   u.proj = cam.projection;
+  u.max_cov16 = max_cov;
   return u;
 }
 
@@ -353,22 +462,40 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   ev_stage.clear();
   for (auto &t : taps) t.clear();
 
+  if (opts.tile_size != 0 && opts.tile_size != 8 && opts.tile_size != 16 && opts.tile_size != 32)
+    fail(DL_ERR_INVALID, "tile_size must be 0, 8, 16 or 32");
+  if (opts.raster_group > 2) fail(DL_ERR_INVALID, "raster_group must be 0, 1 or 2");
+
   const uint32_t n = sc.count;
   const uint32_t W = cam.width, H = cam.height;
-  const uint32_t tiles_x = div_up(W, kTile), tiles_y = div_up(H, kTile);
+  // tile size and rasterizer group of this frame: 16 px, one wave per tile
+  // where the lists or counts must be the reference's
+  uint32_t T = opts.tile_size ? opts.tile_size : kTile;
+  uint32_t G = opts.raster_group == 2 ? 2u : 1u;
+  if (keep_taps || train_mode || counting || force_tile16 || last_cov16) T = kTile;
+  if (keep_taps || train_mode || counting) G = 1;
+  if (T == 32) G = 1;
+  const TileGrid tg = tile_grid(W, H, T);
+  const uint32_t tiles_x = tg.tiles_x, tiles_y = tg.tiles_y;
   const uint32_t num_tiles = tiles_x * tiles_y;
+  // workgroups of the rasterizer launch: tiles, or 2x2 groups of them
+  const uint32_t units = G == 2 ? div_up(tiles_x, 2) * div_up(tiles_y, 2) : num_tiles;
+  const uint32_t shape = T * 4 + G;
   const size_t npx = (size_t)W * H;
   stats = dl_render_stats{};
   stats.num_tiles = num_tiles;
   stats.tiles_x = tiles_x;
   stats.tiles_y = tiles_y;
+  stats.tile_size = T;
+  stats.raster_group = G;
   width = W;
   height = H;
 
   rgba.reserve(npx * 4);
   small.reserve(256);
-  // device scalars: [0] visible, [1] chunk pairs, [4..6] span-binning
-  // scalars, [8..9] u64 total pairs, [10..15] u64 loop iterations and
+  // device scalars: [0] visible, [1] chunk pairs, [2..3] least / greatest
+  // visible depth key, [4..6] span-binning scalars, [7] maxCoverage
+  // truncation at 16-px tiles, [8..9] u64 total pairs, [10..15] u64 loop iterations and
   // consumed list entries (counting frames), list entries built, [16..21]
   // u64 binned splats, row spans and group spans (counting frames)
   uint32_t *d_nvis = small.as<uint32_t>();
@@ -400,8 +527,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rp.bg_b = bg.b;
   rp.bg_a = bg.a;
   rp.row_iters = counting ? row_iters.as<unsigned long long>() : nullptr;
+  rp.tiles_y = tiles_y;
+  rp.t16x = tg.t16x;
+  rp.t16y = tg.t16y;
 
-  const ProjUniforms u = make_uniforms(cam, max_cov);
+  const ProjUniforms u = make_uniforms(cam, max_cov, T);
 
   // ---- cull + project (splat order), depth sort ----
   uint32_t nvis = 0;
@@ -432,13 +562,14 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     if (want_rect8) rect8.reserve((size_t)n * 8);
     counts.reserve((size_t)n * 4);
     const size_t pstride = ((size_t)nparts * 8 + 255) & ~(size_t)255;
-    parts.reserve(4 * pstride);
+    parts.reserve(5 * pstride);
     sort_ws.reserve(radix_workspace_bytes(n, 32));
-    CullParts cp;
+    CullParts cp{};
     cp.pairs = parts.as<unsigned long long>();
     cp.vis = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + pstride);
     cp.kmin = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 2 * pstride);
     cp.kmax = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 3 * pstride);
+    cp.cov = reinterpret_cast<uint32_t *>(parts.as<uint8_t>() + 4 * pstride);
     mark(DL_STAGE_CULL);
     cull_project_for(sc.bands, !defer_color)<<<nparts, kProjBlock, 0, s>>>(sc, u, cam.near_plane, cam.far_plane,
                                                              keys.as<uint32_t>(),
@@ -480,9 +611,18 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       nvis = n;
       pairs_all = spec_cap;
     } else {
-      DL_HIP(hipMemcpyAsync(&nvis, d_nvis, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      uint32_t head[8];
+      DL_HIP(hipMemcpyAsync(head, d_nvis, sizeof(head), hipMemcpyDeviceToHost, s));
       DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
       DL_HIP(hipStreamSynchronize(s));
+      nvis = head[0];
+      if (T != kTile && head[7]) {
+        // maxCoverage truncates a splat at 16-px tiles: only they give the
+        // reference's image
+        last_cov16 = true;
+        rerender_at16(sc, cam, bg);
+        return;
+      }
     }
     // a frame above the pair budget is cut into passes that each fit (below)
     if (pairs_all > budget() && train_mode)
@@ -518,7 +658,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     rp.write_accum = need_accum;
     if (need_accum) accum.reserve(npx * 16);
     mark(DL_STAGE_RASTERIZE);
-    launch_rasterize(counting, num_tiles, s, nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp,
+    launch_rasterize(counting, T, G, units, s, nullptr, nullptr, nullptr, accum.as<float4>(), rgba.as<uint32_t>(), rp,
                      d_iters);
   } else {
     const bool span = opts.binning == 0 && span_binning_supported(tiles_x, tiles_y);
@@ -623,7 +763,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         if (last) {
           rp.has_pairs = 0;
           mark(DL_STAGE_RASTERIZE);
-          launch_rasterize(counting, num_tiles, s, nullptr, nullptr, nullptr, accum.as<float4>(),
+          launch_rasterize(counting, T, G, units, s, nullptr, nullptr, nullptr, accum.as<float4>(),
                            rgba.as<uint32_t>(), rp, d_iters);
         }
         continue;
@@ -633,7 +773,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       if (span) {
         // stable two-level bucketing straight into per-tile lists (binning.hip)
         const LiveMap live = (track_closed && c > 0) ? LiveMap{open_pre.as<uint16_t>(), tiles_x + 1} : LiveMap{};
-        const BinGrid grid{tiles_x, tiles_y, max_cov};
+        const BinGrid grid{tiles_x, tiles_y, u.max_cov, u.tg};
         // first pass: its level-1 records and colours in one kernel
         const bool fused = defer_color && c == 0;
         if (fused) {
@@ -660,8 +800,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         ptvals.reserve((size_t)npairs * 4);
         psort_ws.reserve(radix_workspace_bytes(npairs, key_bits));
         mark(DL_STAGE_EMIT);
-        k_emit<<<div_up(cn, 128), 128, 0, s>>>(rec.as<float4>(), cids, offsets.as<uint32_t>(), cn, tiles_x, tiles_y,
-                                               max_cov, pkeys.as<uint32_t>(), pvals.as<uint32_t>());
+        k_emit<<<div_up(cn, 128), 128, 0, s>>>(rec.as<float4>(), cids, offsets.as<uint32_t>(), cn, u.tg, u.max_cov,
+                                               pkeys.as<uint32_t>(), pvals.as<uint32_t>());
         DL_HIP(hipGetLastError());
         mark(DL_STAGE_PAIR_SORT);
         const bool in_tmp = radix_sort_pairs(s, pkeys.as<uint32_t>(), pvals.as<uint32_t>(), ptkeys.as<uint32_t>(),
@@ -690,18 +830,18 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       train_vals = svals;
       mark(DL_STAGE_RASTERIZE);
       RastParams rpc = rp;
-      const bool ordered = c == 0 && num_tiles >= kOrderMinTiles;
+      const bool ordered = c == 0 && units >= kOrderMinTiles;
       if (ordered) {
-        if (next_order && tile_key_tiles == num_tiles) {
+        if (next_order && tile_key_tiles == units && tile_key_shape == shape) {
           // the order sorted from the last frame's keys; the wait also keeps
           // this launch from overwriting the keys before that sort read them
           DL_HIP(hipStreamWaitEvent(s, order_done, 0));
           rpc.order = next_order;
         }
-        tile_key.reserve((size_t)num_tiles * 4);
+        tile_key.reserve((size_t)units * 4);
         rpc.cost_key = tile_key.as<uint32_t>();
       }
-      launch_rasterize(counting, num_tiles, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
+      launch_rasterize(counting, T, G, units, s, rec.as<float4>(), ranges.as<uint32_t>(), svals, accum.as<float4>(),
                        rgba.as<uint32_t>(), rpc, d_iters);
       if (ordered) {
         // sort this frame's keys into the next frame's order on the second
@@ -711,18 +851,19 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
           DL_HIP(hipEventCreateWithFlags(&keys_done, hipEventDisableTiming));
           DL_HIP(hipEventCreateWithFlags(&order_done, hipEventDisableTiming));
         }
-        tile_key_tmp.reserve((size_t)num_tiles * 4);
-        tile_order.reserve((size_t)num_tiles * 4);
-        tile_order_tmp.reserve((size_t)num_tiles * 4);
-        tile_sort_ws.reserve(radix_workspace_bytes(num_tiles, 6));
+        tile_key_tmp.reserve((size_t)units * 4);
+        tile_order.reserve((size_t)units * 4);
+        tile_order_tmp.reserve((size_t)units * 4);
+        tile_sort_ws.reserve(radix_workspace_bytes(units, 6));
         DL_HIP(hipEventRecord(keys_done, s));
         DL_HIP(hipStreamWaitEvent(order_stream, keys_done, 0));
         const bool in_tmp = radix_sort_pairs(order_stream, tile_key.as<uint32_t>(), tile_order.as<uint32_t>(),
-                                             tile_key_tmp.as<uint32_t>(), tile_order_tmp.as<uint32_t>(), num_tiles, 6,
+                                             tile_key_tmp.as<uint32_t>(), tile_order_tmp.as<uint32_t>(), units, 6,
                                              tile_sort_ws.get(), 6, RadixKeyMap{}, true);
         DL_HIP(hipEventRecord(order_done, order_stream));
         next_order = in_tmp ? tile_order_tmp.as<uint32_t>() : tile_order.as<uint32_t>();
-        tile_key_tiles = num_tiles;
+        tile_key_tiles = units;
+        tile_key_shape = shape;
       }
       accum_valid = true;
       if (track_closed && !last) {
@@ -733,7 +874,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   }
   mark(-1);
   {
-    // small: [0] visible, [8..9] all pairs, [10..15] loop counters, [16..21] tallies
+    // small: [0] visible, [2..3] key span, [7] coverage flag, [8..9] all
+    // pairs, [10..15] loop counters, [16..21] tallies
     uint32_t sm[22] = {};
     DL_HIP(hipMemcpyAsync(sm, d_nvis, sizeof(sm), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
@@ -766,6 +908,19 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         if (!keys_fit) key_headroom = std::min<uint32_t>(2, key_headroom + 1);
         spec_key_bits = std::min<uint32_t>(32, need + key_headroom);
       }
+      stats.coverage_truncated = sm[7] ? 1u : 0u;
+      if (T != kTile && sm[7]) {
+        last_cov16 = true;
+        rerender_at16(sc, cam, bg);
+        return;
+      }
+      // an exact re-render below keeps this frame's tile size; the next
+      // frame starts from what this one found
+      struct SetCov {
+        bool &f;
+        bool v;
+        ~SetCov() { f = v; }
+      } set_cov{last_cov16, sm[7] != 0};
       if (speculative) {
         stats.num_visible = sm[0];
         stats.num_pairs = all;
@@ -796,6 +951,16 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       stats.total_ms += ms;
     }
   }
+}
+
+void Renderer::rerender_at16(const SceneView &sc, const dl_camera &cam, const dl_background &bg) {
+  struct Reset {
+    bool &f;
+    ~Reset() { f = false; }
+  } reset{force_tile16};
+  force_tile16 = true;
+  render(sc, cam, bg);
+  stats.coverage_truncated = 1;
 }
 
 void Renderer::read_image(uint8_t *out) {

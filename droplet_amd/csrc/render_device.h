@@ -116,7 +116,8 @@ __device__ inline void splat_color(const SceneView &sc, uint32_t id, float pcx, 
 // later for the splats that reach a tile list.
 template <int BANDS, bool COLOR = true>
 __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const ProjUniforms &u, float4 &o0,
-                                         float4 &o1, float4 &o2) {
+                                         float4 &o1, float4 &o2, bool &cov16) {
+  cov16 = false;
   const float *m = u.view;
   o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
   o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -232,15 +233,16 @@ __device__ inline uint32_t project_splat(const SceneView &sc, uint32_t id, const
   o0 = make_float4(sx, sy, radius, depth);
   o1 = make_float4(conic_a, conic_b, conic_c, alpha);
   o2 = make_float4(col[0], col[1], col[2], tau);
-  const uint32_t min_tx = u32sat(wmax(0.0f, __builtin_floorf((sx - radius) / 16.0f)));
-  const uint32_t min_ty = u32sat(wmax(0.0f, __builtin_floorf((sy - radius) / 16.0f)));
-  const uint32_t max_tx = u32sat(wmin((float)u.tiles_x - 1.0f, __builtin_floorf((sx + radius) / 16.0f)));
-  const uint32_t max_ty = u32sat(wmin((float)u.tiles_y - 1.0f, __builtin_floorf((sy + radius) / 16.0f)));
-  uint32_t count = 0;
-  if (max_tx >= min_tx && max_ty >= min_ty) {
+  // tile coverage (project.ts:223-233), at the grid's tile size
+  uint32_t min_tx, max_tx, min_ty, max_ty, count = 0;
+  if (tile_range(sx, sy, radius, u.tg, min_tx, max_tx, min_ty, max_ty)) {
     count = (max_tx - min_tx + 1u) * (max_ty - min_ty + 1u);
     if (count > u.max_cov) count = u.max_cov;
   }
+  // would the reference (16-px tiles) truncate this splat's pairs?
+  tile_range16(sx, sy, radius, u.tg.t16x, u.tg.t16y, min_tx, max_tx, min_ty, max_ty);
+  cov16 = max_tx >= min_tx && max_ty >= min_ty &&
+          (uint64_t)(max_tx - min_tx + 1u) * (max_ty - min_ty + 1u) > u.max_cov16;
   return count;
 }
 
@@ -253,7 +255,7 @@ __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms 
                                         double far_plane, uint32_t i, size_t o, uint32_t val,
                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                         float4 *__restrict__ rec, uint32_t *__restrict__ counts, bool &vis,
-                                        uint32_t &count) {
+                                        uint32_t &count, bool &cov16) {
   const float *m = u.view;
   const double x = sc.pos[3 * (size_t)i], y = sc.pos[3 * (size_t)i + 1], z = sc.pos[3 * (size_t)i + 2];
   float depth_f;
@@ -283,7 +285,8 @@ __device__ inline void cull_project_one(const SceneView &sc, const ProjUniforms 
   if (vals) vals[o] = val;
   float4 o0 = make_float4(0.0f, 0.0f, -1.0f, 0.0f), o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), o2 = o1;
   count = 0;
-  if (vis) count = project_splat<BANDS, COLOR>(sc, i, u, o0, o1, o2);
+  cov16 = false;
+  if (vis) count = project_splat<BANDS, COLOR>(sc, i, u, o0, o1, o2, cov16);
   rec[3 * o + 0] = o0;
   rec[3 * o + 1] = o1;
   rec[3 * o + 2] = o2;
@@ -397,7 +400,211 @@ struct RastLds {
   float acc[16 * 64];
 };
 
-// The work of one wave64 on one tile: tile (tx, ty) of the view, whose
+// Synchronisation of a staging batch. A one-wave workgroup takes the
+// workgroup barrier. In a workgroup of several waves each wave stages its own
+// batches and leaves its loop on its own pixels, so only the wave's own lanes
+// must see the staged records: the same fences without the s_barrier.
+template <bool MULTI>
+__device__ inline void stage_sync() {
+  if constexpr (MULTI) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
+// A tile that a pass skips: closed (and final, or not the last pass), or
+// without entries in a non-final pass while the accumulator holds its state.
+__device__ inline bool tile_skipped(const RastParams &p, uint32_t vtile, uint32_t start, uint32_t end) {
+  return p.closed && p.resume && ((p.closed[vtile] && (p.final_on_close || !p.finalize)) || (!p.finalize && start == end));
+}
+
+// The pixels of one wave over a 16x16 region at (ox, oy): lane l owns column
+// l % 16 and rows l / 16 + 4k (strip k = 0..3). Pixels outside the image
+// start with T = 0, so they are never live and never written.
+struct Px16 {
+  float cr[4], cg[4], cb[4], T[4], pyf[4];
+  float pxf;
+  uint32_t px, py0;
+  bool inb[4];
+};
+
+__device__ inline void px16_load(Px16 &s, uint32_t ox, uint32_t oy, const float4 *__restrict__ accum,
+                                 const RastParams &p) {
+  const int lane = threadIdx.x & 63;
+  s.px = ox + (lane & 15);
+  s.py0 = oy + (lane >> 4);
+  s.pxf = (float)s.px + 0.5f;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t y = s.py0 + 4 * k;
+    s.inb[k] = s.px < p.width && y < p.height;
+    s.pyf[k] = (float)y + 0.5f;
+    s.cr[k] = 0.0f; s.cg[k] = 0.0f; s.cb[k] = 0.0f; s.T[k] = s.inb[k] ? 1.0f : 0.0f;
+    if (s.inb[k] && p.resume) {
+      const float4 a = accum[(size_t)y * p.width + s.px];
+      s.cr[k] = a.x; s.cg[k] = a.y; s.cb[k] = a.z; s.T[k] = a.w;
+    }
+  }
+}
+
+__device__ inline bool px16_open(const Px16 &s) {
+  return __ballot(!(s.T[0] < kMinT) || !(s.T[1] < kMinT) || !(s.T[2] < kMinT) || !(s.T[3] < kMinT)) != 0;
+}
+
+__device__ inline void px16_store(const Px16 &s, const RastParams &p, float4 *__restrict__ accum,
+                                  uint32_t *__restrict__ out_rgba, bool write_accum, bool finalize) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!s.inb[k]) continue;
+    const size_t pix = (size_t)(s.py0 + 4 * k) * p.width + s.px;
+    if (write_accum) accum[pix] = make_float4(s.cr[k], s.cg[k], s.cb[k], s.T[k]);
+    if (finalize) out_rgba[pix] = pack_rgba(s.cr[k], s.cg[k], s.cb[k], s.T[k], p);
+  }
+}
+
+// The reference's per-pixel loop body (rasterize.ts:63-78) for one splat.
+__device__ inline void blend_splat(const float4 a, const float4 b, const float4 col, float pxf, float pyf,
+                                   float &cr, float &cg, float &cb, float &T) {
+  const float dx = pxf - a.x;
+  const float dy = pyf - a.y;
+  const float r = a.z;
+  if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) return;
+  const float power = gauss_power(b.x, b.y, b.z, dx, dy);
+  if (power > 0.0f || power < col.w) return;
+  // min(cap, opacity * max(0, exp(power) - floor)) (rasterize.ts:73)
+  // without the max: the record's opacity is sigmoid * fade * aa, so
+  // >= 0 or NaN, and for exp - floor <= 0 the product is <= 0 and
+  // fails the 1/255 test below exactly like 0 does (NaN stays NaN)
+  const float alpha = wmin(kOpacityCap, b.w * (dl_expf_nonpos(power) - kGaussFloor));
+  if (alpha < kMinAlpha) return;
+  const float w = T * alpha;
+  cr = __builtin_fmaf(w, col.x, cr);
+  cg = __builtin_fmaf(w, col.y, cg);
+  cb = __builtin_fmaf(w, col.z, cb);
+  T = T * (1.0f - alpha);
+}
+
+// Front to back over list entries [start, end) of pvals (which index rec)
+// for one wave's 16x16 region at (ox, oy). The splats are staged 32 at a
+// time in LDS; the strips each staged splat can reach are four 32-bit
+// ballots, and the wave visits only the splats with a bit set. Every pixel
+// runs exactly the reference's loop body in the reference's order; a pixel
+// stops when T < 1e-4 and the wave stops when all its pixels have. QUAD: the
+// region is the 16-px tile (q16x, q16y) of a 32-px tile's list, and only
+// the splats whose 16-px range holds that tile reach it (the reference's
+// list of that tile). Returns the staged batches x 8 plus the visited
+// (splat, strip) bits; consumed counts the entries read before the stop.
+template <bool COUNT, bool MULTI, bool QUAD>
+__device__ inline uint32_t px16_run(Px16 &s, uint32_t ox, uint32_t oy, uint32_t start, uint32_t end,
+                                    const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
+                                    const RastParams &p, RastLds &L, uint32_t q16x, uint32_t q16y,
+                                    uint32_t &iters, uint32_t &consumed) {
+  uint32_t work = 0;
+  consumed = 0;
+  if (start >= end) return 0;
+  const int lane = threadIdx.x & 63;
+  const double tile_x0 = (double)ox + 0.5, tile_x1 = tile_x0 + 15.0;
+  uint32_t stop = end;
+  for (uint32_t base = start; base < end; base += kStage) {
+    uint32_t strips_live = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) strips_live |= (__ballot(!(s.T[k] < kMinT)) != 0 ? 1u : 0u) << k;
+    if (strips_live == 0) {
+      stop = base;
+      break;
+    }
+    // Staging: the accumulators wait in LDS while the f64 strip tests run,
+    // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
+    // splats, two lanes per splat with two strips each, so that the LDS
+    // of a wave stays under 6 KB. Both bounds together allow 6 waves per
+    // SIMD.
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      L.acc[(4 * k + 0) * 64 + lane] = s.cr[k];
+      L.acc[(4 * k + 1) * 64 + lane] = s.cg[k];
+      L.acc[(4 * k + 2) * 64 + lane] = s.cb[k];
+      L.acc[(4 * k + 3) * 64 + lane] = s.T[k];
+    }
+    const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
+    const uint32_t i = base + slot;
+    uint32_t m = 0;
+    if (i < end) {
+      const uint32_t j = pvals[i];
+      const float4 a = rec[3 * (size_t)j + 0];
+      const float4 b = rec[3 * (size_t)j + 1];
+      const float4 col = rec[3 * (size_t)j + 2];
+      if (half == 0) {
+        L.rec[3 * slot + 0] = a;
+        L.rec[3 * slot + 1] = b;
+        L.rec[3 * slot + 2] = col;
+      }
+      bool listed = true;
+      if constexpr (QUAD) {
+        uint32_t x0, x1, y0, y1;
+        tile_range16(a.x, a.y, a.z, p.t16x, p.t16y, x0, x1, y0, y1);
+        listed = q16x >= x0 && q16x <= x1 && q16y >= y0 && q16y <= y1;
+      }
+      // one strip at a time (unrolled, the f64 temporaries of several
+      // strips take more VGPRs)
+      if (listed) {
+#pragma unroll 1
+        for (uint32_t q = 0; q < 2; q++) {
+          const uint32_t k = 2 * half + q;
+          const double y0 = (double)(oy + 4 * k) + 0.5;
+          if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+        }
+      }
+    }
+    // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
+    // strips 0 and 1, lanes 32-63 strips 2 and 3)
+    const uint64_t b02 = __ballot((m & 0x5u) != 0u), b13 = __ballot((m & 0xAu) != 0u);
+    const uint32_t s0 = (strips_live & 1u) ? (uint32_t)b02 : 0u;
+    const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
+    const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
+    const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
+    work += 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1) + __builtin_popcount(s2) +
+                            __builtin_popcount(s3));
+    stage_sync<MULTI>();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      s.cr[k] = L.acc[(4 * k + 0) * 64 + lane];
+      s.cg[k] = L.acc[(4 * k + 1) * 64 + lane];
+      s.cb[k] = L.acc[(4 * k + 2) * 64 + lane];
+      s.T[k] = L.acc[(4 * k + 3) * 64 + lane];
+    }
+    const uint32_t cnt = min(kStage, end - base);
+    // COUNT visits every staged splat to count the reference's loop
+    // iterations; otherwise only the splats with a strip bit
+    uint32_t todo = COUNT ? (cnt == 32u ? ~0u : (1u << cnt) - 1u) : (s0 | s1 | s2 | s3);
+    while (todo) {
+      const uint32_t c = (uint32_t)__builtin_ctz(todo);
+      todo &= todo - 1u;
+      if (COUNT)
+        iters += (uint32_t)!(s.T[0] < kMinT) + (uint32_t)!(s.T[1] < kMinT) + (uint32_t)!(s.T[2] < kMinT) +
+                 (uint32_t)!(s.T[3] < kMinT);
+      const uint32_t mm = ((s0 >> c) & 1u) | (((s1 >> c) & 1u) << 1) | (((s2 >> c) & 1u) << 2) |
+                          (((s3 >> c) & 1u) << 3);
+      if (mm == 0) continue;
+      const float4 a = L.rec[3 * c + 0];
+      const float4 b = L.rec[3 * c + 1];
+      const float4 col = L.rec[3 * c + 2];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (!((mm >> k) & 1u)) continue;
+        if (s.T[k] < kMinT) continue;
+        blend_splat(a, b, col, s.pxf, s.pyf[k], s.cr[k], s.cg[k], s.cb[k], s.T[k]);
+      }
+    }
+    stage_sync<MULTI>();
+  }
+  consumed = stop - start;
+  return work;
+}
+
+// The work of one wave64 on one 16-px tile: tile (tx, ty) of the view, whose
 // accum / out_rgba are the view's own row-major pixels, with list entries
 // [start, end) of pvals (which index rec); vtile
 // indexes the open/closed flags (p.closed). A tile is closed once all its
@@ -406,67 +613,84 @@ struct RastLds {
 // accumulator) and every later pass skips it; otherwise later passes skip it
 // until the final one, which finalizes it from the accumulator. A tile
 // without entries in a non-final pass is skipped too, as long as the
-// accumulator already holds its state.
+// accumulator already holds its state. Returns the tile's work measure
+// (0 when skipped).
+template <bool COUNT, bool MULTI>
+__device__ inline uint32_t rasterize_tile16(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
+                                            const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
+                                            float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
+                                            const RastParams &p, unsigned long long *__restrict__ d_iters,
+                                            RastLds &L) {
+  if (tile_skipped(p, vtile, start, end)) return 0;
+  const int lane = threadIdx.x & 63;
+  Px16 s;
+  px16_load(s, tx * kTile, ty * kTile, accum, p);
+  uint32_t iters = 0, consumed = 0;
+  const uint32_t work =
+      px16_run<COUNT, MULTI, false>(s, tx * kTile, ty * kTile, start, end, rec, pvals, p, L, 0, 0, iters, consumed);
+  bool closing = false;
+  if (p.closed && !p.finalize) {
+    const bool open = px16_open(s);
+    if (lane == 0) p.closed[vtile] = open ? 0 : 1;
+    closing = !open && p.final_on_close;
+  }
+  px16_store(s, p, accum, out_rgba, p.write_accum && !closing, p.finalize || closing);
+  if (COUNT) {
+    uint32_t sum = iters;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) {
+      atomicAdd(d_iters, (unsigned long long)sum);
+      atomicAdd(d_iters + 1, (unsigned long long)consumed);
+      if (p.row_iters) atomicAdd(p.row_iters + ty, (unsigned long long)sum);
+    }
+  }
+  return work;
+}
+
+// One wave64 per 16-px tile, the tile's work key recorded by the tile
+// (the batch's and the single-view default rasterizer).
 template <bool COUNT>
 __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
                                       const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
                                       float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
                                       const RastParams &p, unsigned long long *__restrict__ d_iters,
                                       RastLds &L) {
-  if (p.closed && p.resume && ((p.closed[vtile] && (p.final_on_close || !p.finalize)) || (!p.finalize && start == end))) {
-    if (p.cost_key && threadIdx.x == 0) p.cost_key[vtile] = work_key(0);
-    return;
+  const uint32_t work = rasterize_tile16<COUNT, false>(tx, ty, vtile, start, end, rec, pvals, accum, out_rgba, p,
+                                                       d_iters, L);
+  if (p.cost_key && threadIdx.x == 0) p.cost_key[vtile] = work_key(work);
+}
+
+// One wave64 per 8-px tile (tx, ty): lane l owns pixel (l % 8, l / 8) of the
+// tile; strip h = 0, 1 holds rows 4h .. 4h + 3 (lanes 32h .. 32h + 31).
+// Same staging, tests and loop body as the 16-px tile, with closed flags
+// and skips per 8-px tile. Returns the work measure.
+template <bool MULTI>
+__device__ inline uint32_t rasterize_tile8(uint32_t tx, uint32_t ty, uint32_t vtile, uint32_t start, uint32_t end,
+                                           const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
+                                           float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba,
+                                           const RastParams &p, RastLds &L) {
+  if (tile_skipped(p, vtile, start, end)) return 0;
+  const int lane = threadIdx.x & 63;
+  const uint32_t px = tx * 8u + (uint32_t)(lane & 7), py = ty * 8u + (uint32_t)(lane >> 3);
+  const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+  const bool inb = px < p.width && py < p.height;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = inb ? 1.0f : 0.0f;
+  if (inb && p.resume) {
+    const float4 a = accum[(size_t)py * p.width + px];
+    cr = a.x; cg = a.y; cb = a.z; T = a.w;
   }
-  uint32_t iters = 0;
-  uint32_t work = 0;  // staged batches x 8 + visited (splat, strip) bits
-  const int lane = threadIdx.x;
-  // lane -> pixel column lane%16; strip k (0..3) covers tile rows 4k..4k+3,
-  // this lane's row in strip k is 4k + lane/16
-  const uint32_t px = tx * kTile + (lane & 15);
-  const uint32_t py0 = ty * kTile + (lane >> 4);
-  const float pxf = (float)px + 0.5f;
-  // A pixel is live while !(T < kMinT); pixels outside the image start
-  // with T = 0, so they are never live and never written.
-  float cr[4], cg[4], cb[4], T[4], pyf[4];
-  bool inb[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint32_t y = py0 + 4 * k;
-    inb[k] = px < p.width && y < p.height;
-    pyf[k] = (float)y + 0.5f;
-    cr[k] = 0.0f; cg[k] = 0.0f; cb[k] = 0.0f; T[k] = inb[k] ? 1.0f : 0.0f;
-    if (inb[k] && p.resume) {
-      const float4 a = accum[(size_t)y * p.width + px];
-      cr[k] = a.x; cg[k] = a.y; cb[k] = a.z; T[k] = a.w;
-    }
-  }
-  uint32_t consumed = 0;
+  uint32_t work = 0;
   if (start < end) {
-    const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
-    uint32_t stop = end;
+    const double tile_x0 = (double)(tx * 8u) + 0.5, tile_x1 = tile_x0 + 7.0;
+    const uint32_t mine = (uint32_t)lane >> 5;  // this lane's strip
     for (uint32_t base = start; base < end; base += kStage) {
-      uint32_t strips_live = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) strips_live |= (__ballot(!(T[k] < kMinT)) != 0 ? 1u : 0u) << k;
-      if (strips_live == 0) {
-        stop = base;
-        break;
-      }
-      // Staging: the accumulators wait in LDS while the f64 strip tests run,
-      // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
-      // splats, two lanes per splat with two strips each, so that the LDS
-      // of a wave stays under 6 KB. Both bounds together allow 6 waves per
-      // SIMD.
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        L.acc[(4 * k + 0) * 64 + lane] = cr[k];
-        L.acc[(4 * k + 1) * 64 + lane] = cg[k];
-        L.acc[(4 * k + 2) * 64 + lane] = cb[k];
-        L.acc[(4 * k + 3) * 64 + lane] = T[k];
-      }
+      const uint64_t live = __ballot(!(T < kMinT));
+      if (live == 0) break;
+      // lanes h * 32 + c stage splat c and test it against strip h
       const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
       const uint32_t i = base + slot;
-      uint32_t m = 0;
+      bool hit = false;
       if (i < end) {
         const uint32_t j = pvals[i];
         const float4 a = rec[3 * (size_t)j + 0];
@@ -477,100 +701,37 @@ __device__ inline void rasterize_tile(uint32_t tx, uint32_t ty, uint32_t vtile, 
           L.rec[3 * slot + 1] = b;
           L.rec[3 * slot + 2] = col;
         }
-        // one strip at a time (unrolled, the f64 temporaries of several
-        // strips take more VGPRs)
-#pragma unroll 1
-        for (uint32_t q = 0; q < 2; q++) {
-          const uint32_t k = 2 * half + q;
-          const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
-          if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
-        }
+        const double y0 = (double)(ty * 8u + 4u * half) + 0.5;
+        hit = rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0);
       }
-      // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
-      // strips 0 and 1, lanes 32-63 strips 2 and 3)
-      const uint64_t b02 = __ballot((m & 0x5u) != 0u), b13 = __ballot((m & 0xAu) != 0u);
-      const uint32_t s0 = (strips_live & 1u) ? (uint32_t)b02 : 0u;
-      const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
-      const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
-      const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
-      work += 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1) + __builtin_popcount(s2) +
-                              __builtin_popcount(s3));
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        cr[k] = L.acc[(4 * k + 0) * 64 + lane];
-        cg[k] = L.acc[(4 * k + 1) * 64 + lane];
-        cb[k] = L.acc[(4 * k + 2) * 64 + lane];
-        T[k] = L.acc[(4 * k + 3) * 64 + lane];
-      }
-      const uint32_t cnt = min(kStage, end - base);
-      // COUNT visits every staged splat to count the reference's loop
-      // iterations; otherwise only the splats with a strip bit
-      uint32_t todo = COUNT ? (cnt == 32u ? ~0u : (1u << cnt) - 1u) : (s0 | s1 | s2 | s3);
+      const uint64_t bits = __ballot(hit);
+      const uint32_t s0 = (uint32_t)live ? (uint32_t)bits : 0u;
+      const uint32_t s1 = (uint32_t)(live >> 32) ? (uint32_t)(bits >> 32) : 0u;
+      work += 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1));
+      stage_sync<MULTI>();
+      const uint32_t my_bits = mine ? s1 : s0;
+      uint32_t todo = s0 | s1;
       while (todo) {
         const uint32_t c = (uint32_t)__builtin_ctz(todo);
         todo &= todo - 1u;
-        if (COUNT)
-          iters += (uint32_t)!(T[0] < kMinT) + (uint32_t)!(T[1] < kMinT) + (uint32_t)!(T[2] < kMinT) +
-                   (uint32_t)!(T[3] < kMinT);
-        const uint32_t m = ((s0 >> c) & 1u) | (((s1 >> c) & 1u) << 1) | (((s2 >> c) & 1u) << 2) |
-                           (((s3 >> c) & 1u) << 3);
-        if (m == 0) continue;
-        const float4 a = L.rec[3 * c + 0];
-        const float4 b = L.rec[3 * c + 1];
-        const float4 col = L.rec[3 * c + 2];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          if (!((m >> k) & 1u)) continue;
-          if (T[k] < kMinT) continue;
-          const float dx = pxf - a.x;
-          const float dy = pyf[k] - a.y;
-          const float r = a.z;
-          if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) continue;
-          const float power = gauss_power(b.x, b.y, b.z, dx, dy);
-          if (power > 0.0f || power < col.w) continue;
-          // min(cap, opacity * max(0, exp(power) - floor)) (rasterize.ts:73)
-          // without the max: the record's opacity is sigmoid * fade * aa, so
-          // >= 0 or NaN, and for exp - floor <= 0 the product is <= 0 and
-          // fails the 1/255 test below exactly like 0 does (NaN stays NaN)
-          const float alpha = wmin(kOpacityCap, b.w * (dl_expf_nonpos(power) - kGaussFloor));
-          if (alpha < kMinAlpha) continue;
-          const float w = T[k] * alpha;
-          cr[k] = __builtin_fmaf(w, col.x, cr[k]);
-          cg[k] = __builtin_fmaf(w, col.y, cg[k]);
-          cb[k] = __builtin_fmaf(w, col.z, cb[k]);
-          T[k] = T[k] * (1.0f - alpha);
-        }
+        if (!((my_bits >> c) & 1u) || T < kMinT) continue;
+        blend_splat(L.rec[3 * c + 0], L.rec[3 * c + 1], L.rec[3 * c + 2], pxf, pyf, cr, cg, cb, T);
       }
-      __syncthreads();
+      stage_sync<MULTI>();
     }
-    consumed = stop - start;
   }
   bool closing = false;
   if (p.closed && !p.finalize) {
-    const bool open = __ballot(!(T[0] < kMinT) || !(T[1] < kMinT) || !(T[2] < kMinT) || !(T[3] < kMinT)) != 0;
+    const bool open = __ballot(!(T < kMinT)) != 0;
     if (lane == 0) p.closed[vtile] = open ? 0 : 1;
     closing = !open && p.final_on_close;
   }
-  const bool write_accum = p.write_accum && !closing, finalize = p.finalize || closing;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    if (!inb[k]) continue;
-    const size_t pix = (size_t)(py0 + 4 * k) * p.width + px;
-    if (write_accum) accum[pix] = make_float4(cr[k], cg[k], cb[k], T[k]);
-    if (finalize) out_rgba[pix] = pack_rgba(cr[k], cg[k], cb[k], T[k], p);
+  if (inb) {
+    const size_t pix = (size_t)py * p.width + px;
+    if (p.write_accum && !closing) accum[pix] = make_float4(cr, cg, cb, T);
+    if (p.finalize || closing) out_rgba[pix] = pack_rgba(cr, cg, cb, T, p);
   }
-  if (p.cost_key && lane == 0) p.cost_key[vtile] = work_key(work);
-  if (COUNT) {
-    uint32_t s = iters;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) {
-      atomicAdd(d_iters, (unsigned long long)s);
-      atomicAdd(d_iters + 1, (unsigned long long)consumed);
-      if (p.row_iters) atomicAdd(p.row_iters + ty, (unsigned long long)s);
-    }
-  }
+  return work;
 }
 
 // Open-tile map for the next pass: per row, prefix counts of open tiles.

@@ -41,6 +41,11 @@ class RendererOptions:
     pass_growth: Optional[int] = None
     full_lists: bool = False
     sort_digit_bits: Optional[int] = None   # radix digit width, 1..8 (default 8)
+    # tile side 8, 16 or 32 px (default 16; other sizes keep the image and
+    # fall back to 16 px where maxCoverage truncates a splat at 16 px), and
+    # rasterizer workgroups of 1 or 2x2 tiles (single-view renderer only)
+    tile_size: Optional[int] = None
+    raster_group: Optional[int] = None
 
 
 class Device:
@@ -150,17 +155,7 @@ class Renderer:
 
     def __init__(self, device: Device, opts: Optional[RendererOptions] = None):
         self.device = device
-        opts = opts or RendererOptions()
-        o = L.DlRendererOpts()
-        o.chunk_cap = int(opts.chunk_cap or 0)
-        o.max_coverage_per_splat = int(opts.max_coverage_per_splat or 0)
-        if opts.binning not in ("span", "sort"):
-            raise ValueError("binning must be 'span' or 'sort'")
-        o.binning = 0 if opts.binning == "span" else 1
-        o.first_pass = int(opts.first_pass or 0)
-        o.pass_growth = int(opts.pass_growth or 0)
-        o.full_lists = 1 if opts.full_lists else 0
-        o.sort_digit_bits = int(opts.sort_digit_bits or 0)
+        o = opts_struct(opts or RendererOptions())
         h = C.c_void_p()
         L.check(L.load().dl_renderer_create(device.handle, C.byref(o), C.byref(h)))
         self._h = h
@@ -245,6 +240,8 @@ class Renderer:
             "pairs_consumed": int(st.pairs_consumed), "list_entries": int(st.list_entries),
             "binned_splats": int(st.binned_splats), "row_spans": int(st.row_spans),
             "group_spans": int(st.group_spans), "depth_key_bits": int(st.depth_key_bits),
+            "tile_size": int(st.tile_size), "raster_group": int(st.raster_group),
+            "coverage_truncated": bool(st.coverage_truncated),
         }
 
     def forward(self, scene: "Scene", camera: RenderCamera,
@@ -279,8 +276,9 @@ class Renderer:
         return out
 
     def autotune(self, scene: "Scene", camera: RenderCamera) -> dict:
-        """Times candidate options (radix digit width, pass schedule) on this
-        scene and camera and keeps the fastest (dl_renderer_autotune)."""
+        """Times candidate options (radix digit width, pass schedule, then
+        tile size and rasterizer workgroup) on this scene and camera and keeps
+        the fastest (dl_renderer_autotune)."""
         res = L.DlTuneResult()
         cam = L.camera_struct(camera)
         L.check(L.load().dl_renderer_autotune(self.handle, scene._h, C.byref(cam), C.byref(res)))
@@ -312,10 +310,48 @@ class Renderer:
             pass
 
 
+def opts_struct(opts: RendererOptions) -> "L.DlRendererOpts":
+    o = L.DlRendererOpts()
+    o.chunk_cap = int(opts.chunk_cap or 0)
+    o.max_coverage_per_splat = int(opts.max_coverage_per_splat or 0)
+    if opts.binning not in ("span", "sort"):
+        raise ValueError("binning must be 'span' or 'sort'")
+    o.binning = 0 if opts.binning == "span" else 1
+    o.first_pass = int(opts.first_pass or 0)
+    o.pass_growth = int(opts.pass_growth or 0)
+    o.full_lists = 1 if opts.full_lists else 0
+    o.sort_digit_bits = int(opts.sort_digit_bits or 0)
+    o.tile_size = int(opts.tile_size or 0)
+    o.raster_group = int(opts.raster_group or 0)
+    return o
+
+
+def opts_from_struct(o) -> RendererOptions:
+    return RendererOptions(chunk_cap=o.chunk_cap or None, max_coverage_per_splat=o.max_coverage_per_splat or None,
+                           binning="span" if o.binning == 0 else "sort", first_pass=o.first_pass or None,
+                           pass_growth=o.pass_growth or None, full_lists=bool(o.full_lists),
+                           sort_digit_bits=o.sort_digit_bits or None, tile_size=o.tile_size or None,
+                           raster_group=o.raster_group or None)
+
+
+def tune_candidates(base: Optional[RendererOptions], num_splats: int) -> list:
+    """The options Renderer.autotune times (dl_renderer_tune_candidates):
+    digit widths and pass schedules around `base`, then tile sizes and
+    rasterizer workgroups around it."""
+    b = opts_struct(base or RendererOptions())
+    cnt = C.c_uint32()
+    lib = L.load()
+    L.check(lib.dl_renderer_tune_candidates(C.byref(b), int(num_splats), None, 0, C.byref(cnt)))
+    arr = (L.DlRendererOpts * max(cnt.value, 1))()
+    L.check(lib.dl_renderer_tune_candidates(C.byref(b), int(num_splats), arr, cnt.value, C.byref(cnt)))
+    return [opts_from_struct(arr[k]) for k in range(cnt.value)]
+
+
 def tune_result_dict(res) -> dict:
     o = res.chosen
     return {"chosen": {"sort_digit_bits": o.sort_digit_bits or 8, "first_pass": o.first_pass,
-                       "pass_growth": o.pass_growth, "full_lists": bool(o.full_lists)},
+                       "pass_growth": o.pass_growth, "full_lists": bool(o.full_lists),
+                       "tile_size": o.tile_size or 16, "raster_group": o.raster_group or 1},
             "chosen_ms": float(res.chosen_ms), "default_ms": float(res.default_ms),
             "candidates": int(res.candidates)}
 

@@ -103,6 +103,18 @@ typedef struct {
     uint32_t full_lists;
     /* digit width of the LSD radix sorts, 1..8 bits (0: 8) */
     uint32_t sort_digit_bits;
+    /* Tile side in pixels: 0 or 16 (the reference's, constants.ts:19), 8
+     * or 32. Other sizes change the tile lists, not the image: a frame in
+     * which maxCoverage truncates a splat at 16-px tiles renders at 16 px
+     * (the truncation is the reference's at 16 px), and so do frames with
+     * debug taps, counting or dl_render_forward. Single-view renderer only
+     * (a batch uses 16). */
+    uint32_t tile_size;
+    /* Rasterizer workgroup: 0 or 1, one wave per tile (the reference's
+     * 16x16 workgroup per tile, rasterize.ts:36); 2, 2x2 tiles per
+     * 256-thread workgroup. 32-px tiles always take one 256-thread
+     * workgroup (a wave per 16-px quadrant). */
+    uint32_t raster_group;
 } dl_renderer_opts;
 
 /* ---- device (replaces createDevice, src/gpu/device.ts:27-41) ---- */
@@ -179,6 +191,9 @@ typedef struct {
     /* key bits of the frame's depth sort: 32, or the span bits of a
      * speculative frame that sorted on key - (least key) */
     uint32_t depth_key_bits;
+    /* tile side (px) and rasterizer group of the frame, and 1 when
+     * maxCoverage truncated a splat at 16-px tiles (the frame then used 16) */
+    uint32_t tile_size, raster_group, coverage_truncated;
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
 /* Counts rasterize loop iterations (the work unit of the rasterize roofline)
@@ -244,6 +259,13 @@ typedef struct {
     uint32_t candidates;         /* configurations timed */
 } dl_tune_result;
 int dl_renderer_autotune(dl_renderer *r, const dl_scene *scene, const dl_camera *cam, dl_tune_result *out);
+/* The options dl_renderer_autotune times, without a device: stage 1 around
+ * base (radix digit widths 8 / 6 / 4 and pass schedules, for scenes of
+ * num_splats splats), then stage 2 around base (tile sizes 16 / 8 / 32 and
+ * rasterizer workgroups; the tuner takes these around its stage-1 winner).
+ * *count = the number of candidates; out (may be NULL) takes up to cap. */
+int dl_renderer_tune_candidates(const dl_renderer_opts *base, uint64_t num_splats, dl_renderer_opts *out,
+                                uint32_t cap, uint32_t *count);
 
 /* ---- batched views (no reference counterpart: it stands beside
  * Renderer.render, src/render/pipeline.ts:181-185, whose per-view image it

@@ -58,7 +58,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(L.DlSplats) == 8 + 6 * 8 + 8
     assert ctypes.sizeof(L.DlCamera) == 8 + 64 + 24 + 8
     assert ctypes.sizeof(L.DlBackground) == 16
-    assert ctypes.sizeof(L.DlRendererOpts) == 28
+    assert ctypes.sizeof(L.DlRendererOpts) == 36
     assert ctypes.sizeof(L.DlView) == 8 + 104 + 16
     assert ctypes.sizeof(L.DlBatchStats) == 32
     assert ctypes.sizeof(L.DlVoxelOpts) == 8 + 8 + 8 + 48

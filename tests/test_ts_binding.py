@@ -95,13 +95,15 @@ int main(void) {
   S(dl_splats) O(dl_splats, positions) O(dl_splats, colors_sh) O(dl_splats, sh_bands)
   S(dl_camera) O(dl_camera, view_matrix) O(dl_camera, fov_y) O(dl_camera, near_plane) O(dl_camera, far_plane)
   O(dl_camera, projection)
-  S(dl_background) S(dl_renderer_opts) S(dl_device_opts)
+  S(dl_background) S(dl_renderer_opts) O(dl_renderer_opts, tile_size) O(dl_renderer_opts, raster_group)
+  S(dl_device_opts)
   S(dl_voxel_opts) O(dl_voxel_opts, opacity_cutoff) O(dl_voxel_opts, has_bounds) O(dl_voxel_opts, bounds_min)
   O(dl_voxel_opts, bounds_max)
   S(dl_grads) S(dl_view) O(dl_view, camera) O(dl_view, background)
   S(dl_tune_result) O(dl_tune_result, chosen_ms) O(dl_tune_result, default_ms) O(dl_tune_result, candidates)
   S(dl_render_stats) O(dl_render_stats, num_pairs) O(dl_render_stats, stage_ms) O(dl_render_stats, total_ms)
   O(dl_render_stats, pixel_iters) O(dl_render_stats, list_entries) O(dl_render_stats, depth_key_bits)
+  O(dl_render_stats, tile_size) O(dl_render_stats, raster_group)
   S(dl_batch_stats) S(dl_voxel_stats) S(dl_voxel_device_view)
   return 0;
 }
@@ -136,7 +138,9 @@ def test_ts_layouts_are_the_c_layouts(c_layout):
             "dl_camera far_plane": 88, "dl_camera projection": 96,
             "dl_voxel_opts opacity_cutoff": 8, "dl_voxel_opts has_bounds": 16, "dl_voxel_opts bounds_min": 24,
             "dl_voxel_opts bounds_max": 48, "dl_view camera": 8, "dl_view background": 112,
-            "dl_tune_result chosen_ms": 28, "dl_tune_result default_ms": 32, "dl_tune_result candidates": 36,
+            "dl_tune_result chosen_ms": 36, "dl_tune_result default_ms": 40, "dl_tune_result candidates": 44,
+            "dl_renderer_opts tile_size": 28, "dl_renderer_opts raster_group": 32,
+            "dl_render_stats tile_size": 124, "dl_render_stats raster_group": 128,
             "dl_render_stats num_pairs": 24, "dl_render_stats stage_ms": 32, "dl_render_stats total_ms": 64,
             "dl_render_stats pixel_iters": 72, "dl_render_stats list_entries": 88}
     for k, v in want.items():

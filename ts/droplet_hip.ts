@@ -42,7 +42,13 @@ export interface RenderCamera {
 }
 export interface BackgroundRGBA { r: number; g: number; b: number; a: number; }
 export interface Bounds { min: [number, number, number]; max: [number, number, number]; }
-export interface RendererOptions { chunkCap?: number; maxCoveragePerSplat?: number; }
+export interface RendererOptions {
+    chunkCap?: number; maxCoveragePerSplat?: number;
+    /** 8, 16 or 32 px (single-view renderer; the image is the 16-px one) */
+    tileSize?: number;
+    /** 1: one wave per tile, 2: 2x2 tiles per workgroup */
+    rasterGroup?: number;
+}
 export interface VoxelizeOptions { voxelResolution?: number; opacityCutoff?: number; bounds?: Bounds; }
 export interface VoxelBlock { blockIdx: number; maskLo: number; maskHi: number; }
 
@@ -77,6 +83,7 @@ export const symbols = {
     dl_render_forward: { args: [P, P, P, P, P], returns: I32 },
     dl_render_backward: { args: [P, P, P], returns: I32 },
     dl_renderer_autotune: { args: [P, P, P, P], returns: I32 },
+    dl_renderer_tune_candidates: { args: [P, U64, P, U32, P], returns: I32 },
     dl_batch_create: { args: [P, P, P], returns: I32 },
     dl_batch_destroy: { args: [P], returns: VOID },
     dl_batch_set_graph: { args: [P, I32], returns: I32 },
@@ -126,13 +133,13 @@ export const LAYOUT = {
     dl_splats: 64,          // u32 count | 6 pointers at 8..48 | u32 sh_bands at 56
     dl_camera: 104,         // u32 w, h | f32 view[16] at 8 | f64 fov_y 72, near 80, far 88 | u32 projection 96
     dl_background: 16,      // f32 r, g, b, a
-    dl_renderer_opts: 28,   // 7 x u32
+    dl_renderer_opts: 36,   // 9 x u32: ... sort_digit_bits 24, tile_size 28, raster_group 32
     dl_device_opts: 4,      // i32 device_index
     dl_voxel_opts: 72,      // f64 res 0, cutoff 8 | i32 has_bounds 16 | f64 min[3] 24, max[3] 48
     dl_grads: 48,           // 6 pointers
     dl_view: 128,           // u32 scene | dl_camera at 8 | dl_background at 112
-    dl_tune_result: 40,     // dl_renderer_opts chosen | f32 chosen_ms 28, default_ms 32 | u32 candidates 36
-    dl_render_stats: 128,
+    dl_tune_result: 48,     // dl_renderer_opts chosen | f32 chosen_ms 36, default_ms 40 | u32 candidates 44
+    dl_render_stats: 136,
     dl_batch_stats: 32,
     dl_voxel_stats: 24,
 } as const;
@@ -227,6 +234,7 @@ export class Scene {
 export interface RenderStats {
     numVisible: number; numChunks: number; numTiles: number; numPairs: bigint;
     stageMs: number[]; totalMs: number; pixelIters: bigint; listEntries: bigint;
+    tileSize: number; rasterGroup: number;
 }
 
 export class Renderer {
@@ -235,7 +243,8 @@ export class Renderer {
     private trainSize: [number, number] = [0, 0];
 
     constructor(device: Device, opts: RendererOptions = {}) {
-        const o = new Uint32Array([opts.chunkCap ?? 0, opts.maxCoveragePerSplat ?? 0, 0, 0, 0, 0, 0]);
+        const o = new Uint32Array([opts.chunkCap ?? 0, opts.maxCoveragePerSplat ?? 0, 0, 0, 0, 0, 0,
+                                   opts.tileSize ?? 0, opts.rasterGroup ?? 0]);
         const h = handleOut();
         check(L.dl_renderer_create(device.handle, o, h));
         this.h = asHandle(h);
@@ -287,7 +296,7 @@ export class Renderer {
     autotune(scene: Scene, camera: RenderCamera): { chosenMs: number; defaultMs: number; candidates: number } {
         const r = new Uint8Array(LAYOUT.dl_tune_result), v = new DataView(r.buffer);
         check(L.dl_renderer_autotune(this.h, scene.handle, cameraStruct(camera), r));
-        return { chosenMs: v.getFloat32(28, true), defaultMs: v.getFloat32(32, true), candidates: v.getUint32(36, true) };
+        return { chosenMs: v.getFloat32(36, true), defaultMs: v.getFloat32(40, true), candidates: v.getUint32(44, true) };
     }
 
     setTiming(on: boolean): void { check(L.dl_renderer_set_timing(this.h, on ? 1 : 0)); }
@@ -301,6 +310,7 @@ export class Renderer {
             numVisible: v.getUint32(0, true), numChunks: v.getUint32(4, true), numTiles: v.getUint32(8, true),
             numPairs: v.getBigUint64(24, true), stageMs, totalMs: v.getFloat32(64, true),
             pixelIters: v.getBigUint64(72, true), listEntries: v.getBigUint64(88, true),
+            tileSize: v.getUint32(124, true), rasterGroup: v.getUint32(128, true),
         };
     }
 
@@ -314,7 +324,8 @@ export class BatchRenderer {
     private h: Handle;
     private sizes: [number, number][] = [];
     constructor(device: Device, opts: RendererOptions = {}) {
-        const o = new Uint32Array([opts.chunkCap ?? 0, opts.maxCoveragePerSplat ?? 0, 0, 0, 0, 0, 0]);
+        const o = new Uint32Array([opts.chunkCap ?? 0, opts.maxCoveragePerSplat ?? 0, 0, 0, 0, 0, 0,
+                                   opts.tileSize ?? 0, opts.rasterGroup ?? 0]);
         const h = handleOut();
         check(L.dl_batch_create(device.handle, o, h));
         this.h = asHandle(h);
