@@ -103,8 +103,8 @@ def stage_table(stage_ms, st, n, bands):
     """Per-stage hipEvent times with their algorithmic HBM bytes and the
     fraction of the 8 TB/s peak those bytes take at that time. Stage names
     say what runs on the span-binning path; the byte figures are the
-    minimum each stage must move (counted from the counting frame's own
-    pass sizes), not the bytes the counters measure."""
+    minimum each stage must move (counted from a tally frame's own pass
+    sizes at the timed tile size), not the bytes the counters measure."""
     passes = max(1, st["num_chunks"])
     items, spans, gspans = st["binned_splats"], st["row_spans"], st["group_spans"]
     entries = st["list_entries"]
@@ -134,7 +134,7 @@ def stage_table(stage_ms, st, n, bands):
         out[name] = {"ms": round(ms, 4), "kernels": kernels, "algorithmic_bytes": int(nbytes),
                      "GB_s": round(gbs, 1) if gbs else None,
                      "hbm_frac": round(gbs / PEAK_HBM_GBS, 4) if gbs else None}
-    out["rasterize"] = {"ms": round(stage_ms.get("rasterize", 0.0), 4), "kernels": "k_rasterize (per pass)",
+    out["rasterize"] = {"ms": round(stage_ms.get("rasterize", 0.0), 4), "kernels": "rasterizer (per pass)",
                         "bound": "valu (see roofline)"}
     for k in ("scan", "pair_sort", "ranges"):  # the emit + pair sort path only
         if stage_ms.get(k, 0.0) > 0:
@@ -345,6 +345,8 @@ def main() -> int:
     ap.add_argument("--splats", type=int, default=6_000_000)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-tune", action="store_true", help="time the default options (no autotune)")
+    ap.add_argument("--tile-size", type=int, default=None, help="renderer tile size (8, 16, 32) before tuning")
+    ap.add_argument("--raster-group", type=int, default=None, help="rasterizer group (1, 2) before tuning")
     ap.add_argument("--no-voxel", action="store_true", help="skip the voxelizer leg")
     ap.add_argument("--no-equirect", action="store_true", help="skip the equirect leg")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -362,7 +364,7 @@ def main() -> int:
 
     from droplet_amd import synthetic as S
     from droplet_amd import _lib as L
-    from droplet_amd.renderer import Renderer, Scene, create_device
+    from droplet_amd.renderer import Renderer, RendererOptions, Scene, create_device
 
     def note(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -388,7 +390,7 @@ def main() -> int:
         L.check(L.load().dl_debug_valu_probe(dev.handle, VALU_PROBE_BLOCKS, VALU_PROBE_ITERS, C.byref(chk)))
         note(f"VALU probe: {VALU_PROBE_BLOCKS} x 256 threads x {16 * VALU_PROBE_ITERS} FMAs (checksum {chk.value:.6g})")
     scene = Scene(dev, data)
-    r = Renderer(dev)
+    r = Renderer(dev, RendererOptions(tile_size=args.tile_size, raster_group=args.raster_group))
     note(f"scene resident on {dev.name}")
 
     tune = None
@@ -407,13 +409,19 @@ def main() -> int:
     ms_mean = total_ms / args.steps
     # one untimed instrumented frame with the tuned options: rasterize work
     # units (pixel loop iterations) and the passes' binning sizes
+    # (the loop count is the reference's, on 16-px tiles whatever the tuned
+    # tile size), then one with the binning tallies at the tuned shape
     r.set_counting(True)
     r.render_scene(scene, cam, readback=False)
     cst = r.stats()
+    r.set_counting("tallies")
+    r.render_scene(scene, cam, readback=False)
+    tst = r.stats()
     r.set_counting(False)
     iters, consumed = cst["pixel_iters"], cst["pairs_consumed"]
-    note(f"instrumented frame: {iters} pixel iterations, {consumed} tile-list entries read, "
-         f"{cst['binned_splats']} binned splats, {cst['row_spans']} row spans, {cst['group_spans']} group spans")
+    note(f"instrumented frame: {iters} pixel iterations, {consumed} tile-list entries read; at "
+         f"{tst['tile_size']}-px tiles: {tst['binned_splats']} binned splats, {tst['row_spans']} row spans, "
+         f"{tst['group_spans']} group spans, {tst['list_entries']} list entries")
     # the same number of frames again with hipEvents at the stage boundaries
     # (each event leaves the GPU idle for a few microseconds)
     r.set_timing(True)
@@ -450,8 +458,12 @@ def main() -> int:
          f"p90 {_pct(per, 90):.3f}, mean {ms_mean:.3f})")
 
     rast_ms = stage_ms["rasterize"]
+    rast_kernel = {(16, 1): "k_rasterize<false>", (16, 2): "k_rasterize_2x2<16>", (8, 1): "k_rasterize8",
+                   (8, 2): "k_rasterize_2x2<8>", (32, 1): "k_rasterize32"}[(st["tile_size"], st["raster_group"])]
     achieved = iters * RAST_OPS_PER_ITER / (rast_ms * 1e-3) / 1e12
     traffic = _load_traffic()
+    if traffic and traffic.get("kernel") != rast_kernel:
+        traffic = None  # the committed counters are of another rasterizer
     line = {
         "metric": METRIC,
         "value": round(ms_median, 3),
@@ -471,10 +483,11 @@ def main() -> int:
             "workload": "large forward render: 6M Gaussians, SH bands 3, 3840x2160 pinhole, scene resident in HBM",
             "splats": data.count, "sh_bands": data.sh_bands, "width": cam.width, "height": cam.height,
             "projection": cam.projection, "visible": st["num_visible"], "pairs": st["num_pairs"],
-            "tiles": st["num_tiles"], "pairs_read_by_rasterize": consumed,
+            "tiles": st["num_tiles"], "tile_size": st["tile_size"], "raster_group": st["raster_group"],
+            "reference_pairs_16px": cst["num_pairs"], "pairs_read_by_rasterize_16px": consumed,
         },
         "roofline": {
-            "kernel": "k_rasterize", "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "kernel": rast_kernel, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             # one unit = the frame's rasterize stage: cst["num_chunks"] launches
             # (front-to-back passes); work, time and traffic all cover the
@@ -483,14 +496,15 @@ def main() -> int:
             "traffic_source": ("profiles/rasterize_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                                "this command's rasterize launches, scaled by a 1 GiB copy"
                                if traffic else None),
-            "work": (f"{iters} pixel loop iterations x {RAST_OPS_PER_ITER} ops: the reference's loop, of which "
-                     "the kernel skips whole strips and exps it proves cannot change a pixel, so this rate "
-                     "exceeds the 78.6 TFLOP/s of non-packed FP32 instructions"),
-            "launches_per_frame": cst["num_chunks"],
+            "work": (f"{iters} pixel loop iterations x {RAST_OPS_PER_ITER} ops: the reference's loop (16-px "
+                     "tiles), of which the kernel skips whole strips and exps it proves cannot change a pixel: "
+                     "a work rate, not the FP32 instructions executed (those are in "
+                     "profiles/rasterize_voxelize_valu_counters.md)"),
+            "launches_per_frame": st["num_chunks"],
             "stage_ms": round(rast_ms, 4),
         },
         "autotune": tune,
-        "stages": stage_table(stage_ms, cst, data.count, data.sh_bands),
+        "stages": stage_table(stage_ms, tst, data.count, data.sh_bands),
         "readback_ms": round(readback_ms, 3),
         "readback_GB_s": round(img.nbytes / (readback_ms * 1e-3) / 1e9, 2),
         "readback_reused_buffer_ms": round(readback_reuse_ms, 3),

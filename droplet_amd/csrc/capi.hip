@@ -462,7 +462,9 @@ int dl_renderer_set_timing(dl_renderer *r, int enabled) {
 int dl_renderer_set_counting(dl_renderer *r, int enabled) {
   return guard([&] {
     if (!r) fail(DL_ERR_INVALID, "null renderer");
-    r->r.counting = enabled != 0;
+    if (enabled < 0 || enabled > 2) fail(DL_ERR_INVALID, "counting must be 0, 1 or 2");
+    r->r.counting = enabled == 1;
+    r->r.tallying = enabled == 2;
   });
 }
 

@@ -140,6 +140,7 @@ struct Renderer {
   bool timing = false;
   bool keep_taps = false;
   bool counting = false;
+  bool tallying = false;  // binning tallies only (dl_renderer_set_counting 2)
   // Frames of span binning normally run without a host read of the
   // visible-splat and pair totals after the depth sort: passes cover all
   // sorted splats (invisible ones have no tiles) and the pair capacity is

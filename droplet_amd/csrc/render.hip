@@ -497,7 +497,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   // visible depth key, [4..6] span-binning scalars, [7] maxCoverage
   // truncation at 16-px tiles, [8..9] u64 total pairs, [10..15] u64 loop iterations and
   // consumed list entries (counting frames), list entries built, [16..21]
-  // u64 binned splats, row spans and group spans (counting frames)
+  // u64 binned splats, row spans and group spans (counting and tally frames)
   uint32_t *d_nvis = small.as<uint32_t>();
   uint32_t *d_total = d_nvis + 1;
   uint32_t *d_bin = d_nvis + 4;
@@ -505,10 +505,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   unsigned long long *d_iters = reinterpret_cast<unsigned long long *>(d_nvis + 10);
   unsigned long long *d_entries = d_iters + 2;
   unsigned long long *d_tally = reinterpret_cast<unsigned long long *>(d_nvis + 16);
-  if (counting) {
-    DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
-    DL_HIP(hipMemsetAsync(d_tally, 0, 3 * sizeof(unsigned long long), s));
-  }
+  // binning tallies: counting frames and list-tally frames (which keep
+  // the frame's tile size and rasterizer)
+  const bool tally = counting || tallying;
+  if (counting) DL_HIP(hipMemsetAsync(d_iters, 0, 2 * sizeof(unsigned long long), s));
+  if (tally) DL_HIP(hipMemsetAsync(d_tally, 0, 3 * sizeof(unsigned long long), s));
   k_zero_u64<<<1, 64, 0, s>>>(d_entries);
   DL_HIP(hipGetLastError());
   row_iters_rows = 0;
@@ -784,7 +785,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
         mark(DL_STAGE_EMIT);
         span_bin(s, rec.as<float4>(), cids, cn, grid, npairs, live, want_rect8 ? rect8.as<uint2>() : nullptr, n,
                  bin_ws, ranges.as<uint32_t>(), pvals.as<uint32_t>(), d_bin, d_entries, fused);
-        if (counting) {
+        if (tally) {
           const bool compacted = live.pre && want_rect8;
           k_bin_tally<<<1, 64, 0, s>>>(d_bin, compacted ? bin_ws.live_small.as<uint32_t>() : nullptr, cn, d_tally);
           DL_HIP(hipGetLastError());
@@ -883,6 +884,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     if (counting) {
       stats.pixel_iters = u64_at(10);
       stats.pairs_consumed = u64_at(12);
+    }
+    if (tally) {
       stats.binned_splats = u64_at(16);
       stats.row_spans = u64_at(18);
       stats.group_spans = u64_at(20);

@@ -465,14 +465,32 @@ __device__ inline void px16_store(const Px16 &s, const RastParams &p, float4 *__
   }
 }
 
-// The reference's per-pixel loop body (rasterize.ts:63-78) for one splat.
-__device__ inline void blend_splat(const float4 a, const float4 b, const float4 col, float pxf, float pyf,
-                                   float &cr, float &cg, float &cb, float &T) {
+// The reference's per-pixel loop body (rasterize.ts:63-78) for one splat,
+// split at the pixel column: a lane's pixels share one column, so the box
+// test on dx and the dx-only products of the exponent, (2B) dx and
+// (A dx) dx, are computed once per splat (col_terms) for all of the lane's
+// rows. blend_row then evaluates gauss_power's expression with those
+// products, the same operations in the same order.
+struct ColTerms {
+  float bdx, adx2;
+  bool out;  // r <= 0 or |dx| > r: every row of the column skips the splat
+};
+
+__device__ inline ColTerms col_terms(const float4 a, const float4 b, float pxf) {
   const float dx = pxf - a.x;
-  const float dy = pyf - a.y;
   const float r = a.z;
-  if (r <= 0.0f || __builtin_fabsf(dx) > r || __builtin_fabsf(dy) > r) return;
-  const float power = gauss_power(b.x, b.y, b.z, dx, dy);
+  ColTerms c;
+  c.out = r <= 0.0f || __builtin_fabsf(dx) > r;
+  c.bdx = (2.0f * b.y) * dx;
+  c.adx2 = (b.x * dx) * dx;
+  return c;
+}
+
+__device__ inline void blend_row(const ColTerms &c, const float4 a, const float4 b, const float4 col, float pyf,
+                                 float &cr, float &cg, float &cb, float &T) {
+  const float dy = pyf - a.y;
+  if (__builtin_fabsf(dy) > a.z) return;
+  const float power = (-0.5f) * __builtin_fmaf(b.z * dy, dy, __builtin_fmaf(c.bdx, dy, c.adx2));
   if (power > 0.0f || power < col.w) return;
   // min(cap, opacity * max(0, exp(power) - floor)) (rasterize.ts:73)
   // without the max: the record's opacity is sigmoid * fade * aa, so
@@ -591,11 +609,14 @@ __device__ inline uint32_t px16_run(Px16 &s, uint32_t ox, uint32_t oy, uint32_t 
       const float4 a = L.rec[3 * c + 0];
       const float4 b = L.rec[3 * c + 1];
       const float4 col = L.rec[3 * c + 2];
+      // the strip bits are uniform over the wave (scalar branches); the
+      // column test and the saturation test are per lane
+      const ColTerms ct = col_terms(a, b, s.pxf);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         if (!((mm >> k) & 1u)) continue;
-        if (s.T[k] < kMinT) continue;
-        blend_splat(a, b, col, s.pxf, s.pyf[k], s.cr[k], s.cg[k], s.cb[k], s.T[k]);
+        if (s.T[k] < kMinT || ct.out) continue;
+        blend_row(ct, a, b, col, s.pyf[k], s.cr[k], s.cg[k], s.cb[k], s.T[k]);
       }
     }
     stage_sync<MULTI>();
@@ -715,7 +736,9 @@ __device__ inline uint32_t rasterize_tile8(uint32_t tx, uint32_t ty, uint32_t vt
         const uint32_t c = (uint32_t)__builtin_ctz(todo);
         todo &= todo - 1u;
         if (!((my_bits >> c) & 1u) || T < kMinT) continue;
-        blend_splat(L.rec[3 * c + 0], L.rec[3 * c + 1], L.rec[3 * c + 2], pxf, pyf, cr, cg, cb, T);
+        const float4 a = L.rec[3 * c + 0], b = L.rec[3 * c + 1];
+        const ColTerms ct = col_terms(a, b, pxf);
+        if (!ct.out) blend_row(ct, a, b, L.rec[3 * c + 2], pyf, cr, cg, cb, T);
       }
       stage_sync<MULTI>();
     }

@@ -205,8 +205,12 @@ class Renderer:
     def set_timing(self, enabled: bool) -> None:
         L.check(L.load().dl_renderer_set_timing(self.handle, int(enabled)))
 
-    def set_counting(self, enabled: bool) -> None:
-        L.check(L.load().dl_renderer_set_counting(self.handle, int(enabled)))
+    def set_counting(self, enabled) -> None:
+        """True: count the reference's rasterize loop iterations (16-px
+        tiles) and the binning tallies; "tallies": the binning tallies only,
+        at the renderer's own tile size; False: off."""
+        mode = 2 if enabled == "tallies" else (1 if enabled else 0)
+        L.check(L.load().dl_renderer_set_counting(self.handle, mode))
 
     def set_pair_budget(self, pairs: int) -> None:
         """Frames with more (tile, splat) pairs than the budget render in

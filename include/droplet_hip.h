@@ -196,8 +196,11 @@ typedef struct {
     uint32_t tile_size, raster_group, coverage_truncated;
 } dl_render_stats;
 int dl_renderer_set_timing(dl_renderer *r, int enabled);
-/* Counts rasterize loop iterations (the work unit of the rasterize roofline)
- * with an instrumented kernel variant; meant for one untimed frame. */
+/* enabled = 1: counts rasterize loop iterations (the work unit of the
+ * rasterize roofline) with an instrumented kernel variant, at 16-px tiles
+ * one wave per tile, and the binning tallies of dl_render_stats; 2: the
+ * binning tallies only, at the frame's own tile size and rasterizer; 0:
+ * off. Meant for one untimed frame. */
 int dl_renderer_set_counting(dl_renderer *r, int enabled);
 int dl_renderer_stats(dl_renderer *r, dl_render_stats *out);
 /* A frame whose (tile, splat) pairs exceed 2^32 - 16 renders in passes of at
