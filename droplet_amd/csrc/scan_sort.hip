@@ -1,7 +1,9 @@
 // Device-wide exclusive scan and stable LSD radix sort (u32 key / u32 value)
 // for gfx950.
 //
-// Both are tiled 256-thread workgroups (4 wave64) over 4096-element tiles.
+// Both run over 4096-element tiles: the scan in 256-thread workgroups (4
+// wave64 of 16 elements per lane), the sort in 512-thread workgroups (8
+// wave64 of 8 elements per lane).
 // The sort is the replacement for src/gpu/radix-sort.ts:91-216, whose scatter
 // takes slots with atomicAdd and is therefore unstable; this one ranks every
 // element inside its wave with ballots in input order, so equal keys keep
@@ -29,6 +31,26 @@ constexpr uint32_t kMaxDigits = 1u << kMaxDigitBits;
 // the totals.
 constexpr int kSumPartialsMax = 4096;
 
+// Radix sort tiles: kRBlock threads of kRItems keys each per workgroup,
+// 4096 keys per tile. Measured on the 6M-key depth sort (three passes):
+// 512 x 8 takes 176 us; 256 x 16 199 us (113-120 VGPRs leave 3-4 waves
+// per SIMD); 1024 x 4 234 us and 512 x 16 or 256 x 32 (8192-key tiles)
+// 250-280 us, the LDS of the larger tiles or waves limiting occupancy.
+// Ranking by ballots in the upsweep as in the downsweep (one LDS update per
+// distinct digit) cost 10 us per pass more than LDS atomics. Tiles below
+// 4096 keys would outgrow the no-alloc sort's workspace.
+#ifndef DL_RADIX_THREADS
+#define DL_RADIX_THREADS 512
+#endif
+#ifndef DL_RADIX_ITEMS
+#define DL_RADIX_ITEMS 8
+#endif
+constexpr int kRBlock = DL_RADIX_THREADS;
+constexpr int kRItems = DL_RADIX_ITEMS;
+constexpr int kRTile = kRBlock * kRItems;
+constexpr int kRWaves = kRBlock / 64;
+constexpr int kRWaveItems = 64 * kRItems;  // elements of one wave's stripe
+
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = __lane_id();
 #pragma unroll
@@ -45,8 +67,9 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-// Block-wide exclusive scan of one value per thread; returns the exclusive
-// prefix and writes the block total to *total (all threads).
+// Block-wide exclusive scan of one value per thread (WAVES waves); returns
+// the exclusive prefix and writes the block total to *total (all threads).
+template <int WAVES = kWaves>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave, uint32_t *total) {
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
@@ -55,7 +78,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave, ui
   __syncthreads();
   uint32_t wave_off = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; w++) {
+  for (int w = 0; w < WAVES; w++) {
     uint32_t c = s_wave[w];
     if (w < wave) wave_off += c;
     tot += c;
@@ -163,8 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *__restrict
 }
 
 // ------------------------------- radix ------------------------------------
-// A sort block owns one 4096-element tile. Unsegmented: tile b is elements
-// [4096 b, 4096 b + 4096) and the count of digit d of block b sits at
+// A sort block owns one kRTile-element tile. Unsegmented: tile b is elements
+// [kRTile b, kRTile (b + 1)) and the count of digit d of block b sits at
 // d * nblocks + b. Segmented (SegGeom::seg_start set): segments are
 // contiguous element ranges from 0, tiles never straddle a segment, and the
 // counts of segment s are laid out [digit][tile of s] from R * blk0[s]
@@ -180,8 +203,8 @@ struct TileSpan {
 __device__ inline TileSpan tile_span(const SegGeom &g, size_t n, uint32_t nblocks, uint32_t b, uint32_t ndig) {
   TileSpan t;
   if (!g.seg_start) {
-    t.i0 = (size_t)b * kTileN;
-    t.n = t.i0 < n ? (uint32_t)min((size_t)kTileN, n - t.i0) : 0u;
+    t.i0 = (size_t)b * kRTile;
+    t.n = t.i0 < n ? (uint32_t)min((size_t)kRTile, n - t.i0) : 0u;
     t.cbase = b;
     t.cstride = nblocks;
     return t;
@@ -193,9 +216,9 @@ __device__ inline TileSpan tile_span(const SegGeom &g, size_t n, uint32_t nblock
   if (b >= g.blk0[g.nseg]) return t;
   const uint32_t sg = g.blk_seg[b];
   const uint32_t b0 = g.blk0[sg], nb = g.blk0[sg + 1] - b0, k = b - b0;
-  const uint32_t s0 = g.seg_start[sg] + k * (uint32_t)kTileN;
+  const uint32_t s0 = g.seg_start[sg] + k * (uint32_t)kRTile;
   t.i0 = s0;
-  t.n = min((uint32_t)kTileN, g.seg_start[sg + 1] - s0);
+  t.n = min((uint32_t)kRTile, g.seg_start[sg + 1] - s0);
   t.cbase = (size_t)ndig * b0 + k;
   t.cstride = nb;
   return t;
@@ -206,30 +229,41 @@ __device__ inline uint32_t mapped_key(uint32_t k, const RadixKeyMap &m, uint32_t
   return m.kmin ? (k == m.invisible ? m.sentinel : k - kmin) : k;
 }
 
-// Tile layout: wave w owns elements [w*1024, (w+1)*1024) of the tile; item k
-// of lane l is element w*1024 + k*64 + l. Loads are coalesced and the
-// (k, lane) processing order is the input order, which makes ranks stable.
-__global__ __launch_bounds__(kBlock) void k_radix_upsweep(const uint32_t *__restrict__ keys, size_t n, int shift,
-                                                          int bits, uint32_t *__restrict__ counts,
-                                                          uint32_t nblocks, SegGeom g, RadixKeyMap km) {
-  __shared__ uint32_t s_hist[kMaxDigits];
+// Tile layout: wave w owns elements [w*kRWaveItems, (w+1)*kRWaveItems) of
+// the tile; item k of lane l is element w*kRWaveItems + k*64 + l. Loads are
+// coalesced and the (k, lane) processing order is the input order, which
+// makes ranks stable.
+
+__global__ __launch_bounds__(kRBlock) void k_radix_upsweep(const uint32_t *__restrict__ keys, size_t n, int shift,
+                                                           int bits, uint32_t *__restrict__ counts,
+                                                           uint32_t nblocks, SegGeom g, RadixKeyMap km) {
   const uint32_t mask = (1u << bits) - 1u;
   const TileSpan ts = tile_span(g, n, nblocks, blockIdx.x, mask + 1u);
   if (ts.n == 0 && g.seg_start) return;  // unused capacity block
   const uint32_t kmin = km.kmin ? *km.kmin : 0u;
-  for (uint32_t d = threadIdx.x; d < kMaxDigits; d += kBlock) s_hist[d] = 0;
-  __syncthreads();
+  // all of a thread's loads are issued before its first LDS atomic (loads
+  // and atomics interleaved left one load in flight per thread: 14 us per
+  // pass against 10 at 6M keys)
+  __shared__ uint32_t s_hist[kMaxDigits];
+  for (uint32_t d = threadIdx.x; d < kMaxDigits; d += kRBlock) s_hist[d] = 0;
+  uint32_t key[kRItems];
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    const uint32_t li = (uint32_t)k * kBlock + threadIdx.x;
-    if (li < ts.n) atomicAdd(&s_hist[(mapped_key(keys[ts.i0 + li], km, kmin) >> shift) & mask], 1u);
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t li = (uint32_t)k * kRBlock + threadIdx.x;
+    key[k] = li < ts.n ? keys[ts.i0 + li] : 0u;
   }
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d <= mask; d += kBlock) counts[ts.cbase + (size_t)d * ts.cstride] = s_hist[d];
+#pragma unroll
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t li = (uint32_t)k * kRBlock + threadIdx.x;
+    if (li < ts.n) atomicAdd(&s_hist[(mapped_key(key[k], km, kmin) >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d <= mask; d += kRBlock) counts[ts.cbase + (size_t)d * ts.cstride] = s_hist[d];
 }
 
 template <int BITS>
-__global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
+__global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
                                                             const uint32_t *__restrict__ vals_in,
                                                             uint32_t *__restrict__ keys_out,
                                                             uint32_t *__restrict__ vals_out, size_t n, int shift,
@@ -238,14 +272,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   constexpr uint32_t kDigits = 1u << BITS;
   constexpr uint32_t kMask = kDigits - 1u;
   // digit slots: one per thread, two per thread for 9-bit digits
-  constexpr uint32_t kSlots = kDigits > (uint32_t)kBlock ? kDigits : (uint32_t)kBlock;
-  constexpr uint32_t kPer = kSlots / kBlock;
-  __shared__ uint32_t s_whist[kWaves][kSlots];
-  __shared__ uint32_t s_keys[kTileN];
-  __shared__ uint32_t s_vals[kTileN];
+  constexpr uint32_t kSlots = kDigits > (uint32_t)kRBlock ? kDigits : (uint32_t)kRBlock;
+  constexpr uint32_t kPer = kSlots / kRBlock;
+  __shared__ uint32_t s_whist[kRWaves][kSlots];
+  __shared__ uint32_t s_keys[kRTile];
+  __shared__ uint32_t s_vals[kRTile];
   __shared__ uint32_t s_start[kSlots];
   __shared__ uint32_t s_gbase[kSlots];
-  __shared__ uint32_t s_wave[kWaves];
+  __shared__ uint32_t s_wave[kRWaves];
 
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
@@ -255,14 +289,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   const uint32_t tile_n = ts.n;
   const uint32_t kmin = km.kmin ? *km.kmin : 0u;
 
-  for (uint32_t d = threadIdx.x; d < kWaves * kSlots; d += kBlock) (&s_whist[0][0])[d] = 0;
+  for (uint32_t d = threadIdx.x; d < kRWaves * kSlots; d += kRBlock) (&s_whist[0][0])[d] = 0;
   __syncthreads();
 
-  uint32_t key[kItems], val[kItems], rank[kItems];
+  uint32_t key[kRItems], val[kRItems], rank[kRItems];
   const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t li = (uint32_t)wave * kRWaveItems + (uint32_t)k * 64u + (uint32_t)lane;
     const bool valid = li < tile_n;
     key[k] = valid ? keys_in[tile0 + li] : 0u;
     // without vals_in the values are the elements' indices
@@ -296,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
       const uint32_t d = threadIdx.x * kPer + j;
       uint32_t c = 0;
 #pragma unroll
-      for (int w = 0; w < kWaves; w++) {
+      for (int w = 0; w < kRWaves; w++) {
         const uint32_t x = s_whist[w][d];
         s_whist[w][d] = c;
         c += x;
@@ -305,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
       tsum += cnt[j];
     }
     uint32_t total;
-    uint32_t start = block_exclusive_scan(tsum, s_wave, &total);
+    uint32_t start = block_exclusive_scan<kRWaves>(tsum, s_wave, &total);
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) {
       const uint32_t d = threadIdx.x * kPer + j;
@@ -316,8 +350,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    const uint32_t li = (uint32_t)wave * 1024u + (uint32_t)k * 64u + (uint32_t)lane;
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t li = (uint32_t)wave * kRWaveItems + (uint32_t)k * 64u + (uint32_t)lane;
     if (li < tile_n) {
       const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
       const uint32_t pos = s_start[d] + s_whist[wave][d] + rank[k];
@@ -327,8 +361,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_downsweep(const uint32_t *__re
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    const uint32_t p = (uint32_t)k * kBlock + threadIdx.x;
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t p = (uint32_t)k * kRBlock + threadIdx.x;
     if (p < tile_n) {
       const uint32_t kk = s_keys[p];
       const uint32_t d = (mapped_key(kk, km, kmin) >> shift) & kMask;
@@ -480,7 +514,7 @@ void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_
 
 size_t radix_workspace_bytes(size_t n, int key_bits) {
   (void)key_bits;
-  const size_t nb = std::max<size_t>(1, div_up(n, kTileN));
+  const size_t nb = std::max<size_t>(1, div_up(n, kRTile));
   const size_t counts = ((kMaxDigits * nb * sizeof(uint32_t) + 255) / 256) * 256;
   return counts * 2 + scan_levels_bytes(kMaxDigits * nb);
 }
@@ -497,7 +531,7 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
     return false;
   }
   if (key_bits > 32) key_bits = 32;
-  const uint32_t nb = div_up(n, kTileN);
+  const uint32_t nb = div_up(n, kRTile);
   uint8_t *w = static_cast<uint8_t *>(ws);
   const size_t counts_bytes = ((kMaxDigits * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
   uint32_t *counts = reinterpret_cast<uint32_t *>(w);
@@ -510,10 +544,10 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
   const SegGeom none{};
   for (int p = 0; p < passes; p++) {
     const size_t ndig = (size_t)1 << bits[p];
-    k_radix_upsweep<<<nb, kBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
+    k_radix_upsweep<<<nb, kRBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<nb, kBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko, vo, n, shift,
+    downsweep_for(bits[p])<<<nb, kRBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko, vo, n, shift,
                                                  offsets, nb, none, map);
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
@@ -533,7 +567,7 @@ int radix_passes(int key_bits, int max_digit_bits) {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t seg_radix_workspace_bytes(size_t max_elems, uint32_t nseg) {
-  const size_t mb = (size_t)div_up(max_elems, kTileN) + nseg + 1;
+  const size_t mb = (size_t)div_up(max_elems, kRTile) + nseg + 1;
   return align256((size_t)nseg * 4 + 4) + align256(((size_t)nseg + 1) * 4) + align256(mb * 4) +
          2 * align256(256 * mb * 4) + std::max(scan_levels_bytes(nseg), scan_levels_bytes(256 * mb));
 }
@@ -543,7 +577,7 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
                           int max_digit_bits, void *ws) {
   if (nseg == 0 || key_bits <= 0) return false;
   if (key_bits > 32) key_bits = 32;
-  const uint32_t mb = div_up(max_elems, kTileN) + nseg + 1;
+  const uint32_t mb = div_up(max_elems, kRTile) + nseg + 1;
   uint8_t *w = static_cast<uint8_t *>(ws);
   uint32_t *units = reinterpret_cast<uint32_t *>(w);
   w += align256((size_t)nseg * 4 + 4);
@@ -557,7 +591,7 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
   w += align256(256 * (size_t)mb * 4);
   uint8_t *scan_ws = w;
   // tile table: blk0[s] = first tile of segment s, blk0[nseg] = tiles in use
-  k_seg_units<<<div_up(nseg, 256), 256, 0, s>>>(d_seg_start, nseg, (uint32_t)kTileN, units);
+  k_seg_units<<<div_up(nseg, 256), 256, 0, s>>>(d_seg_start, nseg, (uint32_t)kRTile, units);
   DL_HIP(hipGetLastError());
   scan_rec(s, units, blk0, nseg, blk0 + nseg, scan_ws);
   k_unit_owner<<<div_up(nseg, 256), 256, 0, s>>>(blk0, nseg, blk_seg);
@@ -570,10 +604,10 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
   int shift = 0;
   for (int p = 0; p < passes; p++) {
     const size_t ndig = (size_t)1 << bits[p];
-    k_radix_upsweep<<<mb, kBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
+    k_radix_upsweep<<<mb, kRBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<mb, kBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g, RadixKeyMap{});
+    downsweep_for(bits[p])<<<mb, kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
