@@ -82,8 +82,10 @@ struct RastParams {
   uint32_t *cost_key;
   // counting frames: loop iterations per tile row (null: not recorded)
   unsigned long long *row_iters;
-  // tile rows, and the 16-px grid (the 32-px rasterizer's quadrant filter)
-  uint32_t tiles_y, t16x, t16y;
+  // tile rows, the 16-px grid (the 32-px rasterizer's quadrant filter) and
+  // the width of the grid whose lists off indexes (32-px frames, where
+  // tiles_x is the 16-px width)
+  uint32_t tiles_y, t16x, t16y, list_tiles_x;
 };
 
 // True while the scene with this serial number exists (scenes register

@@ -212,9 +212,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
 
 // The autotuner's other rasterizer shapes (single-view frames without
 // counting): tiles of 8 or 16 px in 2x2 groups per 256-thread workgroup, 8-px
-// tiles one wave each, and 32-px tiles whose four 16-px quadrants take a wave
-// each. A group or a 32-px tile is one launch unit: p.order lists units and
-// p.cost_key records one key per unit.
+// tiles one wave each, and 32-px tile lists rasterized one 16-px quadrant
+// per wave. A group or a quadrant is one launch unit: p.order lists units
+// and p.cost_key records one key per unit.
 template <uint32_t TILE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize_2x2(
     const float4 *__restrict__ rec, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pvals,
@@ -252,43 +252,69 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   if (p.cost_key && threadIdx.x == 0) p.cost_key[tile] = work_key(work);
 }
 
-// One 32-px tile per workgroup: wave w rasterizes 16-px quadrant (w & 1,
-// w >> 1) from the tile's list, keeping to the splats whose 16-px range
-// holds the quadrant; the tile closes when all four quadrants have.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize32(
+// 32-px tile lists, one wave64 workgroup per 16-px quadrant (p.tiles_x is
+// the 16-px grid's width): quadrant q reads the list of its 32-px tile
+// (p.list_tiles_x wide) and keeps to the splats whose 16-px range holds it.
+// Closed flags, work keys and the launch order are per quadrant. Four
+// quadrants as the waves of one workgroup measured 680 us against 585 us
+// at 16-px tiles on the 6M-splat 4K frame: the workgroup held its slots
+// until its slowest quadrant ended.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_rasterize32(
     const float4 *__restrict__ rec, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pvals,
     float4 *__restrict__ accum, uint32_t *__restrict__ out_rgba, RastParams p) {
-  __shared__ RastLds lds[4];
-  __shared__ uint32_t s_open[4], s_work[4];
-  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const uint32_t tile = p.order ? p.order[blockIdx.x] : blockIdx.x;
+  __shared__ RastLds lds;
+  const uint32_t q = p.order ? p.order[blockIdx.x] : blockIdx.x;
+  const uint32_t qx = q % p.tiles_x, qy = q / p.tiles_x;
+  const uint32_t tile = (qy >> 1) * p.list_tiles_x + (qx >> 1);
   const uint32_t start = p.has_pairs ? off[tile] : 0u, end = p.has_pairs ? off[tile + 1] : 0u;
-  if (tile_skipped(p, tile, start, end)) {  // the same for every wave of the tile
-    if (p.cost_key && threadIdx.x == 0) p.cost_key[tile] = work_key(0);
-    return;
+  uint32_t work = 0;
+  if (!tile_skipped(p, q, start, end)) {
+    Px16 s;
+    px16_load(s, qx * kTile, qy * kTile, accum, p);
+    work = px16_run_quad<false>(s, qx * kTile, qy * kTile, start, end, rec, pvals, p, lds, qx, qy);
+    bool closing = false;
+    if (p.closed && !p.finalize) {
+      const bool open = px16_open(s);
+      if (threadIdx.x == 0) p.closed[q] = open ? 0 : 1;
+      closing = !open && p.final_on_close;
+    }
+    px16_store(s, p, accum, out_rgba, p.write_accum && !closing, p.finalize || closing);
   }
-  const uint32_t tx = tile % p.tiles_x, ty = tile / p.tiles_x;
-  const uint32_t qx = 2 * tx + (w & 1), qy = 2 * ty + (w >> 1);
-  Px16 s;
-  px16_load(s, qx * kTile, qy * kTile, accum, p);
-  uint32_t iters = 0, consumed = 0;
-  const uint32_t work =
-      px16_run<false, true, true>(s, qx * kTile, qy * kTile, start, end, rec, pvals, p, lds[w], qx, qy, iters, consumed);
-  bool closing = false;
-  if (p.closed && !p.finalize) {
-    const bool open = px16_open(s);
-    if (lane == 0) s_open[w] = open ? 1u : 0u;
-    __syncthreads();
-    const bool any = (s_open[0] | s_open[1] | s_open[2] | s_open[3]) != 0u;
-    if (threadIdx.x == 0) p.closed[tile] = any ? 0 : 1;
-    closing = !any && p.final_on_close;
+  if (p.cost_key && threadIdx.x == 0) p.cost_key[q] = work_key(work);
+}
+
+// Open map of the 32-px grid (cols x rows) from the 16-px quadrants' closed
+// flags (t16x wide, t16y rows): a 32-px tile is open while a quadrant is.
+// Per row, prefix counts of open tiles, as k_open_prefix. One 256-thread
+// block per row (cols <= 256).
+__global__ __launch_bounds__(256) void k_open_prefix_quads(const uint8_t *__restrict__ closed16, uint32_t t16x,
+                                                           uint32_t t16y, uint32_t cols,
+                                                           uint16_t *__restrict__ pre) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t y = blockIdx.x, x = threadIdx.x;
+  uint32_t open = 0;
+  if (x < cols) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      const uint32_t qx = 2 * x + (k & 1), qy = 2 * y + (k >> 1);
+      if (qx < t16x && qy < t16y && !closed16[(size_t)qy * t16x + qx]) open = 1;
+    }
   }
-  px16_store(s, p, accum, out_rgba, p.write_accum && !closing, p.finalize || closing);
-  if (p.cost_key) {
-    if (lane == 0) s_work[w] = work;
-    __syncthreads();
-    if (threadIdx.x == 0) p.cost_key[tile] = work_key(s_work[0] + s_work[1] + s_work[2] + s_work[3]);
+  uint32_t inc = open;
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
   }
+  if (lane == 63) s_w[x >> 6] = inc;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t w = 0; w < (x >> 6); w++) off += s_w[w];
+  inc += off;
+  uint16_t *r = pre + (size_t)y * (cols + 1);
+  if (x == 0) r[0] = 0;
+  if (x < cols) r[x + 1] = (uint16_t)inc;
 }
 
 __global__ void k_zero_u64(unsigned long long *__restrict__ p) {
@@ -325,7 +351,7 @@ void launch_rasterize(bool count, uint32_t tile, uint32_t group, uint32_t units,
   } else {
     if (count) fail(DL_ERR_INTERNAL, "counting frames rasterize 16-px tiles one per workgroup");
     if (tile == 32)
-      k_rasterize32<<<units, 256, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
+      k_rasterize32<<<units, 64, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
     else if (group == 2 && tile == kTile)
       k_rasterize_2x2<16><<<units, 256, 0, s>>>(rec, off, pvals, accum, out_rgba, p);
     else if (group == 2)
@@ -478,8 +504,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   const TileGrid tg = tile_grid(W, H, T);
   const uint32_t tiles_x = tg.tiles_x, tiles_y = tg.tiles_y;
   const uint32_t num_tiles = tiles_x * tiles_y;
-  // workgroups of the rasterizer launch: tiles, or 2x2 groups of them
-  const uint32_t units = G == 2 ? div_up(tiles_x, 2) * div_up(tiles_y, 2) : num_tiles;
+  // workgroups of the rasterizer launch: tiles, 2x2 groups of them, or the
+  // 16-px quadrants of 32-px tiles (which also carry the closed flags)
+  const uint32_t num16 = tg.t16x * tg.t16y;
+  const uint32_t units = G == 2 ? div_up(tiles_x, 2) * div_up(tiles_y, 2) : T == 32 ? num16 : num_tiles;
+  const uint32_t flag_tiles = T == 32 ? num16 : num_tiles;
   const uint32_t shape = T * 4 + G;
   const size_t npx = (size_t)W * H;
   stats = dl_render_stats{};
@@ -531,6 +560,8 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
   rp.tiles_y = tiles_y;
   rp.t16x = tg.t16x;
   rp.t16y = tg.t16y;
+  rp.list_tiles_x = tiles_x;
+  if (T == 32) rp.tiles_x = tg.t16x;  // the quadrant grid
 
   const ProjUniforms u = make_uniforms(cam, max_cov, T);
 
@@ -718,7 +749,7 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     ranges.reserve(((size_t)num_tiles + 1) * 4);
     if (chunks > 1 || need_accum) accum.reserve(npx * 16);
     if (track_closed) {
-      closed.reserve(num_tiles);
+      closed.reserve(flag_tiles);
       open_pre.reserve((size_t)tiles_y * (tiles_x + 1) * 2);
       // no clear: the first pass is not the last, reads no flag and writes
       // every tile's
@@ -868,7 +899,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       }
       accum_valid = true;
       if (track_closed && !last) {
-        k_open_prefix<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tiles_x, open_pre.as<uint16_t>());
+        if (T == 32)
+          k_open_prefix_quads<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tg.t16x, tg.t16y, tiles_x,
+                                                      open_pre.as<uint16_t>());
+        else
+          k_open_prefix<<<tiles_y, 256, 0, s>>>(closed.as<uint8_t>(), tiles_x, open_pre.as<uint16_t>());
         DL_HIP(hipGetLastError());
       }
     }

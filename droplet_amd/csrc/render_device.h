@@ -392,12 +392,14 @@ __device__ inline uint32_t work_key(uint32_t w) {
   return 62u - min(62u, 2u * lz + half);
 }
 
-// LDS of one rasterizer wave: a batch of kStage staged splat records and the
-// lanes' 16 accumulators while the batch is staged.
+// LDS of one rasterizer wave: a batch of kStage staged splat records, the
+// lanes' 16 accumulators while the batch is staged, and a window of splat
+// ids (32-px tiles).
 constexpr uint32_t kStage = 32;
 struct RastLds {
   float4 rec[kStage * 3];
   float acc[16 * 64];
+  uint32_t ids[64];  // the 32-px rasterizer's filtered window
 };
 
 // Synchronisation of a staging batch. A one-wave workgroup takes the
@@ -505,123 +507,170 @@ __device__ inline void blend_row(const ColTerms &c, const float4 a, const float4
   T = T * (1.0f - alpha);
 }
 
+// Strips (0..3) of a wave's 16x16 region that still hold a live pixel.
+__device__ inline uint32_t px16_live(const Px16 &s) {
+  uint32_t live = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) live |= (__ballot(!(s.T[k] < kMinT)) != 0 ? 1u : 0u) << k;
+  return live;
+}
+
+// One staged batch for a wave's 16x16 region with rows from oy: slots
+// c < cnt (cnt <= 32) hold splat get_j(c), in list order. The records are
+// staged in LDS and read back as broadcasts; the strips each staged splat
+// can reach are four 32-bit ballots, and the wave visits only the splats
+// with a bit set (COUNT: every staged splat, counting the reference's loop
+// iterations of the live pixels). Every pixel runs exactly the reference's
+// loop body in the reference's order. Returns 8 plus the visited (splat,
+// strip) bits.
+template <bool COUNT, bool MULTI, class GetJ>
+__device__ inline uint32_t px16_batch(Px16 &s, uint32_t oy, double tile_x0, double tile_x1, uint32_t cnt,
+                                      uint32_t strips_live, GetJ get_j, const float4 *__restrict__ rec, RastLds &L,
+                                      uint32_t &iters) {
+  const int lane = threadIdx.x & 63;
+  // Staging: the accumulators wait in LDS while the f64 strip tests run,
+  // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
+  // splats, two lanes per splat with two strips each, so that the LDS
+  // of a wave stays under 6 KB. Both bounds together allow 6 waves per
+  // SIMD.
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    L.acc[(4 * k + 0) * 64 + lane] = s.cr[k];
+    L.acc[(4 * k + 1) * 64 + lane] = s.cg[k];
+    L.acc[(4 * k + 2) * 64 + lane] = s.cb[k];
+    L.acc[(4 * k + 3) * 64 + lane] = s.T[k];
+  }
+  const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
+  uint32_t m = 0;
+  if (slot < cnt) {
+    const uint32_t j = get_j(slot);
+    const float4 a = rec[3 * (size_t)j + 0];
+    const float4 b = rec[3 * (size_t)j + 1];
+    const float4 col = rec[3 * (size_t)j + 2];
+    if (half == 0) {
+      L.rec[3 * slot + 0] = a;
+      L.rec[3 * slot + 1] = b;
+      L.rec[3 * slot + 2] = col;
+    }
+    // one strip at a time (unrolled, the f64 temporaries of several
+    // strips take more VGPRs)
+#pragma unroll 1
+    for (uint32_t q = 0; q < 2; q++) {
+      const uint32_t k = 2 * half + q;
+      const double y0 = (double)(oy + 4 * k) + 0.5;
+      if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+    }
+  }
+  // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
+  // strips 0 and 1, lanes 32-63 strips 2 and 3)
+  const uint64_t b02 = __ballot((m & 0x5u) != 0u), b13 = __ballot((m & 0xAu) != 0u);
+  const uint32_t s0 = (strips_live & 1u) ? (uint32_t)b02 : 0u;
+  const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
+  const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
+  const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
+  const uint32_t work = 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1) +
+                                        __builtin_popcount(s2) + __builtin_popcount(s3));
+  stage_sync<MULTI>();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    s.cr[k] = L.acc[(4 * k + 0) * 64 + lane];
+    s.cg[k] = L.acc[(4 * k + 1) * 64 + lane];
+    s.cb[k] = L.acc[(4 * k + 2) * 64 + lane];
+    s.T[k] = L.acc[(4 * k + 3) * 64 + lane];
+  }
+  uint32_t todo = COUNT ? (cnt == 32u ? ~0u : (1u << cnt) - 1u) : (s0 | s1 | s2 | s3);
+  while (todo) {
+    const uint32_t c = (uint32_t)__builtin_ctz(todo);
+    todo &= todo - 1u;
+    if (COUNT)
+      iters += (uint32_t)!(s.T[0] < kMinT) + (uint32_t)!(s.T[1] < kMinT) + (uint32_t)!(s.T[2] < kMinT) +
+               (uint32_t)!(s.T[3] < kMinT);
+    const uint32_t mm = ((s0 >> c) & 1u) | (((s1 >> c) & 1u) << 1) | (((s2 >> c) & 1u) << 2) |
+                        (((s3 >> c) & 1u) << 3);
+    if (mm == 0) continue;
+    const float4 a = L.rec[3 * c + 0];
+    const float4 b = L.rec[3 * c + 1];
+    const float4 col = L.rec[3 * c + 2];
+    // the strip bits are uniform over the wave (scalar branches); the
+    // column test and the saturation test are per lane
+    const ColTerms ct = col_terms(a, b, s.pxf);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (!((mm >> k) & 1u)) continue;
+      if (s.T[k] < kMinT || ct.out) continue;
+      blend_row(ct, a, b, col, s.pyf[k], s.cr[k], s.cg[k], s.cb[k], s.T[k]);
+    }
+  }
+  stage_sync<MULTI>();
+  return work;
+}
+
 // Front to back over list entries [start, end) of pvals (which index rec)
-// for one wave's 16x16 region at (ox, oy). The splats are staged 32 at a
-// time in LDS; the strips each staged splat can reach are four 32-bit
-// ballots, and the wave visits only the splats with a bit set. Every pixel
-// runs exactly the reference's loop body in the reference's order; a pixel
-// stops when T < 1e-4 and the wave stops when all its pixels have. QUAD: the
-// region is the 16-px tile (q16x, q16y) of a 32-px tile's list, and only
-// the splats whose 16-px range holds that tile reach it (the reference's
-// list of that tile). Returns the staged batches x 8 plus the visited
-// (splat, strip) bits; consumed counts the entries read before the stop.
-template <bool COUNT, bool MULTI, bool QUAD>
+// for one wave's 16x16 region at (ox, oy), 32 entries per batch; a pixel
+// stops when T < 1e-4 and the wave stops when all its pixels have. Returns
+// the sum of the batches' work; consumed counts the entries read before
+// the stop.
+template <bool COUNT, bool MULTI>
 __device__ inline uint32_t px16_run(Px16 &s, uint32_t ox, uint32_t oy, uint32_t start, uint32_t end,
-                                    const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
-                                    const RastParams &p, RastLds &L, uint32_t q16x, uint32_t q16y,
+                                    const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals, RastLds &L,
                                     uint32_t &iters, uint32_t &consumed) {
   uint32_t work = 0;
   consumed = 0;
   if (start >= end) return 0;
-  const int lane = threadIdx.x & 63;
   const double tile_x0 = (double)ox + 0.5, tile_x1 = tile_x0 + 15.0;
   uint32_t stop = end;
   for (uint32_t base = start; base < end; base += kStage) {
-    uint32_t strips_live = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) strips_live |= (__ballot(!(s.T[k] < kMinT)) != 0 ? 1u : 0u) << k;
+    const uint32_t strips_live = px16_live(s);
     if (strips_live == 0) {
       stop = base;
       break;
     }
-    // Staging: the accumulators wait in LDS while the f64 strip tests run,
-    // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
-    // splats, two lanes per splat with two strips each, so that the LDS
-    // of a wave stays under 6 KB. Both bounds together allow 6 waves per
-    // SIMD.
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      L.acc[(4 * k + 0) * 64 + lane] = s.cr[k];
-      L.acc[(4 * k + 1) * 64 + lane] = s.cg[k];
-      L.acc[(4 * k + 2) * 64 + lane] = s.cb[k];
-      L.acc[(4 * k + 3) * 64 + lane] = s.T[k];
-    }
-    const uint32_t slot = (uint32_t)lane & (kStage - 1u), half = (uint32_t)lane / kStage;
-    const uint32_t i = base + slot;
-    uint32_t m = 0;
-    if (i < end) {
-      const uint32_t j = pvals[i];
-      const float4 a = rec[3 * (size_t)j + 0];
-      const float4 b = rec[3 * (size_t)j + 1];
-      const float4 col = rec[3 * (size_t)j + 2];
-      if (half == 0) {
-        L.rec[3 * slot + 0] = a;
-        L.rec[3 * slot + 1] = b;
-        L.rec[3 * slot + 2] = col;
-      }
-      bool listed = true;
-      if constexpr (QUAD) {
-        uint32_t x0, x1, y0, y1;
-        tile_range16(a.x, a.y, a.z, p.t16x, p.t16y, x0, x1, y0, y1);
-        listed = q16x >= x0 && q16x <= x1 && q16y >= y0 && q16y <= y1;
-      }
-      // one strip at a time (unrolled, the f64 temporaries of several
-      // strips take more VGPRs)
-      if (listed) {
-#pragma unroll 1
-        for (uint32_t q = 0; q < 2; q++) {
-          const uint32_t k = 2 * half + q;
-          const double y0 = (double)(oy + 4 * k) + 0.5;
-          if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
-        }
-      }
-    }
-    // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
-    // strips 0 and 1, lanes 32-63 strips 2 and 3)
-    const uint64_t b02 = __ballot((m & 0x5u) != 0u), b13 = __ballot((m & 0xAu) != 0u);
-    const uint32_t s0 = (strips_live & 1u) ? (uint32_t)b02 : 0u;
-    const uint32_t s1 = (strips_live & 2u) ? (uint32_t)b13 : 0u;
-    const uint32_t s2 = (strips_live & 4u) ? (uint32_t)(b02 >> 32) : 0u;
-    const uint32_t s3 = (strips_live & 8u) ? (uint32_t)(b13 >> 32) : 0u;
-    work += 8u + (uint32_t)(__builtin_popcount(s0) + __builtin_popcount(s1) + __builtin_popcount(s2) +
-                            __builtin_popcount(s3));
-    stage_sync<MULTI>();
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      s.cr[k] = L.acc[(4 * k + 0) * 64 + lane];
-      s.cg[k] = L.acc[(4 * k + 1) * 64 + lane];
-      s.cb[k] = L.acc[(4 * k + 2) * 64 + lane];
-      s.T[k] = L.acc[(4 * k + 3) * 64 + lane];
-    }
-    const uint32_t cnt = min(kStage, end - base);
-    // COUNT visits every staged splat to count the reference's loop
-    // iterations; otherwise only the splats with a strip bit
-    uint32_t todo = COUNT ? (cnt == 32u ? ~0u : (1u << cnt) - 1u) : (s0 | s1 | s2 | s3);
-    while (todo) {
-      const uint32_t c = (uint32_t)__builtin_ctz(todo);
-      todo &= todo - 1u;
-      if (COUNT)
-        iters += (uint32_t)!(s.T[0] < kMinT) + (uint32_t)!(s.T[1] < kMinT) + (uint32_t)!(s.T[2] < kMinT) +
-                 (uint32_t)!(s.T[3] < kMinT);
-      const uint32_t mm = ((s0 >> c) & 1u) | (((s1 >> c) & 1u) << 1) | (((s2 >> c) & 1u) << 2) |
-                          (((s3 >> c) & 1u) << 3);
-      if (mm == 0) continue;
-      const float4 a = L.rec[3 * c + 0];
-      const float4 b = L.rec[3 * c + 1];
-      const float4 col = L.rec[3 * c + 2];
-      // the strip bits are uniform over the wave (scalar branches); the
-      // column test and the saturation test are per lane
-      const ColTerms ct = col_terms(a, b, s.pxf);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (!((mm >> k) & 1u)) continue;
-        if (s.T[k] < kMinT || ct.out) continue;
-        blend_row(ct, a, b, col, s.pyf[k], s.cr[k], s.cg[k], s.cb[k], s.T[k]);
-      }
-    }
-    stage_sync<MULTI>();
+    work += px16_batch<COUNT, MULTI>(s, oy, tile_x0, tile_x1, min(kStage, end - base), strips_live,
+                                     [&](uint32_t c) { return pvals[base + c]; }, rec, L, iters);
   }
   consumed = stop - start;
+  return work;
+}
+
+// The 16-px quadrant (q16x, q16y) of a 32-px tile whose list is entries
+// [start, end) of pvals: windows of 64 entries are read for the splats'
+// centres and radii, the entries whose 16-px range holds the quadrant (the
+// reference's list of that 16-px tile) are kept in list order by ballot,
+// and only those are staged, 32 per batch. Returns the batches' work.
+template <bool MULTI>
+__device__ inline uint32_t px16_run_quad(Px16 &s, uint32_t ox, uint32_t oy, uint32_t start, uint32_t end,
+                                         const float4 *__restrict__ rec, const uint32_t *__restrict__ pvals,
+                                         const RastParams &p, RastLds &L, uint32_t q16x, uint32_t q16y) {
+  uint32_t work = 0, iters = 0;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const double tile_x0 = (double)ox + 0.5, tile_x1 = tile_x0 + 15.0;
+  for (uint32_t base = start; base < end; base += 64u) {
+    uint32_t strips_live = px16_live(s);
+    if (strips_live == 0) break;
+    const uint32_t i = base + (uint32_t)lane;
+    bool listed = false;
+    uint32_t j = 0;
+    if (i < end) {
+      j = pvals[i];
+      const float4 a = rec[3 * (size_t)j];
+      uint32_t x0, x1, y0, y1;
+      tile_range16(a.x, a.y, a.z, p.t16x, p.t16y, x0, x1, y0, y1);
+      listed = q16x >= x0 && q16x <= x1 && q16y >= y0 && q16y <= y1;
+    }
+    const uint64_t lm = __ballot(listed);
+    if (listed) L.ids[__popcll(lm & lt)] = j;
+    stage_sync<MULTI>();
+    const uint32_t nl = (uint32_t)__popcll(lm);
+    for (uint32_t c0 = 0; c0 < nl; c0 += kStage) {
+      if (c0 > 0) {
+        strips_live = px16_live(s);
+        if (strips_live == 0) break;
+      }
+      work += px16_batch<false, MULTI>(s, oy, tile_x0, tile_x1, min(kStage, nl - c0), strips_live,
+                                       [&](uint32_t c) { return L.ids[c0 + c]; }, rec, L, iters);
+    }
+  }
   return work;
 }
 
@@ -647,8 +696,7 @@ __device__ inline uint32_t rasterize_tile16(uint32_t tx, uint32_t ty, uint32_t v
   Px16 s;
   px16_load(s, tx * kTile, ty * kTile, accum, p);
   uint32_t iters = 0, consumed = 0;
-  const uint32_t work =
-      px16_run<COUNT, MULTI, false>(s, tx * kTile, ty * kTile, start, end, rec, pvals, p, L, 0, 0, iters, consumed);
+  const uint32_t work = px16_run<COUNT, MULTI>(s, tx * kTile, ty * kTile, start, end, rec, pvals, L, iters, consumed);
   bool closing = false;
   if (p.closed && !p.finalize) {
     const bool open = px16_open(s);
