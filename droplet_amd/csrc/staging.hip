@@ -2,9 +2,9 @@
 //
 // A hipMemcpy from or to pageable memory goes through the runtime's own
 // bounce buffers one piece at a time (the 33 MB RGBA8 image of a 4K frame
-// read back at ~3 GB/s). Here a context keeps two pinned chunks
+// read back at ~3 GB/s). Here a context keeps a few pinned chunks
 // (hipHostMalloc): the DMA engine moves one chunk while host threads copy
-// the other between the pinned chunk and the caller's memory, so the DMA and
+// another between a pinned chunk and the caller's memory, so the DMA and
 // the host copy overlap and the host copy runs on several cores.
 #include <hip/hip_runtime.h>
 
@@ -104,12 +104,13 @@ class CopyPool {
   bool stop_ = false;
 };
 
-constexpr size_t kStageChunk = size_t(8) << 20;
+constexpr size_t kStageChunk = size_t(DL_STAGE_CHUNK_MB) << 20;
+constexpr int kSlots = HostStaging::kSlots;
 
 }  // namespace
 
 HostStaging::~HostStaging() {
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < kSlots; k++) {
     if (ev_[k]) (void)hipEventDestroy(ev_[k]);
     if (buf_[k]) (void)hipHostFree(buf_[k]);
   }
@@ -117,7 +118,7 @@ HostStaging::~HostStaging() {
 
 void HostStaging::ensure() {
   if (buf_[0]) return;
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < kSlots; k++) {
     DL_HIP(hipHostMalloc(&buf_[k], kStageChunk, hipHostMallocDefault));
     DL_HIP(hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming));
   }
@@ -133,11 +134,11 @@ void HostStaging::to_device(void *dst, const void *src, size_t bytes, hipStream_
   ensure();
   const uint8_t *sp = static_cast<const uint8_t *>(src);
   uint8_t *dp = static_cast<uint8_t *>(dst);
-  bool used[2] = {false, false};
+  bool used[kSlots] = {};
   for (size_t off = 0, k = 0; off < bytes; off += kStageChunk, k++) {
-    const int slot = (int)(k & 1);
+    const int slot = (int)(k % kSlots);
     const size_t len = std::min(kStageChunk, bytes - off);
-    // the DMA that read this chunk two steps ago must be done before it is refilled
+    // the DMA that read this chunk kSlots steps ago must be done before it is refilled
     if (used[slot]) DL_HIP(hipEventSynchronize(ev_[slot]));
     CopyPool::get().parallel_copy(buf_[slot], sp + off, len);
     DL_HIP(hipMemcpyAsync(dp + off, buf_[slot], len, hipMemcpyHostToDevice, s));
@@ -145,8 +146,8 @@ void HostStaging::to_device(void *dst, const void *src, size_t bytes, hipStream_
     used[slot] = true;
   }
   // the caller's memory may change after return; the pinned chunks are the
-  // DMA's source, so the last two must be read before they are reused
-  for (int k = 0; k < 2; k++)
+  // DMA's source, so the last ones must be read before they are reused
+  for (int k = 0; k < kSlots; k++)
     if (used[k]) DL_HIP(hipEventSynchronize(ev_[k]));
 }
 
@@ -162,18 +163,18 @@ void HostStaging::to_host(void *dst, const void *src, size_t bytes, hipStream_t 
   uint8_t *dp = static_cast<uint8_t *>(dst);
   const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
   auto issue = [&](size_t k) {
-    const int slot = (int)(k & 1);
+    const int slot = (int)(k % kSlots);
     const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
     DL_HIP(hipMemcpyAsync(buf_[slot], sp + off, len, hipMemcpyDeviceToHost, s));
     DL_HIP(hipEventRecord(ev_[slot], s));
   };
-  issue(0);
+  for (size_t k = 0; k + 1 < (size_t)kSlots && k < chunks; k++) issue(k);
   for (size_t k = 0; k < chunks; k++) {
-    // chunk k + 1 lands in the other chunk while the host copies chunk k out;
-    // the other chunk's previous contents (chunk k - 1) were copied out in
-    // the last step
-    if (k + 1 < chunks) issue(k + 1);
-    const int slot = (int)(k & 1);
+    // chunks k + 1 .. k + kSlots - 1 land in the other slots while the host
+    // copies chunk k out; the slot of chunk k + kSlots - 1 held chunk k - 1,
+    // which the last step copied out
+    if (k + kSlots - 1 < chunks) issue(k + kSlots - 1);
+    const int slot = (int)(k % kSlots);
     const size_t off = k * kStageChunk, len = std::min(kStageChunk, bytes - off);
     DL_HIP(hipEventSynchronize(ev_[slot]));
     CopyPool::get().parallel_copy(dp + off, buf_[slot], len);
