@@ -227,13 +227,26 @@ __global__ __launch_bounds__(256) void k_vox_emit(const VGauss *__restrict__ rec
 }
 
 // evalGaussian (voxel/shaders/voxelize.ts:62-84), reference order of operations
-__device__ inline float eval_gauss(float vx, float vy, float vz, float half, const VGauss &g) {
+// One Gaussian's contribution at voxel centre (vx, vy, vz): the reference's
+// per-voxel body (voxel/shaders/voxelize.ts:62-84), with the record as
+// staged in LDS, where ex, ey, ez hold ex + half, ey + half, ez + half (the
+// same f32 sums, made once per staged Gaussian), and lo/hi the voxel's
+// centre -+ half. Exact rewrites, none of which changes a mask bit:
+// - the three box tests are evaluated together;
+// - the clamp uses min/max instructions: they differ from WGSL's
+//   compare-select only in the sign of a zero result when both operands
+//   are zero, which the squares of the distance remove, and the bound
+//   operands are finite, so a NaN centre gives the same NaN either way;
+// - e^x of x = -dd/2 <= 0 (or NaN) is dl_expf_nonpos after clamping x at
+//   -104 as dl_expf does, bit-identical to dl_expf there.
+__device__ inline float eval_gauss(float vx, float vy, float vz, const float lo[3], const float hi[3],
+                                   const VGauss &g) {
   const float d0 = vx - g.px, d1 = vy - g.py, d2 = vz - g.pz;
-  if (__builtin_fabsf(d0) > g.ex + half || __builtin_fabsf(d1) > g.ey + half || __builtin_fabsf(d2) > g.ez + half)
-    return 0.0f;
-  const float c0 = wmin(wmax(g.px, vx - half), vx + half) - g.px;
-  const float c1 = wmin(wmax(g.py, vy - half), vy + half) - g.py;
-  const float c2 = wmin(wmax(g.pz, vz - half), vz + half) - g.pz;
+  const bool out = (__builtin_fabsf(d0) > g.ex) | (__builtin_fabsf(d1) > g.ey) | (__builtin_fabsf(d2) > g.ez);
+  if (out) return 0.0f;
+  const float c0 = __builtin_fminf(__builtin_fmaxf(g.px, lo[0]), hi[0]) - g.px;
+  const float c1 = __builtin_fminf(__builtin_fmaxf(g.py, lo[1]), hi[1]) - g.py;
+  const float c2 = __builtin_fminf(__builtin_fmaxf(g.pz, lo[2]), hi[2]) - g.pz;
   const float qx = -g.qx, qy = -g.qy, qz = -g.qz, qw = g.qw;
   const float t0 = 2.0f * (qy * c2 - qz * c1);
   const float t1 = 2.0f * (qz * c0 - qx * c2);
@@ -243,7 +256,8 @@ __device__ inline float eval_gauss(float vx, float vy, float vz, float half, con
   const float l2 = (c2 + qw * t2) + (qx * t1 - qy * t0);
   const float s0 = l0 * g.is0, s1 = l1 * g.is1, s2 = l2 * g.is2;
   const float dd = (s0 * s0 + s1 * s1) + s2 * s2;
-  return g.opacity * dl_expf((-0.5f) * dd);
+  const float x = (-0.5f) * dd;
+  return g.opacity * dl_expf_nonpos(x < -104.0f ? -104.0f : x);
 }
 
 // One wave64 per block; lane = linear voxel index z*16 + y*4 + x, so the
@@ -271,16 +285,28 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
     const float boff = ((float)(a3[k] - b0) * 4.0f) * G.res;        // voxel/shaders/voxelize.ts:104
     vc[k] = (bmin + boff) + (((float)l3[k] + 0.5f) * G.res);       // voxel/shaders/voxelize.ts:105
   }
+  float lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = vc[k] - G.half;
+    hi[k] = vc[k] + G.half;
+  }
   float sigma = 0.0f;
   for (uint32_t base = start; base < end; base += 64) {
     if (__ballot(sigma < 7.0f) == 0) break;
     const uint32_t i = base + lane;
-    if (i < end) s_g[lane] = rec[vals[i]];
+    if (i < end) {
+      VGauss g = rec[vals[i]];
+      g.ex = g.ex + G.half;
+      g.ey = g.ey + G.half;
+      g.ez = g.ez + G.half;
+      s_g[lane] = g;
+    }
     __syncthreads();
     const uint32_t cnt = min(64u, end - base);
     if (sigma < 7.0f) {
       for (uint32_t c = 0; c < cnt; c++) {
-        sigma = sigma + eval_gauss(vc[0], vc[1], vc[2], G.half, s_g[c]);
+        sigma = sigma + eval_gauss(vc[0], vc[1], vc[2], lo, hi, s_g[c]);
         if (sigma >= 7.0f) break;
       }
     }
