@@ -418,7 +418,7 @@ void flood_impl(Context &ctx, const VoxelResult &in, bool fill, const double *po
   } events{e0, e1};
   DL_HIP(hipEventRecord(e0, s));
   if (in.total == 0) {
-    voxel_compact(s, out, e1);
+    voxel_compact(ctx, s, out, e1);
     DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
     return;
   }
@@ -469,7 +469,7 @@ void flood_impl(Context &ctx, const VoxelResult &in, bool fill, const double *po
   out.masks.reserve(total * 8);
   k_complement<<<fill_blocks, 256, 0, s>>>(reach.as<uint2>(), out.masks.as<uint2>(), total);
   DL_HIP(hipGetLastError());
-  voxel_compact(s, out, e1);
+  voxel_compact(ctx, s, out, e1);
   DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
 }
 
@@ -538,7 +538,7 @@ int dl_voxel_result_create(dl_context *ctx, const double bounds_min[3], const do
         DL_HIP(hipStreamSynchronize(s));
       }
     }
-    voxel_compact(s, v, nullptr);
+    voxel_compact(c, s, v, nullptr);
     *out = r.release();
   });
 }

@@ -99,6 +99,26 @@ class HostStaging {
   hipEvent_t ev_[kSlots] = {};
 };
 
+// Device scratch of the voxelizer and of the voxel grid's compaction, kept
+// by the context between calls (grow-only): allocating ~0.9 GB of pair
+// arrays and workspaces per call of the 1M-Gaussian workload left the GPU
+// idle for ~2 ms of each call. Calls on one context run one at a time.
+struct VoxelScratch {
+  VoxelScratch() = default;
+  ~VoxelScratch();
+  VoxelScratch(const VoxelScratch &) = delete;
+  VoxelScratch &operator=(const VoxelScratch &) = delete;
+  // voxelize: Gaussian arrays and records, per-Gaussian pair counts and
+  // offsets, the (block, Gaussian) pairs and their sort, block ranges
+  DeviceBuffer pos, rot, ls, op, rec, partials, counts, offs, scan_ws, small;
+  DeviceBuffer keys, vals, tkeys, tvals, sort_ws, ranges;
+  // compaction of non-empty blocks
+  DeviceBuffer flags, fpos, omask, oidx, cscan_ws, csmall;
+  // timing events of a voxelize call (created on first use)
+  hipEvent_t ev[4] = {};
+  void ensure_events();
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -106,6 +126,7 @@ struct Context {
   std::string arch;
   // scratch for the standalone sort entry point
   DeviceBuffer sort_keys, sort_vals, sort_tmp_keys, sort_tmp_vals, sort_ws;
+  VoxelScratch vox;
   // pinned staging of large host <-> device copies
   HostStaging staging;
   void make_current() const;

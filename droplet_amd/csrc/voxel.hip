@@ -366,7 +366,17 @@ double js_round(double x) {
 
 }  // namespace
 
-void voxel_compact(hipStream_t s, VoxelResult &r, hipEvent_t done) {
+VoxelScratch::~VoxelScratch() {
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void VoxelScratch::ensure_events() {
+  for (auto &e : ev)
+    if (!e) DL_HIP(hipEventCreate(&e));
+}
+
+void voxel_compact(Context &c, hipStream_t s, VoxelResult &r, hipEvent_t done) {
   r.count = 0;
   r.host_ready = false;
   if (r.total == 0) {
@@ -377,7 +387,8 @@ void voxel_compact(hipStream_t s, VoxelResult &r, hipEvent_t done) {
   if (r.total >= 0xFFFFFFFFull) fail(DL_ERR_CAPACITY, "voxel grid has more than 2^32-1 blocks");
   const uint32_t total = (uint32_t)r.total;
   GridDims G = {{r.nb[0], r.nb[1], r.nb[2]}};
-  DeviceBuffer flags, fpos, omask, oidx, scan_ws, small;
+  DeviceBuffer &flags = c.vox.flags, &fpos = c.vox.fpos, &omask = c.vox.omask, &oidx = c.vox.oidx;
+  DeviceBuffer &scan_ws = c.vox.cscan_ws, &small = c.vox.csmall;
   flags.reserve((size_t)total * 4);
   fpos.reserve((size_t)total * 4);
   omask.reserve((size_t)total * 8);
@@ -426,11 +437,11 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   out.device = ctx.device;
   out.vres = vres;
   hipStream_t s = ctx.stream;
-  hipEvent_t e0, e1;
-  DL_HIP(hipEventCreate(&e0));
-  DL_HIP(hipEventCreate(&e1));
+  VoxelScratch &W = ctx.vox;
+  W.ensure_events();
+  hipEvent_t e0 = W.ev[0], e1 = W.ev[1];
 
-  DeviceBuffer pos, rot, ls, op, rec, partials;
+  DeviceBuffer &pos = W.pos, &rot = W.rot, &ls = W.ls, &op = W.op, &rec = W.rec, &partials = W.partials;
   std::vector<double> part;
   const uint32_t nprep = std::max<uint32_t>(1, div_up(n, kPrepBlock));
   if (n > 0) {
@@ -493,8 +504,6 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     DL_HIP(hipEventRecord(e1, s));
     DL_HIP(hipStreamSynchronize(s));
     DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return;
   }
   if (total64 >= 0xFFFFFFFFull || nb[0] > 0x7FFFFFFF || nb[1] > 0x7FFFFFFF || nb[2] > 0x7FFFFFFF)
@@ -514,7 +523,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   G.half = G.res * 0.5f;
   G.cutoff = (float)cutoff;
 
-  DeviceBuffer counts, offs, scan_ws, small;
+  DeviceBuffer &counts = W.counts, &offs = W.offs, &scan_ws = W.scan_ws, &small = W.small;
   counts.reserve((size_t)n * 4);
   offs.reserve((size_t)n * 4);
   scan_ws.reserve(scan_workspace_bytes(n));
@@ -524,16 +533,18 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   DL_HIP(hipMemsetAsync(d_total64, 0, sizeof(unsigned long long), s));
   k_vox_count<<<div_up(n, 256), 256, 0, s>>>(rec.as<VGauss>(), n, G, counts.as<uint32_t>(), d_total64);
   DL_HIP(hipGetLastError());
-  unsigned long long total_pairs = 0;
-  DL_HIP(hipMemcpyAsync(&total_pairs, d_total64, sizeof(total_pairs), hipMemcpyDeviceToHost, s));
-  DL_HIP(hipStreamSynchronize(s));
-  if (total_pairs > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (block, Gaussian) pairs");
+  // the u64 total and the scan's u32 total in one read: a total above 2^32
+  // fails before the (wrapped) scan is used
   exclusive_scan_u32(s, counts.as<uint32_t>(), offs.as<uint32_t>(), n, small.as<uint32_t>(), scan_ws.get());
-  uint32_t npairs = 0;
-  DL_HIP(hipMemcpyAsync(&npairs, small.get(), 4, hipMemcpyDeviceToHost, s));
+  uint32_t head[24];
+  DL_HIP(hipMemcpyAsync(head, small.get(), sizeof(head), hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
+  const unsigned long long total_pairs = (unsigned long long)head[16] | ((unsigned long long)head[17] << 32);
+  if (total_pairs > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (block, Gaussian) pairs");
+  const uint32_t npairs = head[0];
 
-  DeviceBuffer keys, vals, tkeys, tvals, sort_ws, ranges;
+  DeviceBuffer &keys = W.keys, &vals = W.vals, &tkeys = W.tkeys, &tvals = W.tvals, &sort_ws = W.sort_ws;
+  DeviceBuffer &ranges = W.ranges;
   DeviceBuffer &masks = out.masks;
   const uint32_t key_bits = std::max<uint32_t>(1, bit_width_u32(total - 1));
   keys.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
@@ -556,9 +567,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     }
   }
   bin_ranges(s, skeys, npairs, total, ranges.as<uint32_t>());
-  hipEvent_t ev_a, ev_b;
-  DL_HIP(hipEventCreate(&ev_a));
-  DL_HIP(hipEventCreate(&ev_b));
+  hipEvent_t ev_a = W.ev[2], ev_b = W.ev[3];
   DL_HIP(hipEventRecord(ev_a, s));
   k_vox_eval<<<total, 64, 0, s>>>(rec.as<VGauss>(), ranges.as<uint32_t>(), svals, G, masks.as<uint2>());
   DL_HIP(hipGetLastError());
@@ -566,13 +575,9 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   out.block_pairs = npairs;
 
   // reference insertion order + compaction of non-empty blocks
-  voxel_compact(s, out, e1);
+  voxel_compact(ctx, s, out, e1);
   DL_HIP(hipEventElapsedTime(&out.device_ms, e0, e1));
   DL_HIP(hipEventElapsedTime(&out.eval_ms, ev_a, ev_b));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(ev_a);
-  (void)hipEventDestroy(ev_b);
 }
 
 }  // namespace dl

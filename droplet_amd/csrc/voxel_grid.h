@@ -34,9 +34,9 @@ struct VoxelResult {
 };
 
 // Builds r.idx / r.lo / r.hi and r.count from r.masks (r.nb and r.total
-// set). Records `done` (if not null) after the last kernel, then waits for
-// the stream.
-void voxel_compact(hipStream_t s, VoxelResult &r, hipEvent_t done);
+// set), with the context's compaction scratch. Records `done` (if not null)
+// after the last kernel, then waits for the stream.
+void voxel_compact(Context &c, hipStream_t s, VoxelResult &r, hipEvent_t done);
 
 // Copies the non-empty block lists to the host vectors once.
 void voxel_lists_to_host(const VoxelResult &r);
