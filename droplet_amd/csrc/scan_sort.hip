@@ -262,13 +262,23 @@ __global__ __launch_bounds__(kRBlock) void k_radix_upsweep(const uint32_t *__res
   for (uint32_t d = threadIdx.x; d <= mask; d += kRBlock) counts[ts.cbase + (size_t)d * ts.cstride] = s_hist[d];
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup i to XCD i mod 8).
+// With xq > 0 (launch of 8 xq workgroups) workgroup i sorts tile
+// (i mod 8) xq + i / 8, so each XCD sorts a contiguous run of tiles: the
+// runs of one digit that neighbouring tiles write are adjacent in the
+// output, and written through one L2 they fill whole lines.
+__device__ inline uint32_t xcd_tile(uint32_t xq) {
+  return xq ? (blockIdx.x & 7u) * xq + (blockIdx.x >> 3) : blockIdx.x;
+}
+
 template <int BITS>
 __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
                                                             const uint32_t *__restrict__ vals_in,
                                                             uint32_t *__restrict__ keys_out,
                                                             uint32_t *__restrict__ vals_out, size_t n, int shift,
                                                             const uint32_t *__restrict__ offsets,
-                                                            uint32_t nblocks, SegGeom g, RadixKeyMap km) {
+                                                            uint32_t nblocks, SegGeom g, RadixKeyMap km,
+                                                            uint32_t xq) {
   constexpr uint32_t kDigits = 1u << BITS;
   constexpr uint32_t kMask = kDigits - 1u;
   // digit slots: one per thread, two per thread for 9-bit digits
@@ -283,7 +293,9 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
 
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
-  const TileSpan ts = tile_span(g, n, nblocks, blockIdx.x, kDigits);
+  const uint32_t b = xcd_tile(xq);
+  if (b >= nblocks) return;
+  const TileSpan ts = tile_span(g, n, nblocks, b, kDigits);
   if (ts.n == 0) return;
   const size_t tile0 = ts.i0;
   const uint32_t tile_n = ts.n;
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(256) void k_unit_owner(const uint32_t *__restrict__
 }
 
 using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
-                        const uint32_t *, uint32_t, SegGeom, RadixKeyMap);
+                        const uint32_t *, uint32_t, SegGeom, RadixKeyMap, uint32_t);
 
 DownFn downsweep_for(int bits) {
   switch (bits) {
@@ -512,6 +524,14 @@ void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_
   DL_HIP(hipGetLastError());
 }
 
+// Downsweep launch over nb tiles: 8 xq workgroups with xq tiles per XCD, or
+// nb workgroups in tile order (DL_RADIX_XCD=0).
+#ifndef DL_RADIX_XCD
+#define DL_RADIX_XCD 1
+#endif
+static inline uint32_t downsweep_xq(uint32_t nb) { return DL_RADIX_XCD ? div_up(nb, 8) : 0u; }
+static inline uint32_t downsweep_grid(uint32_t nb) { return DL_RADIX_XCD ? 8u * div_up(nb, 8) : nb; }
+
 size_t radix_workspace_bytes(size_t n, int key_bits) {
   (void)key_bits;
   const size_t nb = std::max<size_t>(1, div_up(n, kRTile));
@@ -547,8 +567,9 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
     k_radix_upsweep<<<nb, kRBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<nb, kRBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko, vo, n, shift,
-                                                 offsets, nb, none, map);
+    downsweep_for(bits[p])<<<downsweep_grid(nb), kRBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko,
+                                                                  vo, n, shift, offsets, nb, none, map,
+                                                                  downsweep_xq(nb));
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
@@ -607,7 +628,8 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
     k_radix_upsweep<<<mb, kRBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<mb, kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g, RadixKeyMap{});
+    downsweep_for(bits[p])<<<downsweep_grid(mb), kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g,
+                                                                  RadixKeyMap{}, downsweep_xq(mb));
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
