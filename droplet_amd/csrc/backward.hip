@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) void k_backward_raster(const float4 *__restrict
     }
   }
   const uint32_t start = off[tile], end = off[tile + 1];
-  const double tile_x0 = (double)(tx * kTile) + 0.5, tile_x1 = tile_x0 + 15.0;
+  const float tile_x0 = (float)(tx * kTile) + 0.5f, tile_x1 = tile_x0 + 15.0f;
   for (uint32_t base = start; base < end; base += 64) {
     uint32_t strips_live = 0;
 #pragma unroll
@@ -117,10 +117,11 @@ __global__ __launch_bounds__(64) void k_backward_raster(const float4 *__restrict
       s_rec[3 * lane + 2] = col;
       s_id[lane] = j;
       uint32_t m = 0;
+      const StripTest st = strip_test(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const double y0 = (double)(ty * kTile + 4 * k) + 0.5;
-        if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+        const float y0 = (float)(ty * kTile + 4 * k) + 0.5f;
+        if (strip_may_hit(st, y0, y0 + 3.0f)) m |= 1u << k;
       }
       s_mask[lane] = m;
     }

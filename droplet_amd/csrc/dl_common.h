@@ -173,6 +173,75 @@ __host__ __device__ inline bool tile_range(float sx, float sy, float r, const Ti
   return x0 <= x1 && y0 <= y1;
 }
 
+// The rasterizer's strip test: can a pixel of a tile's columns [x0, x1] and
+// a strip's rows [y0, y1] (pixel centres) pass the reference's box test
+// |dx| <= r, |dy| <= r (rasterize.ts:66-69) with power >= tau, for a splat
+// at (sx, sy) with conic (A, B, C)? "false" is a proof that none does.
+//
+// A pixel computes dx = fl(px - sx) in f32; rounding is monotone, so over
+// the tile's columns dx lies in [fl(x0 - sx), fl(x1 - sx)], and it passes
+// the box test only inside [-r, r]: the rectangle W of (dx, dy) is exact.
+// power = -q/2 with q = A dx^2 + 2B dx dy + C dy^2 evaluated in f32 (three
+// roundings per term), so a blended pixel has q(dx, dy) <= -2 tau + 3u mag,
+// u = 2^-24, mag = A mx^2 + 2|B| mx my + C my^2 >= every term on W. With A,
+// C > 0, q restricted to an edge of W is a convex parabola and q has no
+// interior minimum but the centre, so the minimum of q on W is 0 (W holds
+// the centre) or the least of the four edge minima. Each edge minimum is
+// taken at its clamped vertex, computed from a rounded ratio (-B/C, -B/A):
+// an error d in the vertex raises the value by at most 3 C d^2, a few u^2
+// of mag. The candidates are evaluated in f32 (error below 4u mag). Their
+// least value minus 1e-5 mag (and 1e-30 for underflow) therefore lies below
+// the true minimum by more than the 3u mag a pixel needs: if it still
+// exceeds -2 tau, no pixel of the rectangle is blended. NaN or infinite
+// terms make the final comparison false (the strip is tested).
+struct StripTest {
+  float sy, r, A, B, C, kA, thr, dxl, dxh, yv0, yv1, mx;
+  bool never, always;
+};
+
+// The strip-independent part, once per (splat, tile columns).
+__host__ __device__ inline StripTest strip_test(float sx, float sy, float r, float A, float B, float C, float tau,
+                                                float x0, float x1) {
+  StripTest t;
+  t.sy = sy; t.r = r; t.A = A; t.B = B; t.C = C;
+  t.never = !(tau < __builtin_inff());
+  t.always = !(A > 0.0f && C > 0.0f);
+  t.dxl = __builtin_fmaxf(x0 - sx, -r);
+  t.dxh = __builtin_fminf(x1 - sx, r);
+  t.never = t.never || t.dxl > t.dxh;
+  t.mx = __builtin_fmaxf(__builtin_fabsf(t.dxl), __builtin_fabsf(t.dxh));
+  const float kC = -B / C;
+  t.kA = -B / A;
+  t.yv0 = kC * t.dxl;
+  t.yv1 = kC * t.dxh;
+  t.thr = -2.0f * tau;
+  return t;
+}
+
+__host__ __device__ inline float strip_q(const StripTest &t, float x, float y) {
+  return (t.A * x * x + 2.0f * t.B * x * y) + t.C * y * y;
+}
+
+__host__ __device__ inline bool strip_may_hit(const StripTest &t, float y0, float y1) {
+  if (t.never) return false;
+  if (t.always) return true;
+  const float dyl = __builtin_fmaxf(y0 - t.sy, -t.r);
+  const float dyh = __builtin_fminf(y1 - t.sy, t.r);
+  if (dyl > dyh) return false;
+  float qmin = 0.0f;
+  if (!(t.dxl <= 0.0f && 0.0f <= t.dxh && dyl <= 0.0f && 0.0f <= dyh)) {
+    const float ya = __builtin_fminf(__builtin_fmaxf(t.yv0, dyl), dyh);
+    const float yb = __builtin_fminf(__builtin_fmaxf(t.yv1, dyl), dyh);
+    const float xa = __builtin_fminf(__builtin_fmaxf(t.kA * dyl, t.dxl), t.dxh);
+    const float xb = __builtin_fminf(__builtin_fmaxf(t.kA * dyh, t.dxl), t.dxh);
+    qmin = __builtin_fminf(__builtin_fminf(strip_q(t, t.dxl, ya), strip_q(t, t.dxh, yb)),
+                           __builtin_fminf(strip_q(t, xa, dyl), strip_q(t, xb, dyh)));
+  }
+  const float my = __builtin_fmaxf(__builtin_fabsf(dyl), __builtin_fabsf(dyh));
+  const float mag = (t.A * t.mx * t.mx + 2.0f * __builtin_fabsf(t.B) * t.mx * my) + t.C * my * my;
+  return !(qmin - 1e-5f * mag - 1e-30f > t.thr);
+}
+
 // floatToSortable (src/spatial/radix-sort.ts:20-24)
 __host__ __device__ inline uint32_t sortable_key(float f) {
   uint32_t u = f32_bits(f);

@@ -348,41 +348,6 @@ __device__ inline float gauss_power(float A, float B, float C, float dx, float d
   return (-0.5f) * __builtin_fmaf(C * dy, dy, __builtin_fmaf((2.0f * B) * dx, dy, (A * dx) * dx));
 }
 
-// Can a pixel centre of the rectangle [x0, x1] x [y0, y1] pass the
-// reference's r-box test with power >= tau? Evaluated in f64 on the exact
-// quadratic q = A dx^2 + 2B dx dy + C dy^2 (power = -q/2): the minimum of a
-// positive-definite q over a rectangle is 0 when the rectangle holds the
-// centre and otherwise lies on an edge. The dx/dy intervals are widened for
-// the f32 rounding of px - sx and the minimum is lowered by 1e-5 of the
-// terms' magnitudes, far above the f32 evaluation error of q, so "false" is a
-// proof that every pixel of the rectangle skips this splat.
-__device__ inline bool rect_may_hit(double sx, double sy, double r, double A, double B, double C, double tau,
-                                    double x0, double x1, double y0, double y1) {
-  if (!(tau < INFINITY)) return false;
-  if (!(A > 0.0 && C > 0.0)) return true;
-  const double ex = (fabs(sx) + fabs(x1) + 1.0) * 2.4e-7;
-  const double ey = (fabs(sy) + fabs(y1) + 1.0) * 2.4e-7;
-  const double rx = r + ex, ry = r + ey;
-  const double dxl = fmax(x0 - sx - ex, -rx), dxh = fmin(x1 - sx + ex, rx);
-  const double dyl = fmax(y0 - sy - ey, -ry), dyh = fmin(y1 - sy + ey, ry);
-  if (dxl > dxh || dyl > dyh) return false;
-  double qmin = 0.0;
-  if (!(dxl <= 0.0 && 0.0 <= dxh && dyl <= 0.0 && 0.0 <= dyh)) {
-    qmin = INFINITY;
-    const double X[2] = {dxl, dxh}, Y[2] = {dyl, dyh};
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-      const double yv = fmin(fmax(-B * X[e] / C, dyl), dyh);
-      qmin = fmin(qmin, A * X[e] * X[e] + 2.0 * B * X[e] * yv + C * yv * yv);
-      const double xv = fmin(fmax(-B * Y[e] / A, dxl), dxh);
-      qmin = fmin(qmin, A * xv * xv + 2.0 * B * xv * Y[e] + C * Y[e] * Y[e]);
-    }
-  }
-  const double mx = fmax(fabs(dxl), fabs(dxh)), my = fmax(fabs(dyl), fabs(dyh));
-  const double mag = A * mx * mx + 2.0 * fabs(B) * mx * my + C * my * my;
-  return !(qmin - 1e-5 * mag - 1e-30 > -2.0 * tau);
-}
-
 // Ascending 6-bit sort key of a tile's work: 63 for none, then two bands per
 // octave, heavier work first.
 __device__ inline uint32_t work_key(uint32_t w) {
@@ -524,11 +489,11 @@ __device__ inline uint32_t px16_live(const Px16 &s) {
 // loop body in the reference's order. Returns 8 plus the visited (splat,
 // strip) bits.
 template <bool COUNT, bool MULTI, class GetJ>
-__device__ inline uint32_t px16_batch(Px16 &s, uint32_t oy, double tile_x0, double tile_x1, uint32_t cnt,
+__device__ inline uint32_t px16_batch(Px16 &s, uint32_t oy, float tile_x0, float tile_x1, uint32_t cnt,
                                       uint32_t strips_live, GetJ get_j, const float4 *__restrict__ rec, RastLds &L,
                                       uint32_t &iters) {
   const int lane = threadIdx.x & 63;
-  // Staging: the accumulators wait in LDS while the f64 strip tests run,
+  // Staging: the accumulators wait in LDS while the strip tests run,
   // so they hold no VGPRs there (78 VGPRs instead of 94); the batch is 32
   // splats, two lanes per splat with two strips each, so that the LDS
   // of a wave stays under 6 KB. Both bounds together allow 6 waves per
@@ -552,13 +517,12 @@ __device__ inline uint32_t px16_batch(Px16 &s, uint32_t oy, double tile_x0, doub
       L.rec[3 * slot + 1] = b;
       L.rec[3 * slot + 2] = col;
     }
-    // one strip at a time (unrolled, the f64 temporaries of several
-    // strips take more VGPRs)
-#pragma unroll 1
+    const StripTest st = strip_test(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1);
+#pragma unroll
     for (uint32_t q = 0; q < 2; q++) {
       const uint32_t k = 2 * half + q;
-      const double y0 = (double)(oy + 4 * k) + 0.5;
-      if (rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0)) m |= 1u << k;
+      const float y0 = (float)(oy + 4 * k) + 0.5f;
+      if (strip_may_hit(st, y0, y0 + 3.0f)) m |= 1u << k;
     }
   }
   // bit c of sk: staged splat c may reach strip k (lanes 0-31 test
@@ -617,7 +581,7 @@ __device__ inline uint32_t px16_run(Px16 &s, uint32_t ox, uint32_t oy, uint32_t 
   uint32_t work = 0;
   consumed = 0;
   if (start >= end) return 0;
-  const double tile_x0 = (double)ox + 0.5, tile_x1 = tile_x0 + 15.0;
+  const float tile_x0 = (float)ox + 0.5f, tile_x1 = tile_x0 + 15.0f;
   uint32_t stop = end;
   for (uint32_t base = start; base < end; base += kStage) {
     const uint32_t strips_live = px16_live(s);
@@ -644,7 +608,7 @@ __device__ inline uint32_t px16_run_quad(Px16 &s, uint32_t ox, uint32_t oy, uint
   uint32_t work = 0, iters = 0;
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const double tile_x0 = (double)ox + 0.5, tile_x1 = tile_x0 + 15.0;
+  const float tile_x0 = (float)ox + 0.5f, tile_x1 = tile_x0 + 15.0f;
   for (uint32_t base = start; base < end; base += 64u) {
     uint32_t strips_live = px16_live(s);
     if (strips_live == 0) break;
@@ -751,7 +715,7 @@ __device__ inline uint32_t rasterize_tile8(uint32_t tx, uint32_t ty, uint32_t vt
   }
   uint32_t work = 0;
   if (start < end) {
-    const double tile_x0 = (double)(tx * 8u) + 0.5, tile_x1 = tile_x0 + 7.0;
+    const float tile_x0 = (float)(tx * 8u) + 0.5f, tile_x1 = tile_x0 + 7.0f;
     const uint32_t mine = (uint32_t)lane >> 5;  // this lane's strip
     for (uint32_t base = start; base < end; base += kStage) {
       const uint64_t live = __ballot(!(T < kMinT));
@@ -770,8 +734,8 @@ __device__ inline uint32_t rasterize_tile8(uint32_t tx, uint32_t ty, uint32_t vt
           L.rec[3 * slot + 1] = b;
           L.rec[3 * slot + 2] = col;
         }
-        const double y0 = (double)(ty * 8u + 4u * half) + 0.5;
-        hit = rect_may_hit(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1, y0, y0 + 3.0);
+        const float y0 = (float)(ty * 8u + 4u * half) + 0.5f;
+        hit = strip_may_hit(strip_test(a.x, a.y, a.z, b.x, b.y, b.z, col.w, tile_x0, tile_x1), y0, y0 + 3.0f);
       }
       const uint64_t bits = __ballot(hit);
       const uint32_t s0 = (uint32_t)live ? (uint32_t)bits : 0u;
