@@ -29,7 +29,7 @@ namespace {
 
 // the largest sort: 2^20 pairs (8 MiB of ping-pong arrays) and the radix
 // workspace of that size (digit counts of 256 tiles x 512 digits, their
-// offsets and scan levels: about 1.1 MB)
+// offsets and the digit totals: about 1.1 MB)
 constexpr size_t kNoAllocMaxN = size_t(1) << 20;
 constexpr size_t kNoAllocWsBytes = size_t(2) << 20;
 constexpr size_t kScratchBytes = 2 * kNoAllocMaxN * sizeof(uint32_t) + kNoAllocWsBytes;

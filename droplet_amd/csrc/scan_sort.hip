@@ -50,6 +50,16 @@ constexpr int kRItems = DL_RADIX_ITEMS;
 constexpr int kRTile = kRBlock * kRItems;
 constexpr int kRWaves = kRBlock / 64;
 constexpr int kRWaveItems = 64 * kRItems;  // elements of one wave's stripe
+// Waves per SIMD the downsweep is compiled for: six (78 VGPRs, three
+// 512-thread workgroups per CU). Measured at 6M keys / 25 bits: 145 us for the
+// sort at six, 152 at eight (64 VGPRs with spills), 152 at four. A
+// persistent downsweep (each workgroup sorting tiles in turn, the next
+// tile's loads issued before the current tile's writes) took 157 us at two
+// workgroups per CU: its loop needed 108 VGPRs.
+#ifndef DL_RADIX_WAVES_PER_EU
+#define DL_RADIX_WAVES_PER_EU 6
+#endif
+constexpr int kRWaveSlots = DL_RADIX_WAVES_PER_EU;
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = __lane_id();
@@ -262,21 +272,69 @@ __global__ __launch_bounds__(kRBlock) void k_radix_upsweep(const uint32_t *__res
   for (uint32_t d = threadIdx.x; d <= mask; d += kRBlock) counts[ts.cbase + (size_t)d * ts.cstride] = s_hist[d];
 }
 
-// Workgroups are dealt to the 8 XCDs in turn (workgroup i to XCD i mod 8).
-// With xq > 0 (launch of 8 xq workgroups) workgroup i sorts tile
-// (i mod 8) xq + i / 8, so each XCD sorts a contiguous run of tiles: the
-// runs of one digit that neighbouring tiles write are adjacent in the
-// output, and written through one L2 they fill whole lines.
-__device__ inline uint32_t xcd_tile(uint32_t xq) {
-  return xq ? (blockIdx.x & 7u) * xq + (blockIdx.x >> 3) : blockIdx.x;
+// Row scan of the digit-major counts (unsegmented layout): workgroup d
+// writes the exclusive scan of counts[d nb, (d + 1) nb), each tile's place in
+// digit d's run, and the digit's total rowtot[d]. One launch in place of a
+// flat scan's two; the downsweep adds the digit bases.
+__global__ __launch_bounds__(kBlock) void k_radix_rowscan(const uint32_t *__restrict__ counts,
+                                                          uint32_t *__restrict__ offsets, uint32_t nb,
+                                                          uint32_t *__restrict__ rowtot) {
+  __shared__ uint32_t s_data[kTileN];
+  __shared__ uint32_t s_wave[kWaves];
+  const size_t row = (size_t)blockIdx.x * nb;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += kTileN) {
+    const uint32_t m = min((uint32_t)kTileN, nb - c0);
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      const uint32_t li = (uint32_t)k * kBlock + threadIdx.x;
+      s_data[li] = li < m ? counts[row + c0 + li] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[kItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      v[k] = s_data[threadIdx.x * kItems + k];
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, s_wave, &total) + carry;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      s_data[threadIdx.x * kItems + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      const uint32_t li = (uint32_t)k * kBlock + threadIdx.x;
+      if (li < m) offsets[row + c0 + li] = s_data[li];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowtot[blockIdx.x] = carry;
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup i to XCD i mod 8).
+// A launch of 8 xq workgroups has workgroup i sort tile (i mod 8) xq + i / 8,
+// so each XCD sorts a contiguous run of tiles: the runs of one digit that
+// neighbouring tiles write are adjacent in the output, and written through
+// one L2 they fill whole lines.
+__device__ inline uint32_t xcd_tile(uint32_t xq) { return (blockIdx.x & 7u) * xq + (blockIdx.x >> 3); }
+
+// Stable scatter of one radix pass, one tile per workgroup.
+// With rowtot set, offsets[] holds each tile's position inside its digit's
+// run (the row scan above) and the downsweep adds the digit's base, the sum
+// of rowtot over the smaller digits; without it offsets[] are absolute.
 template <int BITS>
-__global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
+__global__ __launch_bounds__(kRBlock, kRWaveSlots) void k_radix_downsweep(const uint32_t *__restrict__ keys_in,
                                                             const uint32_t *__restrict__ vals_in,
                                                             uint32_t *__restrict__ keys_out,
                                                             uint32_t *__restrict__ vals_out, size_t n, int shift,
                                                             const uint32_t *__restrict__ offsets,
+                                                            const uint32_t *__restrict__ rowtot,
                                                             uint32_t nblocks, SegGeom g, RadixKeyMap km,
                                                             uint32_t xq) {
   constexpr uint32_t kDigits = 1u << BITS;
@@ -284,9 +342,13 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
   // digit slots: one per thread, two per thread for 9-bit digits
   constexpr uint32_t kSlots = kDigits > (uint32_t)kRBlock ? kDigits : (uint32_t)kRBlock;
   constexpr uint32_t kPer = kSlots / kRBlock;
-  __shared__ uint32_t s_whist[kRWaves][kSlots];
-  __shared__ uint32_t s_keys[kRTile];
-  __shared__ uint32_t s_vals[kRTile];
+  static_assert(kRWaves * kSlots <= 2 * kRTile, "wave histograms must fit the staging tile");
+  // the per-wave digit histograms are dead once every element has its tile
+  // position, so the staged tile (keys, then values) overlays them
+  __shared__ uint32_t s_stage[2 * kRTile];
+  uint32_t (*s_whist)[kSlots] = reinterpret_cast<uint32_t (*)[kSlots]>(s_stage);
+  uint32_t *s_keys = s_stage;
+  uint32_t *s_vals = s_stage + kRTile;
   __shared__ uint32_t s_start[kSlots];
   __shared__ uint32_t s_gbase[kSlots];
   __shared__ uint32_t s_wave[kRWaves];
@@ -302,6 +364,16 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
   const uint32_t kmin = km.kmin ? *km.kmin : 0u;
 
   for (uint32_t d = threadIdx.x; d < kRWaves * kSlots; d += kRBlock) (&s_whist[0][0])[d] = 0;
+  // this tile's digit offsets and the digit totals do not depend on the
+  // keys: their loads go out with the keys' (thread t owns digits
+  // t*kPer .. t*kPer + kPer - 1, in digit order)
+  uint32_t goff[kPer], dtot[kPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kPer; j++) {
+    const uint32_t d = threadIdx.x * kPer + j;
+    goff[j] = d < kDigits ? offsets[ts.cbase + (size_t)d * ts.cstride] : 0u;
+    dtot[j] = (rowtot && d < kDigits) ? rowtot[d] : 0u;
+  }
   __syncthreads();
 
   uint32_t key[kRItems], val[kRItems], rank[kRItems];
@@ -323,17 +395,14 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
       peers &= set ? bal : ~bal;
     }
     const int leader = peers ? (__ffsll((long long)peers) - 1) : 0;
+    // one leader per distinct digit: its add does not race within the wave
     uint32_t old = 0;
-    if (valid && lane == leader) {
-      old = s_whist[wave][d];
-      s_whist[wave][d] = old + (uint32_t)__popcll(peers);
-    }
+    if (valid && lane == leader) old = atomicAdd(&s_whist[wave][d], (uint32_t)__popcll(peers));
     old = (uint32_t)__shfl((int)old, leader, 64);
     rank[k] = old + (uint32_t)__popcll(peers & lt);
   }
   __syncthreads();
   // per digit: wave offsets and block count, then block-wide digit starts
-  // (thread t owns digits t*kPer .. t*kPer + kPer - 1, in digit order)
   {
     uint32_t cnt[kPer];
     uint32_t tsum = 0;
@@ -352,12 +421,29 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
     }
     uint32_t total;
     uint32_t start = block_exclusive_scan<kRWaves>(tsum, s_wave, &total);
+    // digit bases: exclusive scan of the digit totals (row-local offsets)
+    uint32_t dsum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; j++) dsum += dtot[j];
+    uint32_t dbase = rowtot ? block_exclusive_scan<kRWaves>(dsum, s_wave, &total) : 0u;
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) {
       const uint32_t d = threadIdx.x * kPer + j;
       s_start[d] = start;
       start += cnt[j];
-      if (d < kDigits) s_gbase[d] = offsets[ts.cbase + (size_t)d * ts.cstride];
+      s_gbase[d] = dbase + goff[j];
+      dbase += dtot[j];
+    }
+  }
+  __syncthreads();
+  // tile positions first (the last reads of the wave histograms), then the
+  // staged tile overwrites them
+#pragma unroll
+  for (int k = 0; k < kRItems; k++) {
+    const uint32_t li = (uint32_t)wave * kRWaveItems + (uint32_t)k * 64u + (uint32_t)lane;
+    if (li < tile_n) {
+      const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
+      rank[k] += s_start[d] + s_whist[wave][d];
     }
   }
   __syncthreads();
@@ -365,10 +451,8 @@ __global__ __launch_bounds__(kRBlock) void k_radix_downsweep(const uint32_t *__r
   for (int k = 0; k < kRItems; k++) {
     const uint32_t li = (uint32_t)wave * kRWaveItems + (uint32_t)k * 64u + (uint32_t)lane;
     if (li < tile_n) {
-      const uint32_t d = (mapped_key(key[k], km, kmin) >> shift) & kMask;
-      const uint32_t pos = s_start[d] + s_whist[wave][d] + rank[k];
-      s_keys[pos] = key[k];
-      s_vals[pos] = val[k];
+      s_keys[rank[k]] = key[k];
+      s_vals[rank[k]] = val[k];
     }
   }
   __syncthreads();
@@ -428,7 +512,7 @@ __global__ __launch_bounds__(256) void k_unit_owner(const uint32_t *__restrict__
 }
 
 using DownFn = void (*)(const uint32_t *, const uint32_t *, uint32_t *, uint32_t *, size_t, int,
-                        const uint32_t *, uint32_t, SegGeom, RadixKeyMap, uint32_t);
+                        const uint32_t *, const uint32_t *, uint32_t, SegGeom, RadixKeyMap, uint32_t);
 
 DownFn downsweep_for(int bits) {
   switch (bits) {
@@ -524,19 +608,16 @@ void bin_ranges(hipStream_t s, const uint32_t *d_keys, uint32_t n, uint32_t num_
   DL_HIP(hipGetLastError());
 }
 
-// Downsweep launch over nb tiles: 8 xq workgroups with xq tiles per XCD, or
-// nb workgroups in tile order (DL_RADIX_XCD=0).
-#ifndef DL_RADIX_XCD
-#define DL_RADIX_XCD 1
-#endif
-static inline uint32_t downsweep_xq(uint32_t nb) { return DL_RADIX_XCD ? div_up(nb, 8) : 0u; }
-static inline uint32_t downsweep_grid(uint32_t nb) { return DL_RADIX_XCD ? 8u * div_up(nb, 8) : nb; }
+// Downsweep launch over nb tiles: 8 xq workgroups, xq tiles per XCD
+static inline uint32_t downsweep_xq(uint32_t nb) { return div_up(nb, 8); }
+static inline uint32_t downsweep_grid(uint32_t nb) { return 8u * div_up(nb, 8); }
 
+// counts and offsets ([digit][tile], 512 digits) and the digit totals
 size_t radix_workspace_bytes(size_t n, int key_bits) {
   (void)key_bits;
   const size_t nb = std::max<size_t>(1, div_up(n, kRTile));
   const size_t counts = ((kMaxDigits * nb * sizeof(uint32_t) + 255) / 256) * 256;
-  return counts * 2 + scan_levels_bytes(kMaxDigits * nb);
+  return counts * 2 + kMaxDigits * sizeof(uint32_t);
 }
 
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
@@ -556,19 +637,20 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
   const size_t counts_bytes = ((kMaxDigits * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
   uint32_t *counts = reinterpret_cast<uint32_t *>(w);
   uint32_t *offsets = reinterpret_cast<uint32_t *>(w + counts_bytes);
-  uint8_t *scan_ws = w + 2 * counts_bytes;
+  uint32_t *rowtot = reinterpret_cast<uint32_t *>(w + 2 * counts_bytes);
   int passes, bits[32];
   pass_layout(key_bits, max_digit_bits, &passes, bits);
   uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
   int shift = 0;
   const SegGeom none{};
   for (int p = 0; p < passes; p++) {
-    const size_t ndig = (size_t)1 << bits[p];
+    const uint32_t ndig = 1u << bits[p];
     k_radix_upsweep<<<nb, kRBlock, 0, s>>>(ki, n, shift, bits[p], counts, nb, none, map);
     DL_HIP(hipGetLastError());
-    scan_rec(s, counts, offsets, ndig * nb, nullptr, scan_ws);
+    k_radix_rowscan<<<ndig, kBlock, 0, s>>>(counts, offsets, nb, rowtot);
+    DL_HIP(hipGetLastError());
     downsweep_for(bits[p])<<<downsweep_grid(nb), kRBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko,
-                                                                  vo, n, shift, offsets, nb, none, map,
+                                                                  vo, n, shift, offsets, rowtot, nb, none, map,
                                                                   downsweep_xq(nb));
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
@@ -628,8 +710,8 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
     k_radix_upsweep<<<mb, kRBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<downsweep_grid(mb), kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, mb, g,
-                                                                  RadixKeyMap{}, downsweep_xq(mb));
+    downsweep_for(bits[p])<<<downsweep_grid(mb), kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, nullptr, mb,
+                                                                  g, RadixKeyMap{}, downsweep_xq(mb));
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
