@@ -87,6 +87,28 @@ __host__ __device__ inline float dl_expf_nonpos(float x) {
   return __builtin_ldexpf(p, k);
 }
 
+// The sigma at which a voxel's Gaussian sum may stop early. The reference
+// sums while sigma < 7 and calls a voxel solid when 1 - exp(-sigma) >=
+// cutoff (voxel/shaders/voxelize.ts:111-134). Every term is >= 0, so sigma
+// never decreases, and it never reaches 8 (the sum stops at 7, a term is at
+// most 1); once it reaches the value returned here, 1 - dl_expf(-sigma) >=
+// cutoff holds for every sigma up to 8, so the remaining terms cannot change
+// the mask bit. The value is -ln((1 - c (1 + 2^-22)) / (1 + 2^-20)) rounded
+// up to f32 and raised by two ulps: the margins exceed dl_expf's relative
+// error and the rounding of 1 - e. It is 7 (the reference's own stop) for a
+// cutoff outside (0, 1) or when the bound is not below 7.
+inline float voxel_solid_stop(float cutoff) {
+  const double c = cutoff;
+  if (!(c > 0.0 && c < 1.0)) return 7.0f;
+  const double e = (1.0 - c * (1.0 + 0x1p-22)) / (1.0 + 0x1p-20);
+  if (!(e > 0.0)) return 7.0f;
+  const double sd = -__builtin_log(e);
+  float f = (float)sd;
+  if ((double)f < sd) f = __builtin_nextafterf(f, 8.0f);
+  f = __builtin_nextafterf(__builtin_nextafterf(f, 8.0f), 8.0f);
+  return f < 7.0f ? f : 7.0f;
+}
+
 // sigmoid (project.ts:55, voxel/shaders/voxelize.ts:82)
 __host__ __device__ inline float dl_sigmoidf(float x) { return 1.0f / (1.0f + dl_expf(-x)); }
 

@@ -54,6 +54,7 @@ struct VoxGrid {
   int32_t nb[3];        // blocks per axis
   uint32_t nbatch[3];   // batches per axis
   float res, half, cutoff;
+  float stop;           // sigma at which a voxel's sum ends (voxel_solid_stop)
 };
 
 constexpr int kPrepBlock = 256;
@@ -291,9 +292,11 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
     lo[k] = vc[k] - G.half;
     hi[k] = vc[k] + G.half;
   }
+  // the sum ends at G.stop: the reference's 7, or lower where the voxel is
+  // already solid whatever terms follow (voxel_solid_stop)
   float sigma = 0.0f;
   for (uint32_t base = start; base < end; base += 64) {
-    if (__ballot(sigma < 7.0f) == 0) break;
+    if (__ballot(sigma < G.stop) == 0) break;
     const uint32_t i = base + lane;
     if (i < end) {
       VGauss g = rec[vals[i]];
@@ -304,10 +307,10 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
     }
     __syncthreads();
     const uint32_t cnt = min(64u, end - base);
-    if (sigma < 7.0f) {
+    if (sigma < G.stop) {
       for (uint32_t c = 0; c < cnt; c++) {
         sigma = sigma + eval_gauss(vc[0], vc[1], vc[2], lo, hi, s_g[c]);
-        if (sigma >= 7.0f) break;
+        if (sigma >= G.stop) break;
       }
     }
     __syncthreads();
@@ -522,6 +525,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   G.res = (float)vres;
   G.half = G.res * 0.5f;
   G.cutoff = (float)cutoff;
+  G.stop = voxel_solid_stop(G.cutoff);
 
   DeviceBuffer &counts = W.counts, &offs = W.offs, &scan_ws = W.scan_ws, &small = W.small;
   counts.reserve((size_t)n * 4);
