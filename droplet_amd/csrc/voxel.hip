@@ -136,6 +136,43 @@ __global__ __launch_bounds__(kPrepBlock) void k_vox_prep(const float *__restrict
   }
 }
 
+// The bounds of all Gaussians from the per-workgroup partials of
+// k_vox_prep: out[0..2] min, out[3..5] max (f64 min/max are exact; NaNs are
+// passed over as by the host's std::min / std::max from +-infinity).
+__global__ __launch_bounds__(1024) void k_vox_bounds(const double *__restrict__ partials, uint32_t nparts,
+                                                     double *__restrict__ out) {
+  __shared__ double s_red[6][16];
+  double v[6];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    v[k] = INFINITY;
+    v[3 + k] = -INFINITY;
+  }
+  for (uint32_t b = threadIdx.x; b < nparts; b += 1024) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[k] = fmin(v[k], partials[6 * (size_t)b + k]);
+      v[3 + k] = fmax(v[3 + k], partials[6 * (size_t)b + 3 + k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    for (int d = 32; d >= 1; d >>= 1) {
+      const double o = __shfl_xor(v[k], d, 64);
+      v[k] = k < 3 ? fmin(v[k], o) : fmax(v[k], o);
+    }
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int k = 0; k < 6; k++) s_red[k][threadIdx.x >> 6] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double r = s_red[k][0];
+    for (int w = 1; w < 16; w++) r = k < 3 ? fmin(r, s_red[k][w]) : fmax(r, s_red[k][w]);
+    out[k] = r;
+  }
+}
+
 // Per axis: blocks [lo, hi] (inclusive; lo > hi = none) that Gaussian with
 // centre p and f32 extent e can reach under both filters.
 __device__ inline void axis_block_range(const VoxGrid &G, int k, float pf, float ef, int32_t *lo, int32_t *hi) {
@@ -183,6 +220,8 @@ __device__ inline void axis_block_range(const VoxGrid &G, int k, float pf, float
   *hi = b_hi;
 }
 
+constexpr uint32_t kTotalSlots = 64;
+
 __global__ __launch_bounds__(256) void k_vox_count(const VGauss *__restrict__ rec, uint32_t n, VoxGrid G,
                                                    uint32_t *__restrict__ counts,
                                                    unsigned long long *__restrict__ d_total64) {
@@ -199,32 +238,81 @@ __global__ __launch_bounds__(256) void k_vox_count(const VGauss *__restrict__ re
     for (int k = 0; k < 3; k++) c *= (hi[k] >= lo[k]) ? (unsigned long long)(hi[k] - lo[k] + 1) : 0ull;
     counts[i] = (uint32_t)min(c, 0xFFFFFFFFull);
   }
-  // exact u64 total so the host can refuse grids whose u32 offsets would wrap
+  // exact u64 total so the host can refuse grids whose u32 offsets would
+  // wrap: summed per workgroup and added to one of kTotalSlots slots (every
+  // wave adding to one address serialised the adds: 190 us at 1M Gaussians)
+  __shared__ unsigned long long s_c[4];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (__lane_id() == 0 && c) atomicAdd(d_total64, c);
+  if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = (s_c[0] + s_c[1]) + (s_c[2] + s_c[3]);
+    if (t) atomicAdd(d_total64 + (blockIdx.x % kTotalSlots), t);
+  }
 }
 
+// The (block, Gaussian) pairs in Gaussian order, each Gaussian's blocks in
+// (z, y, x) order from offs[i]. A wave takes 64 Gaussians, whose pairs are
+// one contiguous run of the output, and writes that run with consecutive
+// lanes on consecutive pairs: each lane finds its pair's Gaussian by binary
+// search over the 64 starts and its block from the index inside the
+// Gaussian's box. (A lane writing its own Gaussian's pairs left every store
+// instruction 64 scattered runs: 393 us at 1M Gaussians.)
 __global__ __launch_bounds__(256) void k_vox_emit(const VGauss *__restrict__ rec, uint32_t n, VoxGrid G,
                                                   const uint32_t *__restrict__ offs, uint32_t *__restrict__ keys,
                                                   uint32_t *__restrict__ vals) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const VGauss g = rec[i];
-  int32_t lo[3], hi[3];
-  axis_block_range(G, 0, g.px, g.ex, &lo[0], &hi[0]);
-  axis_block_range(G, 1, g.py, g.ey, &lo[1], &hi[1]);
-  axis_block_range(G, 2, g.pz, g.ez, &lo[2], &hi[2]);
-  if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return;
-  uint32_t o = offs[i];
+  __shared__ uint32_t s_start[4][65];
+  __shared__ uint32_t s_x0[4][64], s_y0[4][64], s_z0[4][64], s_wx[4][64], s_wy[4][64];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t i0 = blockIdx.x * 256 + (uint32_t)wave * 64u;
+  if (i0 >= n) return;
+  const uint32_t i = i0 + (uint32_t)lane;
+  uint32_t cnt = 0, wx = 1, wy = 1;
+  int32_t lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+  if (i < n) {
+    const VGauss g = rec[i];
+    axis_block_range(G, 0, g.px, g.ex, &lo[0], &hi[0]);
+    axis_block_range(G, 1, g.py, g.ey, &lo[1], &hi[1]);
+    axis_block_range(G, 2, g.pz, g.ez, &lo[2], &hi[2]);
+    if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2]) {
+      wx = (uint32_t)(hi[0] - lo[0] + 1);
+      wy = (uint32_t)(hi[1] - lo[1] + 1);
+      cnt = wx * wy * (uint32_t)(hi[2] - lo[2] + 1);
+    }
+  }
+  const uint32_t o0 = offs[i0];
+  const uint32_t last = min(n - i0, 64u) - 1u;
+  const uint32_t rel = i < n ? offs[i] - o0 : 0u;
+  s_start[wave][lane] = rel;
+  s_x0[wave][lane] = (uint32_t)lo[0];
+  s_y0[wave][lane] = (uint32_t)lo[1];
+  s_z0[wave][lane] = (uint32_t)lo[2];
+  s_wx[wave][lane] = wx;
+  s_wy[wave][lane] = wy;
+  // run length: the last Gaussian's start plus its count
+  const uint32_t total = __shfl(rel + cnt, (int)last, 64);
+  if (lane == 0) s_start[wave][64] = 0xFFFFFFFFu;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   const uint32_t nbx = (uint32_t)G.nb[0], nby = (uint32_t)G.nb[1];
-  for (int32_t z = lo[2]; z <= hi[2]; z++)
-    for (int32_t y = lo[1]; y <= hi[1]; y++)
-      for (int32_t x = lo[0]; x <= hi[0]; x++) {
-        keys[o] = (uint32_t)x + (uint32_t)y * nbx + (uint32_t)z * nbx * nby;
-        vals[o] = i;
-        o++;
-      }
+  for (uint32_t q = (uint32_t)lane; q < total; q += 64u) {
+    // the last Gaussian whose start is <= q (Gaussians without pairs share
+    // the next one's start and are passed over)
+    uint32_t g = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1)
+      if (g + step <= last && s_start[wave][g + step] <= q) g += step;
+    const uint32_t local = q - s_start[wave][g];
+    const uint32_t gx = s_wx[wave][g], gy = s_wy[wave][g];
+    const uint32_t t = local / gx;
+    const uint32_t x = local - t * gx;
+    const uint32_t z = t / gy;
+    const uint32_t y = t - z * gy;
+    keys[o0 + q] = (s_x0[wave][g] + x) + (s_y0[wave][g] + y) * nbx + (s_z0[wave][g] + z) * nbx * nby;
+    vals[o0 + q] = i0 + g;
+  }
 }
 
 // evalGaussian (voxel/shaders/voxelize.ts:62-84), reference order of operations
@@ -402,17 +490,18 @@ void voxel_compact(Context &c, hipStream_t s, VoxelResult &r, hipEvent_t done) {
                                                  omask.as<uint2>(), oidx.as<uint32_t>());
   DL_HIP(hipGetLastError());
   exclusive_scan_u32(s, flags.as<uint32_t>(), fpos.as<uint32_t>(), total, small.as<uint32_t>(), scan_ws.get());
-  uint32_t nonempty = 0;
-  DL_HIP(hipMemcpyAsync(&nonempty, small.get(), 4, hipMemcpyDeviceToHost, s));
-  DL_HIP(hipStreamSynchronize(s));
-  r.idx.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
-  r.lo.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
-  r.hi.reserve((size_t)std::max<uint32_t>(nonempty, 1) * 4);
+  // the lists are sized for every block, so the compaction follows the scan
+  // without a host round trip for the non-empty count
+  r.idx.reserve((size_t)total * 4);
+  r.lo.reserve((size_t)total * 4);
+  r.hi.reserve((size_t)total * 4);
   k_vox_compact<<<div_up(total, 256), 256, 0, s>>>(flags.as<uint32_t>(), fpos.as<uint32_t>(), omask.as<uint2>(),
                                                    oidx.as<uint32_t>(), total, r.idx.as<uint32_t>(),
                                                    r.lo.as<uint32_t>(), r.hi.as<uint32_t>());
   DL_HIP(hipGetLastError());
   if (done) DL_HIP(hipEventRecord(done, s));
+  uint32_t nonempty = 0;
+  DL_HIP(hipMemcpyAsync(&nonempty, small.get(), 4, hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
   r.count = nonempty;
 }
@@ -455,7 +544,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     ls.reserve((size_t)n * 12);
     op.reserve((size_t)n * 4);
     rec.reserve((size_t)n * sizeof(VGauss));
-    partials.reserve((size_t)nprep * 6 * sizeof(double));
+    partials.reserve(((size_t)nprep + 1) * 6 * sizeof(double));
     ctx.staging.to_device(pos.get(), sp.positions, (size_t)n * 12, s);
     ctx.staging.to_device(rot.get(), sp.rotations, (size_t)n * 16, s);
     ctx.staging.to_device(ls.get(), sp.log_scales, (size_t)n * 12, s);
@@ -464,8 +553,11 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     k_vox_prep<<<nprep, kPrepBlock, 0, s>>>(pos.as<float>(), rot.as<float>(), ls.as<float>(), op.as<float>(), n,
                                            rec.as<VGauss>(), partials.as<double>());
     DL_HIP(hipGetLastError());
-    part.resize((size_t)nprep * 6);
-    DL_HIP(hipMemcpyAsync(part.data(), partials.get(), part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    k_vox_bounds<<<1, 1024, 0, s>>>(partials.as<double>(), nprep, partials.as<double>() + (size_t)nprep * 6);
+    DL_HIP(hipGetLastError());
+    part.resize(6);
+    DL_HIP(hipMemcpyAsync(part.data(), partials.as<double>() + (size_t)nprep * 6, 6 * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
   } else {
     DL_HIP(hipEventRecord(e0, s));
@@ -480,10 +572,10 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   } else {
     // computeBlockAlignedBounds (voxelize.ts:278-306)
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t b = 0; b < (n ? nprep : 0); b++)
+    if (n)
       for (int k = 0; k < 3; k++) {
-        mn[k] = std::min(mn[k], part[6 * (size_t)b + k]);
-        mx[k] = std::max(mx[k], part[6 * (size_t)b + 3 + k]);
+        mn[k] = part[k];
+        mx[k] = part[3 + k];
       }
     for (int k = 0; k < 3; k++) {
       bmn[k] = std::floor(mn[k] / bs) * bs;
@@ -531,19 +623,21 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   counts.reserve((size_t)n * 4);
   offs.reserve((size_t)n * 4);
   scan_ws.reserve(scan_workspace_bytes(n));
-  small.reserve(256);
-  small.reserve(256);
+  // small: [0] the scan's u32 total; from byte 64 the u64 total's slots
+  small.reserve(64 + kTotalSlots * sizeof(unsigned long long));
   unsigned long long *d_total64 = reinterpret_cast<unsigned long long *>(small.as<uint8_t>() + 64);
-  DL_HIP(hipMemsetAsync(d_total64, 0, sizeof(unsigned long long), s));
+  DL_HIP(hipMemsetAsync(d_total64, 0, kTotalSlots * sizeof(unsigned long long), s));
   k_vox_count<<<div_up(n, 256), 256, 0, s>>>(rec.as<VGauss>(), n, G, counts.as<uint32_t>(), d_total64);
   DL_HIP(hipGetLastError());
   // the u64 total and the scan's u32 total in one read: a total above 2^32
   // fails before the (wrapped) scan is used
   exclusive_scan_u32(s, counts.as<uint32_t>(), offs.as<uint32_t>(), n, small.as<uint32_t>(), scan_ws.get());
-  uint32_t head[24];
+  uint32_t head[16 + 2 * kTotalSlots];
   DL_HIP(hipMemcpyAsync(head, small.get(), sizeof(head), hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
-  const unsigned long long total_pairs = (unsigned long long)head[16] | ((unsigned long long)head[17] << 32);
+  unsigned long long total_pairs = 0;
+  for (uint32_t k = 0; k < kTotalSlots; k++)
+    total_pairs += (unsigned long long)head[16 + 2 * k] | ((unsigned long long)head[17 + 2 * k] << 32);
   if (total_pairs > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "more than 2^32 (block, Gaussian) pairs");
   const uint32_t npairs = head[0];
 
