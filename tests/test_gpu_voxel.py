@@ -92,3 +92,21 @@ def test_voxel_single_gaussian(device):
     data = SplatData(1, [0.113, -0.071, 0.052], [1, 0, 0, 0], [-2.5] * 3, [2.0], [0, 0, 0])
     got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
     assert got.buffer.popcount() > 100
+
+
+def test_voxel_grid_wider_than_256_blocks(device):
+    """A long grid: more than 256 blocks along x (22 m at 0.08 m blocks),
+    whose block keys take every bit of the pair sort's passes."""
+    data = _small_box_scene(6000, 5, box=((-11, 0, -0.3), (11, 0.4, 0.3)))
+    got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
+    assert got.num_blocks_x > 256
+    assert got.buffer.count > 1000
+
+
+def test_voxel_grid_with_many_layers(device):
+    """Many block layers along z (a 12 m column at 0.08 m blocks: 150
+    layers of a few blocks each), most of them holding few Gaussians."""
+    data = _small_box_scene(4000, 6, box=((-0.2, 0, -6), (0.2, 0.3, 6)))
+    got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
+    assert got.num_blocks_z > 140
+    assert got.buffer.count > 500
