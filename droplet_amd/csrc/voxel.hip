@@ -322,10 +322,12 @@ __global__ __launch_bounds__(256) void k_vox_emit(const VGauss *__restrict__ rec
 // same f32 sums, made once per staged Gaussian), and lo/hi the voxel's
 // centre -+ half. Exact rewrites, none of which changes a mask bit:
 // - the three box tests are evaluated together;
-// - the clamp uses min/max instructions: they differ from WGSL's
+// - the clamp is one median-of-three instruction (lo < hi, so the median
+//   of (p, lo, hi) is min(max(p, lo), hi)): it differs from WGSL's
 //   compare-select only in the sign of a zero result when both operands
 //   are zero, which the squares of the distance remove, and the bound
-//   operands are finite, so a NaN centre gives the same NaN either way;
+//   operands are finite, so a NaN centre gives NaN either way (the
+//   distance subtracts the NaN centre);
 // - e^x of x = -dd/2 <= 0 (or NaN) is dl_expf_nonpos after clamping x at
 //   -104 as dl_expf does, bit-identical to dl_expf there.
 __device__ inline float eval_gauss(float vx, float vy, float vz, const float lo[3], const float hi[3],
@@ -333,9 +335,9 @@ __device__ inline float eval_gauss(float vx, float vy, float vz, const float lo[
   const float d0 = vx - g.px, d1 = vy - g.py, d2 = vz - g.pz;
   const bool out = (__builtin_fabsf(d0) > g.ex) | (__builtin_fabsf(d1) > g.ey) | (__builtin_fabsf(d2) > g.ez);
   if (out) return 0.0f;
-  const float c0 = __builtin_fminf(__builtin_fmaxf(g.px, lo[0]), hi[0]) - g.px;
-  const float c1 = __builtin_fminf(__builtin_fmaxf(g.py, lo[1]), hi[1]) - g.py;
-  const float c2 = __builtin_fminf(__builtin_fmaxf(g.pz, lo[2]), hi[2]) - g.pz;
+  const float c0 = __builtin_amdgcn_fmed3f(g.px, lo[0], hi[0]) - g.px;
+  const float c1 = __builtin_amdgcn_fmed3f(g.py, lo[1], hi[1]) - g.py;
+  const float c2 = __builtin_amdgcn_fmed3f(g.pz, lo[2], hi[2]) - g.pz;
   const float qx = -g.qx, qy = -g.qy, qz = -g.qz, qw = g.qw;
   const float t0 = 2.0f * (qy * c2 - qz * c1);
   const float t1 = 2.0f * (qz * c0 - qx * c2);
