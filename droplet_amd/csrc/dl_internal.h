@@ -156,6 +156,9 @@ void exclusive_scan_u32_dev(hipStream_t s, const uint32_t *d_in, uint32_t *d_out
 // moved unchanged. When every other key lies in [kmin, kmin + sentinel) the
 // result equals the sort of the raw keys (the map is order-preserving), in
 // fewer passes when sentinel < 2^key_bits.
+// With low_bit > 0 the passes cover bits [low_bit, key_bits) only: the
+// result is sorted by the whole key when the input is already sorted by the
+// low bits among keys whose high bits are equal (the sort is stable).
 struct RadixKeyMap {
   const uint32_t *kmin = nullptr;
   uint32_t invisible = 0, sentinel = 0;
@@ -163,7 +166,7 @@ struct RadixKeyMap {
 size_t radix_workspace_bytes(size_t n, int key_bits);
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys,
                       uint32_t *tmp_vals, size_t n, int key_bits, void *ws, int max_digit_bits = 8,
-                      const RadixKeyMap &map = RadixKeyMap{}, bool index_vals = false);
+                      const RadixKeyMap &map = RadixKeyMap{}, bool index_vals = false, int low_bit = 0);
 int radix_passes(int key_bits, int max_digit_bits);
 
 // Segment table of the segmented sort (device pointers): segments are the

@@ -622,8 +622,10 @@ size_t radix_workspace_bytes(size_t n, int key_bits) {
 
 bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
                       size_t n, int key_bits, void *ws, int max_digit_bits, const RadixKeyMap &map,
-                      bool index_vals) {
-  if (n <= 1 || key_bits <= 0) {
+                      bool index_vals, int low_bit) {
+  if (key_bits > 32) key_bits = 32;
+  if (low_bit < 0) low_bit = 0;
+  if (n <= 1 || key_bits <= low_bit) {
     // nothing to reorder; index values are still owed: value i is i
     if (index_vals && n > 0) {
       k_iota_u32<<<div_up(n, 256), 256, 0, s>>>(vals, n);
@@ -631,7 +633,6 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
     }
     return false;
   }
-  if (key_bits > 32) key_bits = 32;
   const uint32_t nb = div_up(n, kRTile);
   uint8_t *w = static_cast<uint8_t *>(ws);
   const size_t counts_bytes = ((kMaxDigits * (size_t)nb * sizeof(uint32_t) + 255) / 256) * 256;
@@ -639,9 +640,9 @@ bool radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *t
   uint32_t *offsets = reinterpret_cast<uint32_t *>(w + counts_bytes);
   uint32_t *rowtot = reinterpret_cast<uint32_t *>(w + 2 * counts_bytes);
   int passes, bits[32];
-  pass_layout(key_bits, max_digit_bits, &passes, bits);
+  pass_layout(key_bits - low_bit, max_digit_bits, &passes, bits);
   uint32_t *ki = keys, *vi = vals, *ko = tmp_keys, *vo = tmp_vals;
-  int shift = 0;
+  int shift = low_bit;
   const SegGeom none{};
   for (int p = 0; p < passes; p++) {
     const uint32_t ndig = 1u << bits[p];

@@ -95,8 +95,8 @@ def test_voxel_single_gaussian(device):
 
 
 def test_voxel_grid_wider_than_256_blocks(device):
-    """A long grid: more than 256 blocks along x (22 m at 0.08 m blocks),
-    whose block keys take every bit of the pair sort's passes."""
+    """A long grid: more than 256 blocks along x (22 m at 0.08 m blocks)
+    and a few block layers, so the pairs take the layer route."""
     data = _small_box_scene(6000, 5, box=((-11, 0, -0.3), (11, 0.4, 0.3)))
     got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
     assert got.num_blocks_x > 256
@@ -105,7 +105,8 @@ def test_voxel_grid_wider_than_256_blocks(device):
 
 def test_voxel_grid_with_many_layers(device):
     """Many block layers along z (a 12 m column at 0.08 m blocks: 150
-    layers of a few blocks each), most of them holding few Gaussians."""
+    layers of a few blocks each): more than 64 layers, so the pairs take the
+    flat route (keyed by the flat block index, sorted on every bit)."""
     data = _small_box_scene(4000, 6, box=((-0.2, 0, -6), (0.2, 0.3, 6)))
     got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
     assert got.num_blocks_z > 140
