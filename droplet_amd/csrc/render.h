@@ -128,6 +128,8 @@ struct Renderer {
   DeviceBuffer rect8;
   // per chunk
   DeviceBuffer chunk_counts, offsets, scan_ws, ranges;
+  // pass cuts above the pair budget: count prefix, block sums, cut list
+  DeviceBuffer cut_ws;
   // pairs
   DeviceBuffer pkeys, pvals, ptkeys, ptvals, psort_ws;
   // span binning

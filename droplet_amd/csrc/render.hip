@@ -175,6 +175,147 @@ __global__ __launch_bounds__(256) void k_gather_counts(const uint32_t *__restric
 }
 
 // ---------------------------------------------------------------------------
+// Pass cuts of a frame above the pair budget, on the device: P[j] = the tile
+// counts of the first j splats in depth order (u64, P[0] = 0), then one wave
+// cuts each pass greedily by 64-way searches on P. A host loop over the n
+// counts gives the same cuts after copying them (24 MB at 6M splats) and
+// walking them on one core, which left the GPU idle for milliseconds.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kCutBlock = 1024;   // splats per block of the prefix kernels (256 threads x 4)
+constexpr uint32_t kMaxCutSegs = 128;  // passes before the cut (more: the host loop)
+
+struct CutSegs {
+  uint32_t n;                      // passes to cut
+  uint32_t at[kMaxCutSegs + 1];    // their starts, then their end
+};
+
+// g[j] = counts[ids[j]] and the block's sum (kCutBlock splats a block)
+__global__ __launch_bounds__(256) void k_cut_gather(const uint32_t *__restrict__ counts,
+                                                    const uint32_t *__restrict__ ids, uint32_t n,
+                                                    uint32_t *__restrict__ g, unsigned long long *__restrict__ bsum) {
+  __shared__ unsigned long long s_w[4];
+  const uint32_t base = blockIdx.x * kCutBlock;
+  unsigned long long sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) {
+    const uint32_t j = base + i * 256 + threadIdx.x;
+    if (j < n) {
+      const uint32_t c = counts[ids[j]];
+      g[j] = c;
+      sum += c;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__device__ inline unsigned long long wave_inclusive_scan_u64(unsigned long long v, uint32_t lane) {
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const unsigned long long t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of the nb block sums in place (one workgroup)
+__global__ __launch_bounds__(1024) void k_cut_scan_blocks(unsigned long long *__restrict__ bsum, uint32_t nb) {
+  __shared__ unsigned long long s_w[16];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += 1024) {
+    const uint32_t j = c0 + threadIdx.x;
+    const unsigned long long v = j < nb ? bsum[j] : 0ull;
+    const unsigned long long inc = wave_inclusive_scan_u64(v, lane);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    unsigned long long wbase = 0, tot = 0;
+    for (uint32_t k = 0; k < 16; k++) {
+      if (k < w) wbase += s_w[k];
+      tot += s_w[k];
+    }
+    if (j < nb) bsum[j] = carry + wbase + inc - v;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+// P[j + 1] = bsum[block] + the counts of the block up to j; P[0] = 0
+__global__ __launch_bounds__(256) void k_cut_prefix(const uint32_t *__restrict__ g,
+                                                    const unsigned long long *__restrict__ bsum, uint32_t n,
+                                                    unsigned long long *__restrict__ P) {
+  __shared__ unsigned long long s_w[4];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t j0 = blockIdx.x * kCutBlock + threadIdx.x * 4;
+  uint32_t c[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) c[i] = j0 + i < n ? g[j0 + i] : 0u;
+  const unsigned long long t = (unsigned long long)c[0] + c[1] + c[2] + c[3];
+  const unsigned long long inc = wave_inclusive_scan_u64(t, lane);
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  unsigned long long run = bsum[blockIdx.x] + inc - t;
+  for (uint32_t k = 0; k < w; k++) run += s_w[k];
+  if (blockIdx.x == 0 && threadIdx.x == 0) P[0] = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) {
+    run += c[i];
+    if (j0 + i < n) P[j0 + i + 1] = run;
+  }
+}
+
+// One wave: each pass [a, e) of segs is cut at b = the first index past a
+// whose prefix from a exceeds cap, less one, and at least a + 1 (a splat
+// alone above cap is a pass of its own): out[0] = the number of passes
+// (~0 when more than maxout), out[1 + 2i] = the end of pass i, out[2 + 2i]
+// = its pair count P[b] - P[a].
+__global__ __launch_bounds__(64) void k_cut_greedy(const unsigned long long *__restrict__ P, CutSegs segs,
+                                                   unsigned long long cap, unsigned long long *__restrict__ out,
+                                                   uint32_t maxout) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t cnt = 0;
+  bool over = false;
+  for (uint32_t sg = 0; sg < segs.n; sg++) {
+    uint32_t a = segs.at[sg];
+    const uint32_t e = segs.at[sg + 1];
+    while (a < e) {
+      const unsigned long long pa = P[a];
+      const unsigned long long target = pa + cap;
+      // the first k in [a + 1, e] with P[k] > target, e + 1 when none: P is
+      // non-decreasing, so the probes above target are a suffix of the lanes
+      uint64_t lo = (uint64_t)a + 1, hi = (uint64_t)e + 1;
+      while (lo < hi) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t k = lo + lane * step;
+        const bool gt = k < hi && P[k] > target;
+        const unsigned long long m = __ballot(gt);
+        if (m == 0) {
+          const uint64_t last = lo + std::min<uint64_t>(63, (hi - 1 - lo) / step) * step;
+          lo = last + 1;
+        } else {
+          const uint32_t f = (uint32_t)__builtin_ctzll(m);
+          hi = lo + f * step;
+          if (f > 0) lo = lo + (f - 1) * step + 1;
+        }
+      }
+      const uint32_t b = lo == (uint64_t)e + 1 ? e : (uint32_t)std::max<uint64_t>((uint64_t)a + 1, lo - 1);
+      if (cnt < maxout) {
+        if (lane == 0) {
+          out[1 + 2 * cnt] = b;
+          out[2 + 2 * cnt] = P[b] - pa;
+        }
+        cnt++;
+      } else {
+        over = true;
+      }
+      a = b;
+    }
+  }
+  if (lane == 0) out[0] = over ? ~0ull : (unsigned long long)cnt;
+}
+
+// ---------------------------------------------------------------------------
 // emit (emit-pairs.ts:27-56); values are splat ids
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(128) void k_emit(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
@@ -736,7 +877,44 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     // order, from the per-splat tile counts. A pass's list never exceeds
     // its splats' counts (closed tiles only remove entries), so each fits.
     std::vector<uint64_t> cut_bound;  // pair bound of each pass (over budget only)
-    if (!speculative && pairs_all > budget()) {
+    if (!speculative && pairs_all > budget() && cuts.size() - 1 <= kMaxCutSegs) {
+      // on the device; the host reads only the cut list
+      const uint64_t cap = cut_pairs();
+      const uint32_t nseg = (uint32_t)cuts.size() - 1;
+      // greedy passes: two in a row hold more than cap, so each early-stop
+      // pass of S pairs yields at most 2 S / cap + 1
+      const uint32_t maxout = (uint32_t)std::min<uint64_t>(nseg + 2 * (pairs_all / cap) + 8, (uint64_t)nvis + nseg);
+      const uint32_t nb = div_up(nvis, kCutBlock);
+      const size_t p_bytes = (((size_t)nvis + 1) * 8 + 255) & ~(size_t)255;
+      const size_t b_bytes = ((size_t)nb * 8 + 255) & ~(size_t)255;
+      chunk_counts.reserve((size_t)nvis * 4);
+      cut_ws.reserve(p_bytes + b_bytes + ((size_t)maxout * 2 + 1) * 8);
+      auto *P = cut_ws.as<unsigned long long>();
+      auto *bsum = reinterpret_cast<unsigned long long *>(cut_ws.as<uint8_t>() + p_bytes);
+      auto *cut_out = reinterpret_cast<unsigned long long *>(cut_ws.as<uint8_t>() + p_bytes + b_bytes);
+      CutSegs segs{};
+      segs.n = nseg;
+      for (uint32_t c = 0; c <= nseg; c++) segs.at[c] = cuts[c];
+      k_cut_gather<<<nb, 256, 0, s>>>(counts.as<uint32_t>(), sorted_ids, nvis, chunk_counts.as<uint32_t>(), bsum);
+      DL_HIP(hipGetLastError());
+      k_cut_scan_blocks<<<1, 1024, 0, s>>>(bsum, nb);
+      DL_HIP(hipGetLastError());
+      k_cut_prefix<<<nb, 256, 0, s>>>(chunk_counts.as<uint32_t>(), bsum, nvis, P);
+      DL_HIP(hipGetLastError());
+      k_cut_greedy<<<1, 64, 0, s>>>(P, segs, cap, cut_out, maxout);
+      DL_HIP(hipGetLastError());
+      std::vector<unsigned long long> hc((size_t)maxout * 2 + 1);
+      DL_HIP(hipMemcpyAsync(hc.data(), cut_out, hc.size() * 8, hipMemcpyDeviceToHost, s));
+      DL_HIP(hipStreamSynchronize(s));
+      if (hc[0] > maxout) fail(DL_ERR_INTERNAL, "pass cuts above their bound");
+      std::vector<uint32_t> fine{0};
+      for (uint64_t i = 0; i < hc[0]; i++) {
+        fine.push_back((uint32_t)hc[1 + 2 * i]);
+        cut_bound.push_back(hc[2 + 2 * i]);
+      }
+      if (fine.back() != nvis) fail(DL_ERR_INTERNAL, "pass cuts do not end at the visible count");
+      cuts.swap(fine);
+    } else if (!speculative && pairs_all > budget()) {
       const uint64_t cap = cut_pairs();
       chunk_counts.reserve((size_t)nvis * 4);
       k_gather_counts<<<div_up(nvis, 256), 256, 0, s>>>(counts.as<uint32_t>(), sorted_ids, nvis,

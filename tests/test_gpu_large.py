@@ -96,6 +96,24 @@ def test_large_equirect_frames_match_oracle(device, large):
     _consecutive_frames(device, sc, cam, ref, ref["tile_rows"], frames=2)
 
 
+def _greedy_passes(counts, seg, cap):
+    """Passes of the greedy cut: each run of `seg` splats in depth order (all
+    of them when seg is None) is cut before a splat that would take its pass
+    above `cap` pairs; a splat alone above cap is a pass of its own."""
+    counts = [int(c) for c in counts]
+    seg = seg or max(len(counts), 1)
+    passes = 0
+    for s0 in range(0, len(counts), seg):
+        run = 0
+        passes += 1
+        for c in counts[s0:s0 + seg]:
+            if run > 0 and run + c > cap:
+                passes += 1
+                run = 0
+            run += c
+    return passes
+
+
 @pytest.mark.parametrize("budget,opts", [
     (3000, None),
     (20_000, RendererOptions(binning="sort")),
@@ -131,6 +149,9 @@ def test_pair_budget_cuts_passes(device, budget, opts):
         r.set_taps(True)
         r.render(data, cam, bg)
         acc = r.tap("accum")
+        # its pass count: the greedy cut of its depth-ordered tile counts
+        seg = opts.chunk_cap if opts is not None and opts.chunk_cap else None
+        assert r.stats()["num_chunks"] == _greedy_passes(r.tap("tile_counts"), seg, budget)
         r.set_taps(False)
         np.testing.assert_array_equal(acc.view(np.uint32), ref["accum"].reshape(-1).view(np.uint32))
         # back to the default budget: one pass again, same image
