@@ -55,9 +55,43 @@ struct VoxGrid {
   uint32_t nbatch[3];   // batches per axis
   float res, half, cutoff;
   float stop;           // sigma at which a voxel's sum ends (voxel_solid_stop)
-  uint32_t layered;     // block lists keyed ((y nb0 + x) << zb) | z (else by flat index)
-  uint32_t zb;          // bits of the layer index z in those keys
+  uint32_t layered;     // block lists keyed ((v nbu + u) << lb) | l (else by flat index)
+  uint32_t lax;         // the layer axis l (0-2); u and v are the other two, u the lower
+  uint32_t lb;          // bits of the layer index l in those keys
 };
+
+// A Gaussian's block box on the layer route's axes: layers [l0, l1] of the
+// layer axis, columns [u0, u1] and rows [v0, v1] of the other two.
+struct LayerBox {
+  int32_t l0, l1, u0, u1, v0, v1;
+};
+
+__device__ inline LayerBox layer_box(const VoxGrid &G, const int32_t lo[3], const int32_t hi[3]) {
+  const uint32_t L = G.lax;
+  LayerBox b;
+  b.l0 = L == 0 ? lo[0] : (L == 1 ? lo[1] : lo[2]);
+  b.l1 = L == 0 ? hi[0] : (L == 1 ? hi[1] : hi[2]);
+  b.u0 = L == 0 ? lo[1] : lo[0];
+  b.u1 = L == 0 ? hi[1] : hi[0];
+  b.v0 = L == 2 ? lo[1] : lo[2];
+  b.v1 = L == 2 ? hi[1] : hi[2];
+  return b;
+}
+
+// blocks along the u axis, and along the layer axis
+__device__ inline uint32_t layer_nbu(const VoxGrid &G) { return (uint32_t)(G.lax == 0 ? G.nb[1] : G.nb[0]); }
+__device__ inline uint32_t layer_nbl(const VoxGrid &G) {
+  return (uint32_t)(G.lax == 0 ? G.nb[0] : (G.lax == 1 ? G.nb[1] : G.nb[2]));
+}
+
+// the layer route's key of block (ax, ay, az)
+__device__ inline uint32_t layer_key(const VoxGrid &G, uint32_t ax, uint32_t ay, uint32_t az) {
+  const uint32_t L = G.lax;
+  const uint32_t l = L == 0 ? ax : (L == 1 ? ay : az);
+  const uint32_t u = L == 0 ? ay : ax;
+  const uint32_t v = L == 2 ? ay : az;
+  return ((v * layer_nbu(G) + u) << G.lb) | l;
+}
 
 constexpr int kPrepBlock = 256;
 
@@ -271,8 +305,9 @@ __device__ inline void wave_layer_range(bool any, int32_t z0, int32_t z1, int32_
 
 // Per Gaussian: its block count (counts, flat route). With zc (layer route)
 // instead, wave w (Gaussians 64 w .. 64 w + 63) writes zc[z * nw + w], the
-// pairs its Gaussians have in block layer z, for every z < nb2: the counts
-// of the layer-major placement that k_vox_emit_layers writes.
+// pairs its Gaussians have in block layer z, for every layer z of the layer
+// axis: the counts of the layer-major placement that k_vox_emit_layers
+// writes.
 __global__ __launch_bounds__(256) void k_vox_count(const VGauss *__restrict__ rec, uint32_t n, VoxGrid G,
                                                    uint32_t *__restrict__ counts,
                                                    unsigned long long *__restrict__ d_total64,
@@ -297,17 +332,18 @@ __global__ __launch_bounds__(256) void k_vox_count(const VGauss *__restrict__ re
     // z - z0 of each 64-layer window keeps its layer's sum
     const int lane = __lane_id();
     const bool any = c > 0;
-    const uint32_t per = any ? (uint32_t)(hi[0] - lo[0] + 1) * (uint32_t)(hi[1] - lo[1] + 1) : 0u;
+    const LayerBox bx = layer_box(G, lo, hi);
+    const uint32_t per = any ? (uint32_t)(bx.u1 - bx.u0 + 1) * (uint32_t)(bx.v1 - bx.v0 + 1) : 0u;
     int32_t zmin, zmax;
-    wave_layer_range(any, lo[2], hi[2], &zmin, &zmax);
-    const uint32_t nbz = (uint32_t)G.nb[2];
-    for (uint32_t z = (uint32_t)lane; z < nbz; z += 64)
+    wave_layer_range(any, bx.l0, bx.l1, &zmin, &zmax);
+    const uint32_t nbl = layer_nbl(G);
+    for (uint32_t z = (uint32_t)lane; z < nbl; z += 64)
       if ((int32_t)z < zmin || (int32_t)z > zmax) zc[(size_t)z * nw + w] = 0;
     for (int32_t z0 = zmin; z0 <= zmax; z0 += 64) {
       uint32_t mine = 0;
       const int32_t z1 = min(zmax, z0 + 63);
       for (int32_t z = z0; z <= z1; z++) {
-        const uint32_t inc = wave_inclusive_scan_dpp((any && lo[2] <= z && z <= hi[2]) ? per : 0u);
+        const uint32_t inc = wave_inclusive_scan_dpp((any && bx.l0 <= z && z <= bx.l1) ? per : 0u);
         const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         if (lane == z - z0) mine = v;
       }
@@ -329,17 +365,17 @@ __global__ __launch_bounds__(256) void k_vox_count(const VGauss *__restrict__ re
 }
 
 // The (block, Gaussian) pairs placed layer-major (layer route): the pairs
-// of block layer z come before those of layer z + 1, and inside a layer in
-// Gaussian order, each Gaussian's blocks in (y, x) order. A pair's key is
-// ((y nb0 + x) << zb) | z. Wave w writes its Gaussians' pairs of layer z
-// from zbase[z * nw + w], consecutive lanes on consecutive pairs: a lane
-// finds its pair's Gaussian among the few with blocks in the layer by
-// reading their run starts with readlane (no LDS round trips). A stable
-// sort on the key's bits from zb up then orders the pairs by (y, x, z),
-// Gaussian order inside each block.
+// of block layer z (along the layer axis) come before those of layer z + 1,
+// and inside a layer in Gaussian order, each Gaussian's blocks in (v, u)
+// order. A pair's key is ((v nbu + u) << lb) | z. Wave w writes its
+// Gaussians' pairs of layer z from zbase[z * nw + w], consecutive lanes on
+// consecutive pairs: a lane finds its pair's Gaussian among the few with
+// blocks in the layer by reading their run starts with readlane (no LDS
+// round trips). A stable sort on the key's bits from lb up then orders the
+// pairs by (v, u, z), Gaussian order inside each block.
 __global__ __launch_bounds__(256) void k_vox_emit_layers(const VGauss *__restrict__ rec, uint32_t n, VoxGrid G,
                                                          const uint32_t *__restrict__ zbase, uint32_t nw,
-                                                         uint32_t zb, uint32_t npairs, uint32_t *__restrict__ keys,
+                                                         uint32_t lb, uint32_t npairs, uint32_t *__restrict__ keys,
                                                          uint32_t *__restrict__ vals) {
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const uint32_t i0 = xcd_chunk() * 256 + (uint32_t)wave * 64u;
@@ -354,19 +390,20 @@ __global__ __launch_bounds__(256) void k_vox_emit_layers(const VGauss *__restric
     axis_block_range(G, 2, g.pz, g.ez, &lo[2], &hi[2]);
   }
   const bool any = i < n && hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2];
-  const uint32_t nbx = (uint32_t)G.nb[0];
-  const uint32_t wx = any ? (uint32_t)(hi[0] - lo[0] + 1) : 1u;
-  const uint32_t per = any ? wx * (uint32_t)(hi[1] - lo[1] + 1) : 0u;
-  const uint32_t xy0 = any ? (uint32_t)lo[1] * nbx + (uint32_t)lo[0] : 0u;
+  const LayerBox bx = layer_box(G, lo, hi);
+  const uint32_t nbu = layer_nbu(G);
+  const uint32_t wx = any ? (uint32_t)(bx.u1 - bx.u0 + 1) : 1u;
+  const uint32_t per = any ? wx * (uint32_t)(bx.v1 - bx.v0 + 1) : 0u;
+  const uint32_t xy0 = any ? (uint32_t)bx.v0 * nbu + (uint32_t)bx.u0 : 0u;
   int32_t zmin, zmax;
-  wave_layer_range(any, lo[2], hi[2], &zmin, &zmax);
+  wave_layer_range(any, bx.l0, bx.l1, &zmin, &zmax);
   for (int32_t zw = zmin; zw <= zmax; zw += 64) {
     // the run starts of this 64-layer window, lane z - zw holding layer z's
     const int32_t zl = zw + lane;
     const uint32_t zbl = zl <= zmax ? zbase[(size_t)zl * nw + w] : 0u;
     const int32_t ze = min(zmax, zw + 63);
     for (int32_t z = zw; z <= ze; z++) {
-      const uint32_t m = (any && lo[2] <= z && z <= hi[2]) ? per : 0u;
+      const uint32_t m = (any && bx.l0 <= z && z <= bx.l1) ? per : 0u;
       const uint64_t cov = __ballot(m > 0u);
       if (cov == 0) continue;
       // exclusive scan of m over the lanes: each lane's run start
@@ -395,7 +432,7 @@ __global__ __launch_bounds__(256) void k_vox_emit_layers(const VGauss *__restric
           const uint32_t local = q - gstart;
           const uint32_t y = local / gwx;
           const uint32_t x = local - y * gwx;
-          keys[base + q] = ((gxy0 + y * nbx + x) << zb) | (uint32_t)z;
+          keys[base + q] = ((gxy0 + y * nbu + x) << lb) | (uint32_t)z;
           vals[base + q] = i0 + gl;
         }
       }
@@ -511,9 +548,9 @@ __global__ __launch_bounds__(64) void k_vox_eval(const VGauss *__restrict__ rec,
   const uint32_t blk = blockIdx.x;
   const uint32_t nbx = (uint32_t)G.nb[0], nby = (uint32_t)G.nb[1];
   const uint32_t ax = blk % nbx, ay = (blk / nbx) % nby, az = blk / (nbx * nby);
-  // the block's list: its flat index on the flat route, its (y, x, z) key
-  // on the layer route
-  const uint32_t key = G.layered ? (((ay * nbx + ax) << G.zb) | az) : blk;
+  // the block's list: its flat index on the flat route, its (v, u, layer)
+  // key on the layer route
+  const uint32_t key = G.layered ? layer_key(G, ax, ay, az) : blk;
   const uint32_t start = off[key], end = off[key + 1];
   const int lane = threadIdx.x;
   if (start == end) {
@@ -776,24 +813,29 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   G.stop = voxel_solid_stop(G.cutoff);
 
   // Two routes to the per-block lists (each block's Gaussians in index
-  // order). Layer route: the pairs are placed layer by layer and keyed
-  // ((y nb0 + x) << zb) | z, so a stable sort on the key's bits from zb up
-  // (two passes at the workload's 13 bits, against three on the 20-bit flat
-  // index) orders them by block. Its counting and placement cost grows with
-  // the layers a wave's Gaussians span (each wave loops over them): on the
-  // 1M-Gaussian workload (38 layers, 40 pairs per Gaussian) it saves 0.19 ms
-  // of 3.26, and it is taken for grids of at most 64 layers. Flat route
-  // otherwise: pairs in Gaussian order keyed by the flat block index,
-  // sorted on every bit.
+  // order). Layer route: the layers are the blocks along the axis of fewest
+  // blocks (l; u and v the other two axes in order); the pairs are placed
+  // layer by layer and keyed ((v nbu + u) << lb) | l, so a stable sort on
+  // the key's bits from lb up (two passes on the workload's 14 bits,
+  // against three on the 20-bit flat index) orders them by block. Its
+  // counting and placement cost grows with the layers a wave's Gaussians
+  // span (each wave loops over them), so it is taken for grids of at most
+  // 64 layers. Flat route otherwise: pairs in Gaussian order keyed by the
+  // flat block index, sorted on every bit.
   const uint32_t nw = div_up(n, 64);
-  const uint32_t nbx = (uint32_t)nb[0], nby = (uint32_t)nb[1], nbz = (uint32_t)nb[2];
-  const uint32_t zb = bit_width_u32(nbz - 1);
-  const uint32_t xybits = bit_width_u32(nbx * nby - 1);
-  const bool layered = nbz <= 64 && zb + xybits <= 31 && (uint64_t)nbz * nw <= (1ull << 26);
+  uint32_t lax = 0;
+  for (uint32_t k = 1; k < 3; k++)
+    if (nb[k] < nb[lax]) lax = k;
+  const uint32_t nbl = (uint32_t)nb[lax];
+  const uint32_t nbu = (uint32_t)nb[lax == 0 ? 1 : 0], nbv = (uint32_t)nb[lax == 2 ? 1 : 2];
+  const uint32_t lb = bit_width_u32(nbl - 1);
+  const uint32_t uvbits = bit_width_u32(nbu * nbv - 1);
+  const bool layered = nbl <= 64 && lb + uvbits <= 31 && (uint64_t)nbl * nw <= (1ull << 26);
   G.layered = layered ? 1u : 0u;
-  G.zb = layered ? zb : 0u;
+  G.lax = lax;
+  G.lb = layered ? lb : 0u;
   DeviceBuffer &counts = W.counts, &offs = W.offs, &scan_ws = W.scan_ws, &small = W.small;
-  const size_t nzc = (size_t)nbz * nw;
+  const size_t nzc = (size_t)nbl * nw;
   if (layered) {
     counts.reserve(nzc * 4);
     offs.reserve(nzc * 4);
@@ -830,9 +872,9 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   DeviceBuffer &keys = W.keys, &vals = W.vals, &tkeys = W.tkeys, &tvals = W.tvals, &sort_ws = W.sort_ws;
   DeviceBuffer &ranges = W.ranges;
   DeviceBuffer &masks = out.masks;
-  // list offsets per block key: total blocks, or the (y, x, z) key space
-  const uint32_t nbins = layered ? ((nbx * nby - 1u) << zb) + (1u << zb) : total;
-  const uint32_t key_bits = layered ? zb + xybits : std::max<uint32_t>(1, bit_width_u32(total - 1));
+  // list offsets per block key: total blocks, or the (v, u, l) key space
+  const uint32_t nbins = layered ? ((nbu * nbv - 1u) << lb) + (1u << lb) : total;
+  const uint32_t key_bits = layered ? lb + uvbits : std::max<uint32_t>(1, bit_width_u32(total - 1));
   keys.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
   vals.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
   tkeys.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
@@ -843,7 +885,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   const uint32_t *skeys = keys.as<uint32_t>(), *svals = vals.as<uint32_t>();
   if (npairs > 0) {
     if (layered)
-      k_vox_emit_layers<<<vgrid, 256, 0, s>>>(rec.as<VGauss>(), n, G, offs.as<uint32_t>(), nw, zb, npairs,
+      k_vox_emit_layers<<<vgrid, 256, 0, s>>>(rec.as<VGauss>(), n, G, offs.as<uint32_t>(), nw, lb, npairs,
                                               keys.as<uint32_t>(), vals.as<uint32_t>());
     else
       k_vox_emit<<<div_up(n, 256), 256, 0, s>>>(rec.as<VGauss>(), n, G, offs.as<uint32_t>(), keys.as<uint32_t>(),
@@ -851,7 +893,7 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     DL_HIP(hipGetLastError());
     const bool in_tmp = radix_sort_pairs(s, keys.as<uint32_t>(), vals.as<uint32_t>(), tkeys.as<uint32_t>(),
                                          tvals.as<uint32_t>(), npairs, (int)key_bits, sort_ws.get(), 9,
-                                         RadixKeyMap{}, false, layered ? (int)zb : 0);
+                                         RadixKeyMap{}, false, layered ? (int)lb : 0);
     if (in_tmp) {
       skeys = tkeys.as<uint32_t>();
       svals = tvals.as<uint32_t>();

@@ -95,8 +95,9 @@ def test_voxel_single_gaussian(device):
 
 
 def test_voxel_grid_wider_than_256_blocks(device):
-    """A long grid: more than 256 blocks along x (22 m at 0.08 m blocks)
-    and a few block layers, so the pairs take the layer route."""
+    """A long grid: more than 256 blocks along x (22 m at 0.08 m blocks),
+    5 along y and 8 along z, so the pairs take the layer route with the
+    layers along y."""
     data = _small_box_scene(6000, 5, box=((-11, 0, -0.3), (11, 0.4, 0.3)))
     got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
     assert got.num_blocks_x > 256
@@ -104,10 +105,41 @@ def test_voxel_grid_wider_than_256_blocks(device):
 
 
 def test_voxel_grid_with_many_layers(device):
-    """Many block layers along z (a 12 m column at 0.08 m blocks: 150
-    layers of a few blocks each): more than 64 layers, so the pairs take the
-    flat route (keyed by the flat block index, sorted on every bit)."""
+    """A 12 m column along z (150 blocks) a few blocks wide in x and y: the
+    layer route takes y, the axis of fewest blocks, whatever the others
+    hold."""
     data = _small_box_scene(4000, 6, box=((-0.2, 0, -6), (0.2, 0.3, 6)))
     got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
     assert got.num_blocks_z > 140
     assert got.buffer.count > 500
+
+
+@pytest.mark.parametrize("box", [((-0.15, 0, -1.5), (0.15, 3, 1.5)), ((-1.5, 0, -0.15), (1.5, 3, 0.15))],
+                         ids=["layers_along_x", "layers_along_z"])
+def test_voxel_layer_axis(device, box):
+    """Slabs thin in x, then in z (10 blocks against 43-44): the layer route
+    along x and along z, the key's other two axes ordered (y, z) and
+    (x, y)."""
+    data = _small_box_scene(5000, 8, box=box)
+    got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
+    dims = (got.num_blocks_x, got.num_blocks_y, got.num_blocks_z)
+    assert min(dims) <= 12 and max(dims) >= 40
+    assert got.buffer.count > 500
+
+
+def test_voxel_flat_route(device):
+    """Gaussians along the diagonal of a 5.6 m cube: 76 blocks on every
+    axis, more than the layer route's 64 layers, so the pairs are keyed by
+    the flat block index and sorted on every bit."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    t = rng.random(n)[:, None]
+    pos = (t * np.array([5.6, 5.6, 5.6]) + rng.normal(0, 0.05, (n, 3))).astype(np.float32)
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    ls = rng.uniform(math.log(0.005), math.log(0.08), (n, 3)).astype(np.float32)
+    op = rng.normal(0, 2, n).astype(np.float32)
+    data = SplatData(n, pos, q.astype(np.float32), ls, op, np.zeros(3 * n, np.float32))
+    got, ref = assert_voxel_parity(device, data, 0.02, 0.1)
+    assert min(got.num_blocks_x, got.num_blocks_y, got.num_blocks_z) > 64
+    assert got.buffer.count > 1000
