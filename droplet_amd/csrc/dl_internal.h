@@ -114,8 +114,11 @@ struct VoxelScratch {
   DeviceBuffer keys, vals, tkeys, tvals, sort_ws, ranges;
   // compaction of non-empty blocks
   DeviceBuffer flags, fpos, omask, oidx, cscan_ws, csmall;
-  // timing events of a voxelize call (created on first use)
+  // timing events of a voxelize call and a pinned host block for its small
+  // readbacks (the bounds, the pair totals), created on first use
   hipEvent_t ev[4] = {};
+  static constexpr size_t kHostSmallBytes = 1024;
+  void *host_small = nullptr;
   void ensure_events();
 };
 

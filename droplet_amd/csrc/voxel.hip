@@ -653,11 +653,13 @@ double js_round(double x) {
 VoxelScratch::~VoxelScratch() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (host_small) (void)hipHostFree(host_small);
 }
 
 void VoxelScratch::ensure_events() {
   for (auto &e : ev)
     if (!e) DL_HIP(hipEventCreate(&e));
+  if (!host_small) DL_HIP(hipHostMalloc(&host_small, kHostSmallBytes, hipHostMallocDefault));
 }
 
 void voxel_compact(Context &c, hipStream_t s, VoxelResult &r, hipEvent_t done) {
@@ -748,10 +750,15 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     DL_HIP(hipGetLastError());
     k_vox_bounds<<<1, 1024, 0, s>>>(partials.as<double>(), nprep, partials.as<double>() + (size_t)nprep * 6);
     DL_HIP(hipGetLastError());
+    // the pair total's slots are cleared before the host waits for the
+    // bounds, not after (one enqueue less between the wait and the count)
+    W.small.reserve(64 + kTotalSlots * sizeof(unsigned long long));
+    DL_HIP(hipMemsetAsync(W.small.as<uint8_t>() + 64, 0, kTotalSlots * sizeof(unsigned long long), s));
     part.resize(6);
-    DL_HIP(hipMemcpyAsync(part.data(), partials.as<double>() + (size_t)nprep * 6, 6 * sizeof(double),
+    DL_HIP(hipMemcpyAsync(W.host_small, partials.as<double>() + (size_t)nprep * 6, 6 * sizeof(double),
                           hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
+    std::memcpy(part.data(), W.host_small, 6 * sizeof(double));
   } else {
     DL_HIP(hipEventRecord(e0, s));
   }
@@ -846,9 +853,8 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     scan_ws.reserve(scan_workspace_bytes(n));
   }
   // small: [0] the scan's u32 total; from byte 64 the u64 total's slots
-  small.reserve(64 + kTotalSlots * sizeof(unsigned long long));
+  // (cleared above)
   unsigned long long *d_total64 = reinterpret_cast<unsigned long long *>(small.as<uint8_t>() + 64);
-  DL_HIP(hipMemsetAsync(d_total64, 0, kTotalSlots * sizeof(unsigned long long), s));
   // the layer route deals contiguous runs of chunks to each XCD (xcd_chunk)
   const uint32_t vgrid = layered ? 8u * div_up(div_up(n, 256), 8) : div_up(n, 256);
   k_vox_count<<<vgrid, 256, 0, s>>>(rec.as<VGauss>(), n, G, layered ? nullptr : counts.as<uint32_t>(), d_total64,
@@ -859,8 +865,10 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   exclusive_scan_u32(s, counts.as<uint32_t>(), offs.as<uint32_t>(), layered ? nzc : n, small.as<uint32_t>(),
                      scan_ws.get());
   uint32_t head[16 + 2 * kTotalSlots];
-  DL_HIP(hipMemcpyAsync(head, small.get(), sizeof(head), hipMemcpyDeviceToHost, s));
+  static_assert(sizeof(head) <= VoxelScratch::kHostSmallBytes, "pinned readback too small");
+  DL_HIP(hipMemcpyAsync(W.host_small, small.get(), sizeof(head), hipMemcpyDeviceToHost, s));
   DL_HIP(hipStreamSynchronize(s));
+  std::memcpy(head, W.host_small, sizeof(head));
   unsigned long long total_pairs = 0;
   for (uint32_t k = 0; k < kTotalSlots; k++)
     total_pairs += (unsigned long long)head[16 + 2 * k] | ((unsigned long long)head[17 + 2 * k] << 32);
@@ -881,7 +889,6 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
   tvals.reserve((size_t)std::max<uint32_t>(npairs, 1) * 4);
   sort_ws.reserve(radix_workspace_bytes(std::max<uint32_t>(npairs, 1), (int)key_bits));
   ranges.reserve(((size_t)nbins + 1) * 4);
-  masks.reserve((size_t)total * 8);
   const uint32_t *skeys = keys.as<uint32_t>(), *svals = vals.as<uint32_t>();
   if (npairs > 0) {
     if (layered)
@@ -900,6 +907,9 @@ static void voxelize_impl(Context &ctx, const dl_splats &sp, const dl_voxel_opts
     }
   }
   bin_ranges(s, skeys, npairs, nbins, ranges.as<uint32_t>());
+  // the result's masks are allocated (a new result per call) once the pair
+  // build and sort are queued, so the allocation runs while they do
+  masks.reserve((size_t)total * 8);
   hipEvent_t ev_a = W.ev[2], ev_b = W.ev[3];
   DL_HIP(hipEventRecord(ev_a, s));
   k_vox_eval<<<total, 64, 0, s>>>(rec.as<VGauss>(), ranges.as<uint32_t>(), svals, G, masks.as<uint2>());
