@@ -195,6 +195,12 @@ struct Renderer {
   std::array<std::vector<uint8_t>, 8> taps;
   std::vector<uint32_t> tap_rank;  // splat id -> depth rank (taps only)
   std::vector<hipEvent_t> events;
+  // pinned host block of the frame's small readbacks (the totals read after
+  // the frame, an exact frame's visible and pair totals): the runtime
+  // returns a readback into pageable memory about 30 us later
+  static constexpr size_t kHostSmallBytes = 256;
+  void *host_small = nullptr;
+  void *pinned_small();
   std::vector<int> ev_stage;
   size_t ev_used = 0;
 

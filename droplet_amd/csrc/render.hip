@@ -592,6 +592,11 @@ void Renderer::ensure_events(size_t n) {
   }
 }
 
+void *Renderer::pinned_small() {
+  if (!host_small) DL_HIP(hipHostMalloc(&host_small, kHostSmallBytes, hipHostMallocDefault));
+  return host_small;
+}
+
 void Renderer::mark(int stage) {
   if (!timing) return;
   ensure_events(ev_used + 1);
@@ -803,10 +808,14 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
       nvis = n;
       pairs_all = spec_cap;
     } else {
-      uint32_t head[8];
-      DL_HIP(hipMemcpyAsync(head, d_nvis, sizeof(head), hipMemcpyDeviceToHost, s));
-      DL_HIP(hipMemcpyAsync(&pairs_all, d_pairs_all, sizeof(pairs_all), hipMemcpyDeviceToHost, s));
+      // small[0..7] and the pair total after them, in one pinned read
+      uint32_t head[10];
+      static_assert(sizeof(head) <= kHostSmallBytes, "pinned readback too small");
+      void *hs = pinned_small();
+      DL_HIP(hipMemcpyAsync(hs, d_nvis, sizeof(head), hipMemcpyDeviceToHost, s));
       DL_HIP(hipStreamSynchronize(s));
+      std::memcpy(head, hs, sizeof(head));
+      pairs_all = (unsigned long long)head[8] | ((unsigned long long)head[9] << 32);
       nvis = head[0];
       if (T != kTile && head[7]) {
         // maxCoverage truncates a splat at 16-px tiles: only they give the
@@ -1110,8 +1119,11 @@ void Renderer::render(const SceneView &sc, const dl_camera &cam, const dl_backgr
     // small: [0] visible, [2..3] key span, [7] coverage flag, [8..9] all
     // pairs, [10..15] loop counters, [16..21] tallies
     uint32_t sm[22] = {};
-    DL_HIP(hipMemcpyAsync(sm, d_nvis, sizeof(sm), hipMemcpyDeviceToHost, s));
+    static_assert(sizeof(sm) <= kHostSmallBytes, "pinned readback too small");
+    void *hs = pinned_small();
+    DL_HIP(hipMemcpyAsync(hs, d_nvis, sizeof(sm), hipMemcpyDeviceToHost, s));
     DL_HIP(hipStreamSynchronize(s));
+    std::memcpy(sm, hs, sizeof(sm));
     auto u64_at = [&](int k) { return (unsigned long long)sm[k] | ((unsigned long long)sm[k + 1] << 32); };
     if (counting) {
       stats.pixel_iters = u64_at(10);
@@ -1206,6 +1218,7 @@ void Renderer::read_image(uint8_t *out) {
 
 Renderer::~Renderer() {
   for (auto e : events) (void)hipEventDestroy(e);
+  if (host_small) (void)hipHostFree(host_small);
   if (order_stream) {
     (void)hipStreamSynchronize(order_stream);
     (void)hipStreamDestroy(order_stream);
