@@ -73,6 +73,7 @@ struct Batch {
   DeviceBuffer d_views, d_l1, d_scenes, d_seg, d_parts, d_pairs;
   DeviceBuffer keys, vals, tkeys, tvals, rec, sort_ws, rects, pvals, ranges, images, small;
   DeviceBuffer accum, closed, open_pre;  // passes: accumulator, closed tiles, open-tile map
+  DeviceBuffer live_sat, live_csat;      // passes: the open tiles' summed-area tables
   size_t vtiles_max = 0;
   uint32_t passes_max = 1;
   BinWorkspace bin_ws;

@@ -182,7 +182,7 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
   nv_max = 0;
   passes_max = 1;
   uint32_t blocks_max = 0;
-  size_t rows_max = 0;
+  size_t rows_max = 0, sat_max = 0, csat_max = 0;
   vtiles_max = 0;
   // front-to-back passes of a view of n splats, on the single-view
   // renderer's schedule (one pass with full_lists); the batch always uses
@@ -281,6 +281,8 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
     blocks_max = std::max(blocks_max, g.cull_blocks);
     vtiles_max = std::max(vtiles_max, (size_t)g.rows * g.cols);
     rows_max = std::max<size_t>(rows_max, g.rows);
+    sat_max = std::max<size_t>(sat_max, ((size_t)g.rows + 1) * (g.cols + 1));
+    csat_max = std::max<size_t>(csat_max, ((size_t)(g.rows + 3) / 4 + 1) * ((g.cols + 3) / 4 + 1));
     groups.push_back(g);
   }
   hpairs.assign(groups.size(), 0);
@@ -308,6 +310,8 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
     accum.reserve(std::max<uint64_t>(px_total, 1) * 16);
     closed.reserve(std::max<size_t>(vtiles_max, 1));
     open_pre.reserve(std::max<size_t>(vtiles_max + rows_max, 1) * 2);
+    live_sat.reserve(sat_max * 4);
+    live_csat.reserve(csat_max * 4);
   }
   DL_HIP(hipStreamSynchronize(s));
   stats.num_views = nviews;
@@ -355,7 +359,9 @@ void Batch::enqueue(bool eager) {
     }
     for (uint32_t p = 0; p < passes; p++) {
       const L1View *tab = d_l1.as<L1View>() + g.l1_off[p];
-      batch_rects(s, rec.as<float4>(), sorted, tab, g.nv, g.blocks[p], rects.as<uint4>());
+      const LiveMap live = p > 0 ? LiveMap{open_pre.as<uint16_t>(), g.cols + 1} : LiveMap{};
+      batch_rects(s, rec.as<float4>(), sorted, tab, g.nv, g.blocks[p], rects.as<uint4>(), live, g.rows, g.cols,
+                  live_sat.as<uint32_t>(), live_csat.as<uint32_t>());
       SpanBinArgs a;
       a.rects = rects.as<uint4>();
       a.tab = tab;

@@ -110,9 +110,14 @@ struct L1View {
 
 // Packed rects of every view of a batch: rects[item0 + j] for the j-th
 // depth-ordered item (ids[item0 + j], an index of rec) of each view; blocks
-// of 256 items as the level-1 blocks.
+// of 256 items as the level-1 blocks. With live.pre (a pass after the
+// first; rows x cols is the stacked image) the open tiles' summed-area
+// table is built into sat ((rows + 1) x (cols + 1)) and csat
+// ((ceil(rows / 4) + 1) x (ceil(cols / 4) + 1)), and splats without an open
+// tile get empty rects.
 void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
-                 uint32_t blocks, uint4 *rects);
+                 uint32_t blocks, uint4 *rects, const LiveMap &live, uint32_t rows, uint32_t cols, uint32_t *sat,
+                 uint32_t *csat);
 
 struct BinWorkspace {
   // level 1

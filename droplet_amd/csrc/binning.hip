@@ -100,18 +100,76 @@ __global__ __launch_bounds__(256) void k_open_sat(const uint16_t *__restrict__ p
   }
 }
 
+// The same table for a tall stacked image (a batch group's views): one
+// workgroup per column x <= cols scans that column down the rows (a single
+// workgroup walking thousands of rows per column would take a millisecond),
+// then k_open_csat samples the 4x4-block corners.
+__global__ __launch_bounds__(256) void k_open_sat_cols(const uint16_t *__restrict__ pre, uint32_t rows, uint32_t cols,
+                                                       uint32_t *__restrict__ sat) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t x = blockIdx.x, stride = cols + 1;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sat[x] = 0;
+  uint32_t carry = 0;
+  for (uint32_t y0 = 0; y0 < rows; y0 += 256) {
+    const uint32_t y = y0 + threadIdx.x;
+    const uint32_t v = y < rows ? (uint32_t)pre[(size_t)y * stride + x] : 0u;
+    uint32_t inc = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)inc, d, 64);
+      if (lane >= d) inc += t;
+    }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t before = carry;
+    for (uint32_t w = 0; w < wave; w++) before += s_w[w];
+    if (y < rows) sat[(size_t)(y + 1) * stride + x] = before + inc;
+    carry += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_open_csat(const uint32_t *__restrict__ sat, uint32_t rows, uint32_t cols,
+                                                   uint32_t *__restrict__ csat) {
+  const uint32_t stride = cols + 1, cstride = (cols + 3) / 4 + 1, crows = (rows + 3) / 4 + 1;
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= crows * cstride) return;
+  const uint32_t cy = k / cstride, cx = k % cstride;
+  csat[k] = sat[(size_t)min(4 * cy, rows) * stride + min(4 * cx, cols)];
+}
+
 // The compaction kernels give each thread kLiveItems items (one per 256-item
 // sub-block, loads issued together): the per-item work is a few loads, so
 // one item per thread leaves them latency-bound.
 constexpr int kLiveItems = 8;
 
-// bit i of bits: splat i (id order, packed rects of the projection) is live:
-// an open tile in its full rows (the summed-area table) or in its partial
-// last row (the row prefix)
+// Does a splat's rect (rows from y0 of the live map's virtual image) hold an
+// open tile: in its full rows (the summed-area table) or in its partial last
+// row (the row prefix)? The 4x4-block table first: no open tile in the
+// blocks that cover the rect (rem row included) means none in the rect.
+__device__ inline bool rect_live(const Rect &R, const LiveMap &live) {
+  if (R.nrows == 0) return false;
+  const uint32_t st = live.stride, cs = live.cstride;
+  const uint32_t cy0 = R.y0 >> 2, cy1 = (R.y0 + R.nrows + 3u) >> 2, cx0 = R.x0 >> 2, cx1 = (R.x1 + 4u) >> 2;
+  const bool maybe = live.csat[cy1 * cs + cx1] - live.csat[cy0 * cs + cx1] - live.csat[cy1 * cs + cx0] +
+                         live.csat[cy0 * cs + cx0] != 0u;
+  bool lv = false;
+  if (maybe && R.full_rows > 0) {
+    const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
+    const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
+                          live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
+    lv = open != 0;
+  }
+  if (maybe && !lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
+  return lv;
+}
+
+// bit i of bits: splat i (id order, packed rects of the projection) is live
+// (rect_live)
 __global__ __launch_bounds__(256) void k_live_bits(const uint2 *__restrict__ rect8, uint32_t n, LiveMap live,
                                                    unsigned long long *__restrict__ bits) {
   const size_t base = (size_t)blockIdx.x * 256 * kLiveItems + threadIdx.x;
-  const uint32_t st = live.stride;
   uint2 r[kLiveItems];
 #pragma unroll
   for (int k = 0; k < kLiveItems; k++) {
@@ -121,24 +179,7 @@ __global__ __launch_bounds__(256) void k_live_bits(const uint2 *__restrict__ rec
 #pragma unroll
   for (int k = 0; k < kLiveItems; k++) {
     const size_t i = base + (size_t)k * 256;
-    const Rect R = unpack_rect8(r[k]);
-    // the 4x4-block table first: no open tile in the blocks that cover the
-    // rect (rem row included) means none in the rect
-    bool maybe = false;
-    if (R.nrows > 0) {
-      const uint32_t cs = live.cstride;
-      const uint32_t cy0 = R.y0 >> 2, cy1 = (R.y0 + R.nrows + 3u) >> 2, cx0 = R.x0 >> 2, cx1 = (R.x1 + 4u) >> 2;
-      maybe = live.csat[cy1 * cs + cx1] - live.csat[cy0 * cs + cx1] - live.csat[cy1 * cs + cx0] +
-                  live.csat[cy0 * cs + cx0] != 0u;
-    }
-    bool lv = false;
-    if (maybe && R.full_rows > 0) {
-      const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
-      const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
-                            live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
-      lv = open != 0;
-    }
-    if (maybe && !lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
+    const bool lv = rect_live(unpack_rect8(r[k]), live);
     const unsigned long long b = __ballot(lv);
     const size_t first = i - (size_t)__lane_id();
     if (__lane_id() == 0 && first < n) bits[first >> 6] = b;
@@ -245,14 +286,23 @@ __device__ inline L1View l1_view(const L1View &one, const L1View *__restrict__ t
   return tab[lo];
 }
 
+// With live.sat set (a pass after the first), a splat without an open tile
+// gets an empty rect: level 1 then reads its record and gives it no span,
+// which is what its per-row tests would find, without testing each row.
 __global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
-                                                     const L1View *__restrict__ tab, uint32_t ntab,
+                                                     const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
                                                      uint4 *__restrict__ rects) {
   const L1View v = l1_view(L1View{}, tab, ntab, blockIdx.x);
   const uint32_t jj = (blockIdx.x - v.blk0) * kB1 + threadIdx.x;
   if (jj >= v.n) return;
   const uint32_t id = ids[v.item0 + jj];
-  rects[v.item0 + jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], grid16(v.tiles_x, v.rows), v.max_cov));
+  Rect R = splat_rect(rec[3 * (size_t)id], grid16(v.tiles_x, v.rows), v.max_cov);
+  if (live.sat) {
+    Rect Rv = R;
+    Rv.y0 += v.row0;  // the view's rows in the stacked image
+    if (!rect_live(Rv, live)) R = Rect{0, 0, 0, 0, 0, 0};
+  }
+  rects[v.item0 + jj] = pack_rect(id, R);
 }
 
 // ---------------------------- level 1 -------------------------------------
@@ -612,9 +662,13 @@ __global__ __launch_bounds__(256) void k_bucket_offsets(const uint32_t *__restri
 // 1024-thread workgroup: each thread owns 4 consecutive segments.
 constexpr uint32_t kSmallSegs = 4096;
 
+// The chunk bases also go to s_cb (LDS, nseg + 1 entries), and then every
+// thread fills seg_of for chunks t, t + 1024, ... by a binary search of
+// s_cb (a thread filling its own segments' chunks left a few threads
+// writing hundreds of entries each: 11.9 us for the 68 rows of a 4K pass).
 __device__ inline void chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg, uint32_t cap,
                                          uint32_t *__restrict__ chunk_base, uint32_t *__restrict__ seg_of,
-                                         uint32_t *s_wave) {
+                                         uint32_t *s_wave, uint32_t *s_cb) {
   const uint32_t t = threadIdx.x, s0 = 4 * t;
   uint32_t nch[4], sum = 0;
 #pragma unroll
@@ -643,19 +697,31 @@ __device__ inline void chunk_table_small(const uint32_t *__restrict__ seg_start,
     const uint32_t sg = s0 + k;
     if (sg < nseg) {
       chunk_base[sg] = run;
-      const uint32_t e = min(run + nch[k], cap);
-      for (uint32_t c = run; c < e; c++) seg_of[c] = sg;
+      s_cb[sg] = run;
     }
     run += nch[k];
   }
   if (t == 0) chunk_base[nseg] = total;
+  __syncthreads();
+  // chunk c belongs to the last segment whose base is <= c (a segment
+  // without chunks shares the next one's base and is passed over)
+  const uint32_t e = min(total, cap);
+  for (uint32_t c = t; c < e; c += 1024) {
+    uint32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_cb[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    seg_of[c] = lo;
+  }
 }
 
 __global__ __launch_bounds__(1024) void k_chunk_table_small(const uint32_t *__restrict__ seg_start, uint32_t nseg,
                                                             uint32_t cap, uint32_t *__restrict__ chunk_base,
                                                             uint32_t *__restrict__ seg_of) {
   __shared__ uint32_t s_wave[16];
-  chunk_table_small(seg_start, nseg, cap, chunk_base, seg_of, s_wave);
+  __shared__ uint32_t s_cb[kSmallSegs];
+  chunk_table_small(seg_start, nseg, cap, chunk_base, seg_of, s_wave, s_cb);
 }
 
 // Level 1's row starts, then level 2a's chunk table over them, in one
@@ -668,10 +734,11 @@ __global__ __launch_bounds__(1024) void k_row_start_chunks(const uint32_t *__res
                                                           uint32_t *__restrict__ chunk_base,
                                                           uint32_t *__restrict__ seg_of) {
   __shared__ uint32_t s_wave[16];
+  __shared__ uint32_t s_cb[kSmallSegs];
   for (uint32_t r = threadIdx.x; r <= rows; r += 1024)
     row_start_one(r, base1, n1, one, tab, ntab, rows, d_items, d_num_spans, cap, row_start);
   __syncthreads();
-  chunk_table_small(row_start, rows, max_chunks, chunk_base, seg_of, s_wave);
+  chunk_table_small(row_start, rows, max_chunks, chunk_base, seg_of, s_wave, s_cb);
 }
 
 // Level 2a's group offsets (the segment starts of level 2b), then level 2b's
@@ -684,11 +751,12 @@ __global__ __launch_bounds__(1024) void k_offsets_chunks(const uint32_t *__restr
                                                         uint32_t max_ch_b, uint32_t *__restrict__ chunk_base_b,
                                                         uint32_t *__restrict__ seg_of_b) {
   __shared__ uint32_t s_wave[16];
+  __shared__ uint32_t s_cb[kSmallSegs];
   for (uint32_t o = threadIdx.x; o <= nseg_b; o += 1024)
     bucket_offset_one(o, base, chunk_base_a, rows, max_ch_a, nseg_b, groups, tiles_x, 0, d_total, cap, seg_start_b,
                       nullptr);
   __syncthreads();
-  chunk_table_small(seg_start_b, nseg_b, max_ch_b, chunk_base_b, seg_of_b, s_wave);
+  chunk_table_small(seg_start_b, nseg_b, max_ch_b, chunk_base_b, seg_of_b, s_wave, s_cb);
 }
 
 // Chunk table of one MBS level, built on the device: chunks per segment,
@@ -831,9 +899,22 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
 }
 
 void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
-                 uint32_t blocks, uint4 *rects) {
+                 uint32_t blocks, uint4 *rects, const LiveMap &live, uint32_t rows, uint32_t cols, uint32_t *sat,
+                 uint32_t *csat) {
   if (!blocks) return;
-  k_batch_rects<<<blocks, kB1, 0, s>>>(rec, ids, tab, ntab, rects);
+  LiveMap lm{};
+  if (live.pre) {
+    // the open tiles' summed-area table of the stacked image
+    lm = live;
+    lm.sat = sat;
+    lm.csat = csat;
+    lm.cstride = div_up(cols, 4) + 1;
+    k_open_sat_cols<<<cols + 1, 256, 0, s>>>(live.pre, rows, cols, sat);
+    DL_HIP(hipGetLastError());
+    k_open_csat<<<div_up((size_t)(div_up(rows, 4) + 1) * lm.cstride, 256), 256, 0, s>>>(sat, rows, cols, csat);
+    DL_HIP(hipGetLastError());
+  }
+  k_batch_rects<<<blocks, kB1, 0, s>>>(rec, ids, tab, ntab, lm, rects);
   DL_HIP(hipGetLastError());
 }
 
