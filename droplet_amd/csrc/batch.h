@@ -26,6 +26,7 @@ struct BView {
   uint32_t row0;       // first tile row in the group's virtual image
   uint32_t width, height, tiles_x, tiles_y;
   uint64_t px0;        // first pixel of the view's image
+  uint32_t bit0;       // first 64-bit word of the view's live bits (passes after the first)
 };
 
 // Views [v0, v0 + nv) rendered by one launch of every stage. Binning and
@@ -41,6 +42,7 @@ struct BatchGroup {
   std::vector<uint32_t> blocks;
   std::vector<size_t> n1;
   uint32_t pair_cap = 0;   // capacity of the group's tile lists (any pass)
+  uint32_t bit_words = 0;  // live-bit words of all its views
 };
 
 // (tile, splat) pairs a group's lists may hold (the pair total of all its
@@ -73,7 +75,9 @@ struct Batch {
   DeviceBuffer d_views, d_l1, d_scenes, d_seg, d_parts, d_pairs;
   DeviceBuffer keys, vals, tkeys, tvals, rec, sort_ws, rects, pvals, ranges, images, small;
   DeviceBuffer accum, closed, open_pre;  // passes: accumulator, closed tiles, open-tile map
-  DeviceBuffer live_sat, live_csat;      // passes: the open tiles' summed-area tables
+  // passes after the first: the open tiles' summed-area tables, packed
+  // rects in item order, live bits, per-block live counts and their scan
+  DeviceBuffer live_sat, live_csat, rect8, live_bits, live_cnt, live_pre, live_scan_ws;
   size_t vtiles_max = 0;
   uint32_t passes_max = 1;
   BinWorkspace bin_ws;

@@ -46,7 +46,8 @@ __global__ __launch_bounds__(kItemBlock) void k_batch_cull(const BView *__restri
                                                            const SceneView *__restrict__ scenes,
                                                            uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                            float4 *__restrict__ rec,
-                                                           unsigned long long *__restrict__ parts) {
+                                                           unsigned long long *__restrict__ parts,
+                                                           uint2 *__restrict__ rect8, uint32_t max_cov) {
   __shared__ unsigned long long s_pairs[kItemBlock / 64];
   const BView &V = views[view_of_block(views, nv, blockIdx.x, false)];
   const uint32_t i = (blockIdx.x - V.blk0) * kItemBlock + threadIdx.x;
@@ -62,6 +63,9 @@ __global__ __launch_bounds__(kItemBlock) void k_batch_cull(const BView *__restri
       case 2: cull_project_one<2>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
       default: cull_project_one<3>(sc, u, V.near_plane, V.far_plane, i, o, (uint32_t)o, keys, vals, rec, nullptr, vis, count, cov16); break;
     }
+    // the packed rect in the view's grid (rows from the view's first), for
+    // the live tests of later passes
+    if (rect8) rect8[o] = pack_rect8(splat_rect(rec[3 * o], grid16(V.tiles_x, V.tiles_y), max_cov));
   }
   unsigned long long pc = count;
 #pragma unroll
@@ -74,6 +78,92 @@ __global__ __launch_bounds__(kItemBlock) void k_batch_cull(const BView *__restri
     for (int w = 0; w < kItemBlock / 64; w++) t += s_pairs[w];
     parts[blockIdx.x] = t;
   }
+}
+
+// ---- live-splat compaction of a pass after the first, per view (the
+// single-view renderer's k_live_bits / k_live_count / k_rects_live) ----
+
+// bit i of view V's words (from V.bit0): item V.item0 + i touches an open
+// tile of the stacked image (rect_live)
+__global__ __launch_bounds__(kItemBlock) void k_batch_live_bits(const BView *__restrict__ views, uint32_t nv,
+                                                                const uint2 *__restrict__ rect8, LiveMap live,
+                                                                unsigned long long *__restrict__ bits) {
+  const BView &V = views[view_of_block(views, nv, blockIdx.x, false)];
+  const uint32_t i = (blockIdx.x - V.blk0) * kItemBlock + threadIdx.x;
+  bool lv = false;
+  if (i < V.n) {
+    Rect R = unpack_rect8(rect8[(size_t)V.item0 + i]);
+    R.y0 += V.row0;
+    lv = rect_live(R, live);
+  }
+  const unsigned long long b = __ballot(lv);
+  const uint32_t first = i - (uint32_t)__lane_id();
+  if (__lane_id() == 0 && first < V.n) bits[(size_t)V.bit0 + (first >> 6)] = b;
+}
+
+// index of the level-1 view that owns block b (views ordered by first block)
+__device__ inline uint32_t l1_index(const L1View *__restrict__ tab, uint32_t ntab, uint32_t b) {
+  uint32_t lo = 0, hi = ntab - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (tab[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline bool item_live(const unsigned long long *__restrict__ bits, const L1View &v, uint32_t o) {
+  const uint32_t li = o - v.vitem0;
+  return (bits[(size_t)v.bit0 + (li >> 6)] >> (li & 63u)) & 1ull;
+}
+
+// live items of every level-1 block of the pass
+__global__ __launch_bounds__(kItemBlock) void k_batch_live_count(const uint32_t *__restrict__ sorted,
+                                                                 const L1View *__restrict__ tab, uint32_t ntab,
+                                                                 const unsigned long long *__restrict__ bits,
+                                                                 uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t s_w[kItemBlock / 64];
+  const L1View v = tab[l1_index(tab, ntab, blockIdx.x)];
+  const uint32_t jj = (blockIdx.x - v.blk0) * kItemBlock + threadIdx.x;
+  const bool lv = jj < v.n && item_live(bits, v, sorted[(size_t)v.item0 + jj]);
+  const unsigned long long b = __ballot(lv);
+  if (__lane_id() == 0) s_w[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// The live items of each view moved, in depth order, to the front of the
+// view's part of the pass as level 1 reads them (id and rect), and the
+// view's live count into its table entry (n_live); pre is the exclusive
+// scan of the block counts with the total after it.
+__global__ __launch_bounds__(kItemBlock) void k_batch_rects_live(const uint32_t *__restrict__ sorted,
+                                                                 L1View *__restrict__ tab, uint32_t ntab,
+                                                                 const unsigned long long *__restrict__ bits,
+                                                                 const uint2 *__restrict__ rect8,
+                                                                 const uint32_t *__restrict__ pre,
+                                                                 uint4 *__restrict__ rects) {
+  __shared__ uint32_t s_w[kItemBlock / 64];
+  const uint32_t vi = l1_index(tab, ntab, blockIdx.x);
+  const uint32_t item0 = tab[vi].item0, n = tab[vi].n, blk0 = tab[vi].blk0;
+  L1View v{};
+  v.vitem0 = tab[vi].vitem0;
+  v.bit0 = tab[vi].bit0;
+  const uint32_t kb = blockIdx.x - blk0;
+  const uint32_t jj = kb * kItemBlock + threadIdx.x;
+  uint32_t o = 0;
+  bool lv = false;
+  if (jj < n) {
+    o = sorted[(size_t)item0 + jj];
+    lv = item_live(bits, v, o);
+  }
+  const unsigned long long b = __ballot(lv);
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) s_w[wave] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t pos = pre[blockIdx.x] - pre[blk0];
+  for (int w = 0; w < wave; w++) pos += s_w[w];
+  pos += (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+  if (lv) rects[(size_t)item0 + pos] = pack_rect(o, unpack_rect8(rect8[o]));
+  if (kb == 0 && threadIdx.x == 0) tab[vi].n_live = pre[blk0 + (n + kItemBlock - 1) / kItemBlock] - pre[blk0];
 }
 
 __global__ __launch_bounds__(1024) void k_sum_pairs(const unsigned long long *__restrict__ parts, uint32_t n,
@@ -181,7 +271,7 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
   items_max = 0;
   nv_max = 0;
   passes_max = 1;
-  uint32_t blocks_max = 0;
+  uint32_t blocks_max = 0, pass_blocks_max = 0, words_max = 0;
   size_t rows_max = 0, sat_max = 0, csat_max = 0;
   vtiles_max = 0;
   // front-to-back passes of a view of n splats, on the single-view
@@ -232,6 +322,8 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
       b.tiles_x = div_up(c.width, kTile);
       b.tiles_y = div_up(c.height, kTile);
       b.px0 = px_total;
+      b.bit0 = g.bit_words;
+      g.bit_words += div_up(n, 64);
       hseg.push_back(b.item0);
       cuts.push_back(cuts_of(n));
       items += n;
@@ -266,6 +358,9 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
         l.c1_base = (uint32_t)n1;
         l.tiles_x = b.tiles_x;
         l.max_cov = max_cov;
+        l.n_live = l.n;
+        l.vitem0 = b.item0;
+        l.bit0 = b.bit0;
         hl1.push_back(l);
         const uint32_t nb = div_up(l.n, kItemBlock);
         blocks += nb;
@@ -274,11 +369,13 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
       if (n1 > 0xFFFFFFF0ull) fail(DL_ERR_CAPACITY, "batch group too large");
       g.blocks.push_back(blocks);
       g.n1.push_back(n1);
+      pass_blocks_max = std::max(pass_blocks_max, blocks);
     }
     items_max = std::max(items_max, g.items);
     nv_max = std::max(nv_max, g.nv);
     passes_max = std::max<uint32_t>(passes_max, (uint32_t)passes);
     blocks_max = std::max(blocks_max, g.cull_blocks);
+    words_max = std::max(words_max, g.bit_words);
     vtiles_max = std::max(vtiles_max, (size_t)g.rows * g.cols);
     rows_max = std::max<size_t>(rows_max, g.rows);
     sat_max = std::max<size_t>(sat_max, ((size_t)g.rows + 1) * (g.cols + 1));
@@ -312,6 +409,11 @@ void Batch::plan(const std::vector<SceneView> &scenes, const dl_view *views, uin
     open_pre.reserve(std::max<size_t>(vtiles_max + rows_max, 1) * 2);
     live_sat.reserve(sat_max * 4);
     live_csat.reserve(csat_max * 4);
+    rect8.reserve(im * 8);
+    live_bits.reserve(((size_t)words_max + 1) * 8);
+    live_cnt.reserve(((size_t)pass_blocks_max + 1) * 4);
+    live_pre.reserve(((size_t)pass_blocks_max + 1) * 4);
+    live_scan_ws.reserve(scan_workspace_bytes(std::max<uint32_t>(pass_blocks_max, 1)));
   }
   DL_HIP(hipStreamSynchronize(s));
   stats.num_views = nviews;
@@ -332,7 +434,8 @@ void Batch::enqueue(bool eager) {
     const BView *gv = d_views.as<BView>() + g.v0;
     if (g.cull_blocks) {
       k_batch_cull<<<g.cull_blocks, kItemBlock, 0, s>>>(gv, g.nv, d_sc, keys.as<uint32_t>(), vals.as<uint32_t>(),
-                                                        rec.as<float4>(), d_parts.as<unsigned long long>());
+                                                        rec.as<float4>(), d_parts.as<unsigned long long>(),
+                                                        g.blocks.size() > 1 ? rect8.as<uint2>() : nullptr, max_cov);
       DL_HIP(hipGetLastError());
     }
     k_sum_pairs<<<1, 1024, 0, s>>>(d_parts.as<unsigned long long>(), g.cull_blocks, d_gp + gi);
@@ -358,10 +461,24 @@ void Batch::enqueue(bool eager) {
       DL_HIP(hipGetLastError());
     }
     for (uint32_t p = 0; p < passes; p++) {
-      const L1View *tab = d_l1.as<L1View>() + g.l1_off[p];
-      const LiveMap live = p > 0 ? LiveMap{open_pre.as<uint16_t>(), g.cols + 1} : LiveMap{};
-      batch_rects(s, rec.as<float4>(), sorted, tab, g.nv, g.blocks[p], rects.as<uint4>(), live, g.rows, g.cols,
-                  live_sat.as<uint32_t>(), live_csat.as<uint32_t>());
+      L1View *tab = d_l1.as<L1View>() + g.l1_off[p];
+      if (p == 0) {
+        batch_rects(s, rec.as<float4>(), sorted, tab, g.nv, g.blocks[p], rects.as<uint4>());
+      } else if (g.blocks[p]) {
+        // only the splats that touch an open tile, compacted per view
+        const LiveMap lm = open_sat_tall(s, open_pre.as<uint16_t>(), g.rows, g.cols, live_sat.as<uint32_t>(),
+                                         live_csat.as<uint32_t>());
+        unsigned long long *bits = live_bits.as<unsigned long long>();
+        k_batch_live_bits<<<g.cull_blocks, kItemBlock, 0, s>>>(gv, g.nv, rect8.as<uint2>(), lm, bits);
+        DL_HIP(hipGetLastError());
+        k_batch_live_count<<<g.blocks[p], kItemBlock, 0, s>>>(sorted, tab, g.nv, bits, live_cnt.as<uint32_t>());
+        DL_HIP(hipGetLastError());
+        exclusive_scan_u32(s, live_cnt.as<uint32_t>(), live_pre.as<uint32_t>(), g.blocks[p],
+                           live_pre.as<uint32_t>() + g.blocks[p], live_scan_ws.get());
+        k_batch_rects_live<<<g.blocks[p], kItemBlock, 0, s>>>(sorted, tab, g.nv, bits, rect8.as<uint2>(),
+                                                              live_pre.as<uint32_t>(), rects.as<uint4>());
+        DL_HIP(hipGetLastError());
+      }
       SpanBinArgs a;
       a.rects = rects.as<uint4>();
       a.tab = tab;

@@ -98,6 +98,35 @@ struct LiveMap {
   uint32_t cstride = 0;
 };
 
+// Does row y (virtual) hold a live tile in columns [xa, xb]? Without a map
+// every tile is live.
+__device__ inline bool live_range(const LiveMap &L, uint32_t y, uint32_t xa, uint32_t xb) {
+  if (!L.pre) return true;
+  const uint16_t *r = L.pre + (size_t)y * L.stride;
+  return r[xb + 1] != r[xa];
+}
+
+// Does a splat's rect (rows from y0 of the live map's virtual image) hold an
+// open tile: in its full rows (the summed-area table) or in its partial last
+// row (the row prefix)? The 4x4-block table first: no open tile in the
+// blocks that cover the rect (rem row included) means none in the rect.
+__device__ inline bool rect_live(const Rect &R, const LiveMap &live) {
+  if (R.nrows == 0) return false;
+  const uint32_t st = live.stride, cs = live.cstride;
+  const uint32_t cy0 = R.y0 >> 2, cy1 = (R.y0 + R.nrows + 3u) >> 2, cx0 = R.x0 >> 2, cx1 = (R.x1 + 4u) >> 2;
+  const bool maybe = live.csat[cy1 * cs + cx1] - live.csat[cy0 * cs + cx1] - live.csat[cy1 * cs + cx0] +
+                         live.csat[cy0 * cs + cx0] != 0u;
+  bool lv = false;
+  if (maybe && R.full_rows > 0) {
+    const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
+    const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
+                          live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
+    lv = open != 0;
+  }
+  if (maybe && !lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
+  return lv;
+}
+
 // Level-1 geometry of one view. A batch stacks the tile rows of its views
 // into one virtual image (rows [row0, row0 + rows) belong to this view);
 // items [item0, item0 + n) of the depth-ordered rect array are the view's,
@@ -106,18 +135,25 @@ struct LiveMap {
 struct L1View {
   uint32_t item0, n, blk0, row0, rows, c1_base;
   uint32_t tiles_x, max_cov;  // the view's tile grid is tiles_x x rows
+  // items level 1 reads (<= n; the layout of blocks and counts follows n):
+  // n, or the live splats that a batch pass compacted to the front
+  uint32_t n_live;
+  // a batch view's first item in the group's item arrays and first word of
+  // its live bits (compacted passes)
+  uint32_t vitem0, bit0;
 };
 
 // Packed rects of every view of a batch: rects[item0 + j] for the j-th
 // depth-ordered item (ids[item0 + j], an index of rec) of each view; blocks
-// of 256 items as the level-1 blocks. With live.pre (a pass after the
-// first; rows x cols is the stacked image) the open tiles' summed-area
-// table is built into sat ((rows + 1) x (cols + 1)) and csat
-// ((ceil(rows / 4) + 1) x (ceil(cols / 4) + 1)), and splats without an open
-// tile get empty rects.
+// of 256 items as the level-1 blocks.
 void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
-                 uint32_t blocks, uint4 *rects, const LiveMap &live, uint32_t rows, uint32_t cols, uint32_t *sat,
-                 uint32_t *csat);
+                 uint32_t blocks, uint4 *rects);
+
+// The open tiles' summed-area tables of a tall stacked image (rows x cols
+// tiles, row prefixes pre): sat ((rows + 1) x (cols + 1)) and csat
+// ((ceil(rows / 4) + 1) x (ceil(cols / 4) + 1)), one workgroup per column.
+LiveMap open_sat_tall(hipStream_t s, const uint16_t *pre, uint32_t rows, uint32_t cols, uint32_t *sat,
+                      uint32_t *csat);
 
 struct BinWorkspace {
   // level 1

@@ -62,14 +62,6 @@ __global__ __launch_bounds__(256) void k_rects(const float4 *__restrict__ rec, c
   rects[jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], g.tg, g.max_cov));
 }
 
-// Does row y (virtual) hold a live tile in columns [xa, xb]? Without a map
-// every tile is live.
-__device__ inline bool live_range(const LiveMap &L, uint32_t y, uint32_t xa, uint32_t xb) {
-  if (!L.pre) return true;
-  const uint16_t *r = L.pre + (size_t)y * L.stride;
-  return r[xb + 1] != r[xa];
-}
-
 // ------------------------ live-splat compaction ----------------------------
 // In a pass after the first, a splat enters level 1 only if some row of its
 // rect holds a live tile in the row's column span: exactly the splats that
@@ -143,27 +135,6 @@ __global__ __launch_bounds__(256) void k_open_csat(const uint32_t *__restrict__ 
 // sub-block, loads issued together): the per-item work is a few loads, so
 // one item per thread leaves them latency-bound.
 constexpr int kLiveItems = 8;
-
-// Does a splat's rect (rows from y0 of the live map's virtual image) hold an
-// open tile: in its full rows (the summed-area table) or in its partial last
-// row (the row prefix)? The 4x4-block table first: no open tile in the
-// blocks that cover the rect (rem row included) means none in the rect.
-__device__ inline bool rect_live(const Rect &R, const LiveMap &live) {
-  if (R.nrows == 0) return false;
-  const uint32_t st = live.stride, cs = live.cstride;
-  const uint32_t cy0 = R.y0 >> 2, cy1 = (R.y0 + R.nrows + 3u) >> 2, cx0 = R.x0 >> 2, cx1 = (R.x1 + 4u) >> 2;
-  const bool maybe = live.csat[cy1 * cs + cx1] - live.csat[cy0 * cs + cx1] - live.csat[cy1 * cs + cx0] +
-                         live.csat[cy0 * cs + cx0] != 0u;
-  bool lv = false;
-  if (maybe && R.full_rows > 0) {
-    const uint32_t ya = R.y0, yb = R.y0 + R.full_rows, xa = R.x0, xb = R.x1 + 1;
-    const uint32_t open = live.sat[(size_t)yb * st + xb] - live.sat[(size_t)ya * st + xb] -
-                          live.sat[(size_t)yb * st + xa] + live.sat[(size_t)ya * st + xa];
-    lv = open != 0;
-  }
-  if (maybe && !lv && R.rem > 0) lv = live_range(live, R.y0 + R.full_rows, R.x0, R.x0 + R.rem - 1u);
-  return lv;
-}
 
 // bit i of bits: splat i (id order, packed rects of the projection) is live
 // (rect_live)
@@ -286,23 +257,14 @@ __device__ inline L1View l1_view(const L1View &one, const L1View *__restrict__ t
   return tab[lo];
 }
 
-// With live.sat set (a pass after the first), a splat without an open tile
-// gets an empty rect: level 1 then reads its record and gives it no span,
-// which is what its per-row tests would find, without testing each row.
 __global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ rec, const uint32_t *__restrict__ ids,
-                                                     const L1View *__restrict__ tab, uint32_t ntab, LiveMap live,
+                                                     const L1View *__restrict__ tab, uint32_t ntab,
                                                      uint4 *__restrict__ rects) {
   const L1View v = l1_view(L1View{}, tab, ntab, blockIdx.x);
   const uint32_t jj = (blockIdx.x - v.blk0) * kB1 + threadIdx.x;
   if (jj >= v.n) return;
   const uint32_t id = ids[v.item0 + jj];
-  Rect R = splat_rect(rec[3 * (size_t)id], grid16(v.tiles_x, v.rows), v.max_cov);
-  if (live.sat) {
-    Rect Rv = R;
-    Rv.y0 += v.row0;  // the view's rows in the stacked image
-    if (!rect_live(Rv, live)) R = Rect{0, 0, 0, 0, 0, 0};
-  }
-  rects[v.item0 + jj] = pack_rect(id, R);
+  rects[v.item0 + jj] = pack_rect(id, splat_rect(rec[3 * (size_t)id], grid16(v.tiles_x, v.rows), v.max_cov));
 }
 
 // ---------------------------- level 1 -------------------------------------
@@ -314,7 +276,10 @@ __global__ __launch_bounds__(kB1) void k_batch_rects(const float4 *__restrict__ 
 __device__ inline L1View l1_view_dev(const L1View &one, const L1View *__restrict__ tab, uint32_t ntab, uint32_t b,
                                      const uint32_t *__restrict__ d_items) {
   L1View v = l1_view(one, tab, ntab, b);
-  if (d_items) v.n = min(*d_items, v.n);
+  if (d_items) {
+    v.n = min(*d_items, v.n);
+    v.n_live = v.n;
+  }
   return v;
 }
 
@@ -328,10 +293,15 @@ __global__ __launch_bounds__(kB1) void k_rows_count(const uint4 *__restrict__ re
   const uint32_t k_blk = blockIdx.x - v.blk0;
   const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
   if (k_blk >= nb1) return;
+  if (k_blk * kB1 >= v.n_live) {
+    // past the items read (a compacted pass): zero counts
+    for (uint32_t y = t; y < v.rows; y += kB1) c1[(size_t)v.c1_base + (size_t)y * nb1 + k_blk] = 0;
+    return;
+  }
   const uint32_t jj = k_blk * kB1 + t;
   for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
   __syncthreads();
-  if (jj < v.n) {
+  if (jj < v.n_live) {
     const Rect R = unpack_rect(rects[v.item0 + jj]);
     for (uint32_t k = 0; k < R.nrows; k++) {
       const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
@@ -361,7 +331,7 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ 
   const uint32_t t = threadIdx.x;
   const uint32_t k_blk = blockIdx.x - v.blk0;
   const uint32_t nb1 = (v.n + kB1 - 1) / kB1;
-  if (k_blk >= nb1) return;
+  if (k_blk >= nb1 || k_blk * kB1 >= v.n_live) return;
   const uint32_t jj = k_blk * kB1 + t;
   for (uint32_t i = t; i < v.rows * kW1; i += kB1) s_mask[i] = 0;
   for (uint32_t y = t; y < v.rows; y += kB1) s_base[y] = base1[(size_t)v.c1_base + (size_t)y * nb1 + k_blk];
@@ -369,7 +339,7 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ 
   Rect R{0, 0, 0, 0, 0, 0};
   uint32_t id = 0;
   uint32_t rows_live = 0;  // bit k: row R.y0 + k has a live tile in the span (nrows <= 32 rows tracked)
-  if (jj < v.n) {
+  if (jj < v.n_live) {
     const uint4 pr = rects[v.item0 + jj];
     id = pr.x;
     R = unpack_rect(pr);
@@ -391,7 +361,7 @@ __global__ __launch_bounds__(kB1) void k_rows_scatter(const uint4 *__restrict__ 
     }
   }
   __syncthreads();
-  if (jj < v.n) {
+  if (jj < v.n_live) {
     const uint32_t word = t >> 5, below = (1u << (t & 31)) - 1u;
     for (uint32_t k = 0; k < R.nrows; k++) {
       const uint32_t xb = (k < R.full_rows) ? R.x1 : (R.x0 + R.rem - 1u);
@@ -899,23 +869,20 @@ void span_bin_rects(hipStream_t s, const SpanBinArgs &a, BinWorkspace &ws, uint3
 }
 
 void batch_rects(hipStream_t s, const float4 *rec, const uint32_t *ids, const L1View *tab, uint32_t ntab,
-                 uint32_t blocks, uint4 *rects, const LiveMap &live, uint32_t rows, uint32_t cols, uint32_t *sat,
-                 uint32_t *csat) {
+                 uint32_t blocks, uint4 *rects) {
   if (!blocks) return;
-  LiveMap lm{};
-  if (live.pre) {
-    // the open tiles' summed-area table of the stacked image
-    lm = live;
-    lm.sat = sat;
-    lm.csat = csat;
-    lm.cstride = div_up(cols, 4) + 1;
-    k_open_sat_cols<<<cols + 1, 256, 0, s>>>(live.pre, rows, cols, sat);
-    DL_HIP(hipGetLastError());
-    k_open_csat<<<div_up((size_t)(div_up(rows, 4) + 1) * lm.cstride, 256), 256, 0, s>>>(sat, rows, cols, csat);
-    DL_HIP(hipGetLastError());
-  }
-  k_batch_rects<<<blocks, kB1, 0, s>>>(rec, ids, tab, ntab, lm, rects);
+  k_batch_rects<<<blocks, kB1, 0, s>>>(rec, ids, tab, ntab, rects);
   DL_HIP(hipGetLastError());
+}
+
+LiveMap open_sat_tall(hipStream_t s, const uint16_t *pre, uint32_t rows, uint32_t cols, uint32_t *sat,
+                      uint32_t *csat) {
+  LiveMap lm{pre, cols + 1, sat, csat, div_up(cols, 4) + 1};
+  k_open_sat_cols<<<cols + 1, 256, 0, s>>>(pre, rows, cols, sat);
+  DL_HIP(hipGetLastError());
+  k_open_csat<<<div_up((size_t)(div_up(rows, 4) + 1) * lm.cstride, 256), 256, 0, s>>>(sat, rows, cols, csat);
+  DL_HIP(hipGetLastError());
+  return lm;
 }
 
 void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn, const BinGrid &g, uint32_t npairs,
@@ -956,7 +923,7 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
   }
   SpanBinArgs a;
   a.rects = ws.rects.as<uint4>();
-  a.one = L1View{0, cn, 0, 0, g.tiles_y, 0, g.tiles_x, g.max_cov};
+  a.one = L1View{0, cn, 0, 0, g.tiles_y, 0, g.tiles_x, g.max_cov, cn, 0, 0};
   a.tab = nullptr;
   a.ntab = 0;
   a.l1_blocks = div_up(cn, kB1);
