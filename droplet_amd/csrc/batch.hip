@@ -433,7 +433,8 @@ void Batch::enqueue(bool eager) {
     BatchGroup &g = groups[gi];
     const BView *gv = d_views.as<BView>() + g.v0;
     if (g.cull_blocks) {
-      k_batch_cull<<<g.cull_blocks, kItemBlock, 0, s>>>(gv, g.nv, d_sc, keys.as<uint32_t>(), vals.as<uint32_t>(),
+      // the values are the item indices: the sort's first pass writes them
+      k_batch_cull<<<g.cull_blocks, kItemBlock, 0, s>>>(gv, g.nv, d_sc, keys.as<uint32_t>(), nullptr,
                                                         rec.as<float4>(), d_parts.as<unsigned long long>(),
                                                         g.blocks.size() > 1 ? rect8.as<uint2>() : nullptr, max_cov);
       DL_HIP(hipGetLastError());
@@ -452,7 +453,7 @@ void Batch::enqueue(bool eager) {
     const bool in_tmp = seg_radix_sort_pairs(s, keys.as<uint32_t>(), vals.as<uint32_t>(), tkeys.as<uint32_t>(),
                                              tvals.as<uint32_t>(), d_seg.as<uint32_t>() + g.seg_off, g.nv, g.items,
                                              32, opts.sort_digit_bits ? (int)std::min<uint32_t>(opts.sort_digit_bits, 8) : 8,
-                                             sort_ws.get());
+                                             sort_ws.get(), true);
     const uint32_t *sorted = in_tmp ? tvals.as<uint32_t>() : vals.as<uint32_t>();
     const uint32_t passes = (uint32_t)g.blocks.size();
     const size_t vtiles = (size_t)g.rows * g.cols;

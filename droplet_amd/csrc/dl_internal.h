@@ -186,11 +186,12 @@ struct SegGeom {
 // read on the device (d_seg_start, nseg + 1 entries, at most max_elems
 // elements in all): no host synchronisation, so the launches can be captured
 // in a graph. Result placement as radix_sort_pairs (it depends only on the
-// pass count).
+// pass count). With index_vals the input values are not read: value i is i
+// (vals is still scratch).
 size_t seg_radix_workspace_bytes(size_t max_elems, uint32_t nseg);
 bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
                           const uint32_t *d_seg_start, uint32_t nseg, size_t max_elems, int key_bits,
-                          int max_digit_bits, void *ws);
+                          int max_digit_bits, void *ws, bool index_vals = false);
 
 // The no-alloc sort (noalloc.hip): noalloc_setup looks up the device
 // address of its scratch array for `device` (current) once; the sort runs

@@ -710,8 +710,14 @@ size_t seg_radix_workspace_bytes(size_t max_elems, uint32_t nseg) {
 
 bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
                           const uint32_t *d_seg_start, uint32_t nseg, size_t max_elems, int key_bits,
-                          int max_digit_bits, void *ws) {
-  if (nseg == 0 || key_bits <= 0) return false;
+                          int max_digit_bits, void *ws, bool index_vals) {
+  if (nseg == 0 || key_bits <= 0) {
+    if (index_vals && max_elems > 0) {
+      k_iota_u32<<<div_up(max_elems, 256), 256, 0, s>>>(vals, max_elems);
+      DL_HIP(hipGetLastError());
+    }
+    return false;
+  }
   if (key_bits > 32) key_bits = 32;
   const uint32_t mb = div_up(max_elems, kRTile) + nseg + 1;
   uint8_t *w = static_cast<uint8_t *>(ws);
@@ -743,8 +749,9 @@ bool seg_radix_sort_pairs(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_
     k_radix_upsweep<<<mb, kRBlock, 0, s>>>(ki, 0, shift, bits[p], counts, mb, g, RadixKeyMap{});
     DL_HIP(hipGetLastError());
     scan_rec(s, counts, offsets, ndig * mb, nullptr, scan_ws);
-    downsweep_for(bits[p])<<<downsweep_grid(mb), kRBlock, 0, s>>>(ki, vi, ko, vo, 0, shift, offsets, nullptr, mb,
-                                                                  g, RadixKeyMap{}, downsweep_xq(mb));
+    downsweep_for(bits[p])<<<downsweep_grid(mb), kRBlock, 0, s>>>(ki, (p == 0 && index_vals) ? nullptr : vi, ko,
+                                                                  vo, 0, shift, offsets, nullptr, mb, g,
+                                                                  RadixKeyMap{}, downsweep_xq(mb));
     DL_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
