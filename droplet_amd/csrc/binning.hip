@@ -161,9 +161,18 @@ __device__ inline bool live_bit(const unsigned long long *__restrict__ bits, uin
   return (bits[id >> 6] >> (id & 63u)) & 1ull;
 }
 
+// Is splat id live: its bit, or without bits (a pass of few splats) the
+// test of its own rect
+__device__ inline bool splat_live(const unsigned long long *__restrict__ bits, const uint2 *__restrict__ rect8,
+                                  const LiveMap &lm, uint32_t id) {
+  return bits ? live_bit(bits, id) : rect_live(unpack_rect8(rect8[id]), lm);
+}
+
 // live splats per workgroup of kLiveItems 256-item sub-blocks of the pass
+// (from the live bits, or with bits null from each splat's rect)
 __global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__ ids, uint32_t cn,
                                                     const unsigned long long *__restrict__ bits,
+                                                    const uint2 *__restrict__ rect8, LiveMap lm,
                                                     uint32_t *__restrict__ blk_tot) {
   __shared__ uint32_t s_w[kLiveItems][4];
   const uint32_t sub0 = blockIdx.x * kLiveItems;
@@ -173,13 +182,23 @@ __global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__
     const uint32_t jj = (sub0 + k) * 256 + threadIdx.x;
     id[k] = jj < cn ? ids[jj] : 0xFFFFFFFFu;
   }
-  unsigned long long w[kLiveItems];
+  bool lv[kLiveItems];
+  if (bits) {
+    unsigned long long w[kLiveItems];
 #pragma unroll
-  for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
+    for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) lv[k] = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
+  } else {
+    uint2 r[kLiveItems];
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) r[k] = id[k] != 0xFFFFFFFFu ? rect8[id[k]] : make_uint2(0u, 0u);
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) lv[k] = id[k] != 0xFFFFFFFFu && rect_live(unpack_rect8(r[k]), lm);
+  }
 #pragma unroll
   for (int k = 0; k < kLiveItems; k++) {
-    const bool lv = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
-    const unsigned long long b = __ballot(lv);
+    const unsigned long long b = __ballot(lv[k]);
     if (__lane_id() == 0) s_w[k][threadIdx.x >> 6] = (uint32_t)__popcll(b);
   }
   __syncthreads();
@@ -197,7 +216,7 @@ __global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__
 // live_small[1] = their level-1 blocks
 __global__ __launch_bounds__(256) void k_rects_live(const uint2 *__restrict__ rect8, const uint32_t *__restrict__ ids,
                                                     uint32_t cn, const unsigned long long *__restrict__ bits,
-                                                    const uint32_t *__restrict__ blk_tot,
+                                                    LiveMap lm, const uint32_t *__restrict__ blk_tot,
                                                     uint32_t *__restrict__ live_small, uint4 *__restrict__ rects) {
   __shared__ uint32_t s_w[kLiveItems][4];
   __shared__ uint32_t s_pre[4];
@@ -209,18 +228,29 @@ __global__ __launch_bounds__(256) void k_rects_live(const uint2 *__restrict__ re
     const uint32_t jj = (sub0 + k) * 256 + threadIdx.x;
     id[k] = jj < cn ? ids[jj] : 0xFFFFFFFFu;
   }
-  unsigned long long w[kLiveItems];
-#pragma unroll
-  for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
   unsigned long long bal[kLiveItems];
   uint2 r[kLiveItems];
+  if (bits) {
+    unsigned long long w[kLiveItems];
 #pragma unroll
-  for (int k = 0; k < kLiveItems; k++) {
-    const bool lv = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
-    bal[k] = __ballot(lv);
-    r[k] = lv ? rect8[id[k]] : make_uint2(0u, 0u);
-    if (lane == 0) s_w[k][wave] = (uint32_t)__popcll(bal[k]);
+    for (int k = 0; k < kLiveItems; k++) w[k] = id[k] != 0xFFFFFFFFu ? bits[id[k] >> 6] : 0ull;
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) {
+      const bool lv = id[k] != 0xFFFFFFFFu && ((w[k] >> (id[k] & 63u)) & 1ull);
+      bal[k] = __ballot(lv);
+      r[k] = lv ? rect8[id[k]] : make_uint2(0u, 0u);
+    }
+  } else {
+    // a pass of few splats: each splat's own rect, tested here
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++) r[k] = id[k] != 0xFFFFFFFFu ? rect8[id[k]] : make_uint2(0u, 0u);
+#pragma unroll
+    for (int k = 0; k < kLiveItems; k++)
+      bal[k] = __ballot(id[k] != 0xFFFFFFFFu && rect_live(unpack_rect8(r[k]), lm));
   }
+#pragma unroll
+  for (int k = 0; k < kLiveItems; k++)
+    if (lane == 0) s_w[k][wave] = (uint32_t)__popcll(bal[k]);
   // live items of the workgroups before this one
   uint32_t pre = 0;
   for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) pre += blk_tot[b];
@@ -908,14 +938,20 @@ void span_bin(hipStream_t s, const float4 *rec, const uint32_t *ids, uint32_t cn
     lm.sat = ws.live_sat.as<uint32_t>();
     lm.csat = ws.live_csat.as<uint32_t>();
     lm.cstride = cstride;
-    k_live_bits<<<div_up(n_all, 256 * kLiveItems), 256, 0, s>>>(rect8, n_all, lm,
-                                                                ws.live_bits.as<unsigned long long>());
-    DL_HIP(hipGetLastError());
+    // live bits of every splat (a sequential read of the rects) when the
+    // pass holds more than an eighth of them; a smaller pass tests its own
+    // splats' rects as it compacts them (two gathers of cn rects)
+    unsigned long long *bits = nullptr;
+    if ((uint64_t)cn * 8 > n_all) {
+      bits = ws.live_bits.as<unsigned long long>();
+      k_live_bits<<<div_up(n_all, 256 * kLiveItems), 256, 0, s>>>(rect8, n_all, lm, bits);
+      DL_HIP(hipGetLastError());
+    }
     const uint32_t grid = div_up(nblk, kLiveItems);
-    k_live_count<<<grid, 256, 0, s>>>(ids, cn, ws.live_bits.as<unsigned long long>(), ws.live_cnt.as<uint32_t>());
+    k_live_count<<<grid, 256, 0, s>>>(ids, cn, bits, rect8, lm, ws.live_cnt.as<uint32_t>());
     DL_HIP(hipGetLastError());
-    k_rects_live<<<grid, 256, 0, s>>>(rect8, ids, cn, ws.live_bits.as<unsigned long long>(),
-                                      ws.live_cnt.as<uint32_t>(), ls, ws.rects.as<uint4>());
+    k_rects_live<<<grid, 256, 0, s>>>(rect8, ids, cn, bits, lm, ws.live_cnt.as<uint32_t>(), ls,
+                                      ws.rects.as<uint4>());
     DL_HIP(hipGetLastError());
   } else if (cn && !rects_ready) {
     k_rects<<<nblk, 256, 0, s>>>(rec, ids, cn, g, ws.rects.as<uint4>());

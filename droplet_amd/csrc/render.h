@@ -168,7 +168,7 @@ struct Renderer {
   // most cut_pairs() pairs each, 2^30 (about 20 GB of lists and spans per
   // pass), as the reference renders any total in 200000-splat chunks. On
   // the 6M-splat 4096x2048 equirect frame 2^30 measured fastest: 21 passes
-  // in 5.6 ms, against 6.4 ms at 2^29 (42 passes of fixed launches) and
+  // in 5.56 ms, against 6.4 ms at 2^29 (42 passes of fixed launches) and
   // 6.7 ms at 2^31 (11 passes binning twice the list entries).
   // pair_budget != 0 sets both, to test the cut at small sizes.
   uint64_t pair_budget = 0;
